@@ -1,0 +1,192 @@
+/*
+ * qtn.h -- C ABI of libqtn, the gfx950 (MI355X) contraction & truncation
+ * library behind the Qaintensor.jl tensor-network API.
+ *
+ * Conventions shared by every entry point
+ *   - Complex numbers are interleaved (re, im) doubles, bit-compatible with
+ *     Julia ComplexF64, C `double _Complex` and hipDoubleComplex.  They are
+ *     passed as `double*` pointing at 2*count doubles.
+ *   - Arrays are column-major (Julia order): dims are given in Julia order,
+ *     the first index varies fastest.
+ *   - Sizes and dims are int64_t (Julia Int).  Leg numbers are 1-based.
+ *   - Every function returns int32_t status: QTN_OK (0) or a negative code;
+ *     qtn_last_error() then returns a message (thread-local).  Nothing throws
+ *     or aborts across the ABI.
+ *   - Host pointers are only read during the call; the library keeps none.
+ *     Entry points whose name ends in _dev take device pointers instead.
+ *   - Calls are synchronous with respect to the host unless stated.
+ *
+ * Reference interfaces replaced (paths relative to Qaintensor.jl):
+ *   qtn_contract_pair     <- TensorOperations.ncon pairwise step reached from
+ *                            contract(net), src/contract.jl:242-264
+ *   qtn_svd_trunc         <- LinearAlgebra.svd + truncation rule of
+ *                            contract_svd, src/svd.jl:26-33
+ *   qtn_mps_*             <- MPS container / switch! two-site split,
+ *                            src/mps.jl:7-89, src/switch.jl:18-56
+ *   qtn_mpo_* / zip-up    <- apply_MPO wiring + contract(),
+ *                            src/mpo.jl:184-252
+ */
+#ifndef QTN_H
+#define QTN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define QTN_OK              0
+#define QTN_ERR_ARG        -1  /* invalid argument (dims, legs, cutoff, ...) */
+#define QTN_ERR_HIP        -2  /* HIP runtime error                          */
+#define QTN_ERR_OOM        -3  /* device allocation failed                   */
+#define QTN_ERR_NOCONV     -4  /* SVD sweep cap reached before convergence   */
+#define QTN_ERR_TRUNC      -5  /* every singular value is within the cutoff  */
+#define QTN_ERR_NOTFINITE  -6  /* NaN/Inf met in the input of an SVD         */
+
+/* precision forms of the two-site update */
+#define QTN_FORM_PLAIN      0  /* ComplexF64 end to end                       */
+#define QTN_FORM_CHOSEN     1  /* complex FP32 bulk, FP64 core (see docs)     */
+
+typedef struct qtn_ctx qtn_ctx;
+typedef struct qtn_mps qtn_mps;
+
+/* ---- context ----------------------------------------------------------- */
+/* Opens device `device`, creates one HIP stream and a workspace pool.      */
+int32_t qtn_ctx_create(int32_t device, qtn_ctx** ctx);
+int32_t qtn_ctx_destroy(qtn_ctx* ctx);
+/* Blocks until all work queued on the context's stream is done.            */
+int32_t qtn_ctx_sync(qtn_ctx* ctx);
+/* Returns the context's hipStream_t (as void*) so callers can time with
+ * events on the stream the kernels run on.                                 */
+int32_t qtn_ctx_stream(qtn_ctx* ctx, void** stream);
+/* Message of the last failing call on this thread ("" if none).            */
+const char* qtn_last_error(void);
+/* Library version string. */
+const char* qtn_version(void);
+
+/* ---- pairwise contraction (the ncon step) ------------------------------ */
+/* C = contraction of A and B over nc leg pairs (legsA[i] of A with
+ * legsB[i] of B, 1-based).  The free legs of A (in order) followed by the
+ * free legs of B (in order) form the natural output; `outperm` (1-based,
+ * length ndA+ndB-2nc) lists, for each output axis, which natural axis it
+ * is, exactly as permutedims(natural, outperm).  outperm may be NULL for
+ * the identity.  Complex FP64.  Replaces one pairwise step of
+ * TensorOperations.ncon as used by contract(), src/contract.jl:257,263.    */
+int32_t qtn_contract_pair(qtn_ctx* ctx,
+                          const double* A, int32_t ndA, const int64_t* dimsA,
+                          const double* B, int32_t ndB, const int64_t* dimsB,
+                          int32_t nc, const int32_t* legsA,
+                          const int32_t* legsB, const int32_t* outperm,
+                          double* C);
+/* Same with device pointers; enqueued on the context stream, no sync.      */
+int32_t qtn_contract_pair_dev(qtn_ctx* ctx,
+                              const double* A, int32_t ndA,
+                              const int64_t* dimsA, const double* B,
+                              int32_t ndB, const int64_t* dimsB, int32_t nc,
+                              const int32_t* legsA, const int32_t* legsB,
+                              const int32_t* outperm, double* C);
+
+/* Axis permutation of a column-major complex tensor:
+ * out = permutedims(in, perm) (perm 1-based).  Replaces permutedims at
+ * src/svd.jl:20-21, src/switch.jl:28-37, src/contract.jl:244.             */
+int32_t qtn_permute(qtn_ctx* ctx, const double* in, int32_t nd,
+                    const int64_t* dims, const int32_t* perm, double* out);
+int32_t qtn_permute_dev(qtn_ctx* ctx, const double* in, int32_t nd,
+                        const int64_t* dims, const int32_t* perm,
+                        double* out);
+
+/* Complex FP64 GEMM on device pointers, column-major:
+ * C = alpha*op(A)*op(B) + beta*C, op = 'N', 'T' or 'C'.  Batched with
+ * element strides (0 = shared operand).                                    */
+int32_t qtn_zgemm_dev(qtn_ctx* ctx, char opA, char opB, int64_t m,
+                      int64_t n, int64_t k, const double* alpha,
+                      const double* A, int64_t lda, int64_t strideA,
+                      const double* B, int64_t ldb, int64_t strideB,
+                      const double* beta, double* C, int64_t ldc,
+                      int64_t strideC, int64_t batch);
+/* Complex FP32 variant (interleaved float pairs), FP32 accumulation.       */
+int32_t qtn_cgemm_dev(qtn_ctx* ctx, char opA, char opB, int64_t m,
+                      int64_t n, int64_t k, const float* alpha,
+                      const float* A, int64_t lda, int64_t strideA,
+                      const float* B, int64_t ldb, int64_t strideB,
+                      const float* beta, float* C, int64_t ldc,
+                      int64_t strideC, int64_t batch);
+
+/* ---- truncated SVD ----------------------------------------------------- */
+/* A (m x n, leading dim lda) = U S V^H.  Truncation follows
+ * src/svd.jl:29-33: keep the fewest leading singular values whose
+ * discarded tail has 2-norm <= er, where er = cutoff*||s||_2 when
+ * `relative` is nonzero and er = cutoff otherwise; then k = min(k, chimax)
+ * (chimax <= 0: no cap).  Outputs: *k, U (m x k, ld m), S (k), Vh (k x n,
+ * ld k), *disc_weight = sum of squares of the discarded values.  Any of
+ * U/S/Vh may be NULL.  Capacity: U, Vh and S must hold min(m,n) columns /
+ * rows / values.  Returns QTN_ERR_TRUNC when every value is within er
+ * (the reference errors there), QTN_ERR_ARG for cutoff < 0.                */
+int32_t qtn_svd_trunc(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
+                      int64_t lda, double cutoff, int32_t relative,
+                      int64_t chimax, int64_t* k, double* U, double* S,
+                      double* Vh, double* disc_weight);
+/* Full singular value list (min(m,n) values, descending) of A; no vectors. */
+int32_t qtn_svd_values(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
+                       int64_t lda, double* S);
+
+/* ---- matrix-product states --------------------------------------------- */
+/* An MPS of `nsites` qubits lives on the device in right-canonical form
+ * (Vidal-free TEBD with the bond spectra kept beside the site tensors).
+ * Site j tensor is (chi_{j-1}, 2, chi_j), column-major; chi_0 = chi_n = 1.
+ * Created as |0...0>.  `chimax_cap` sizes the per-site storage.            */
+int32_t qtn_mps_create(qtn_ctx* ctx, int32_t nsites, int64_t chimax_cap,
+                       qtn_mps** mps);
+int32_t qtn_mps_destroy(qtn_mps* mps);
+/* Copy state a into b (same nsites and capacity).                          */
+int32_t qtn_mps_copy(const qtn_mps* a, qtn_mps* b);
+/* Replace the state by host site tensors (dims[3*j..3*j+2] for site j,
+ * left-canonical not required); the library brings it to right-canonical
+ * form with bond spectra (exact, no truncation unless chimax_cap binds).   */
+int32_t qtn_mps_set_sites(qtn_mps* mps, const int64_t* dims,
+                          const double* const* sites);
+/* Bond dimensions chi_0..chi_n (n+1 values).                               */
+int32_t qtn_mps_bond_dims(const qtn_mps* mps, int64_t* dims);
+/* Bond spectrum lambda_bond (bond 1..n-1, between site bond and bond+1).  */
+int32_t qtn_mps_bond_spectrum(const qtn_mps* mps, int32_t bond, double* S,
+                              int64_t* k);
+/* Download site tensor (1-based) to host; dims[3] receives its shape.      */
+int32_t qtn_mps_get_site(const qtn_mps* mps, int32_t site, double* out,
+                         int64_t* dims);
+
+/* Single-qubit gate u (2x2, column-major) on `site` (1-based).             */
+int32_t qtn_mps_apply_1q(qtn_mps* mps, int32_t site, const double* u);
+/* Two-qubit gate on sites (site, site+1): g is 4x4 column-major, row/col
+ * index = s_site + 2*s_{site+1} (wire w <-> bit w-1, as apply_MPO wires
+ * gate bit k-1 to iwire[k], src/mpo.jl:199-214).  Truncation as
+ * qtn_svd_trunc with relative cutoff and chimax.  Outputs (may be NULL):
+ * kept rank, discarded weight (of the normalised spectrum), and the full
+ * sorted spectrum before truncation (capacity 2*chimax_cap).              */
+int32_t qtn_mps_apply_2q(qtn_mps* mps, int32_t site, const double* g,
+                         double cutoff, int64_t chimax, int32_t form,
+                         int64_t* kept, double* disc_weight,
+                         double* spectrum, int64_t* spectrum_len);
+/* A layer of gates on disjoint neighbouring pairs, batched in one pass:
+ * sites[i], gates[16*i..] as in apply_2q.  kept/disc per gate.            */
+int32_t qtn_mps_apply_layer(qtn_mps* mps, int32_t ngates,
+                            const int32_t* sites, const double* gates,
+                            double cutoff, int64_t chimax, int32_t form,
+                            int64_t* kept, double* disc_weight);
+
+/* Amplitudes <b|psi> for nb bitstrings; bits is nb x nsites bytes (0/1),
+ * row-major per bitstring (bits[i*nsites + w-1] is wire w).               */
+int32_t qtn_mps_amplitudes(const qtn_mps* mps, int64_t nb,
+                           const uint8_t* bits, double* out);
+/* <Z_w> for every wire (n doubles), normalised by <psi|psi>.              */
+int32_t qtn_mps_expect_z(const qtn_mps* mps, double* out);
+/* <a|b> (complex, 2 doubles).                                               */
+int32_t qtn_mps_overlap(const qtn_mps* a, const qtn_mps* b, double* out);
+/* <psi|psi> (real).                                                        */
+int32_t qtn_mps_norm2(const qtn_mps* mps, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QTN_H */

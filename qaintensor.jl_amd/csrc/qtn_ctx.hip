@@ -1,0 +1,122 @@
+// Context lifetime, error text and device scratch management for libqtn.
+#include <cstring>
+#include <new>
+
+#include "qtn_internal.hpp"
+
+namespace qtn {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+void clear_error() { g_last_error.clear(); }
+
+}  // namespace qtn
+
+int32_t qtn_ctx::buffer(int slot, size_t bytes, void** out) {
+  if (slot < 0) {
+    qtn::set_error("internal: negative buffer slot");
+    return QTN_ERR_ARG;
+  }
+  if ((size_t)slot >= bufs.size()) bufs.resize(slot + 1);
+  qtn::DevBuf& b = bufs[slot];
+  if (b.bytes < bytes) {
+    if (b.ptr) {
+      // Pending kernels may still read the old buffer.
+      QTN_CHECK_HIP(hipStreamSynchronize(stream));
+      QTN_CHECK_HIP(hipFree(b.ptr));
+      b.ptr = nullptr;
+      b.bytes = 0;
+    }
+    size_t want = bytes < 256 ? 256 : bytes;
+    hipError_t e = hipMalloc(&b.ptr, want);
+    if (e != hipSuccess) {
+      b.ptr = nullptr;
+      qtn::set_error(std::string("hipMalloc of ") + std::to_string(want) +
+                     " bytes failed: " + hipGetErrorString(e));
+      return QTN_ERR_OOM;
+    }
+    b.bytes = want;
+  }
+  *out = b.ptr;
+  return QTN_OK;
+}
+
+int32_t qtn_ctx::pinned_buffer(size_t bytes, void** out) {
+  if (pinned_bytes < bytes) {
+    if (pinned) {
+      QTN_CHECK_HIP(hipStreamSynchronize(stream));
+      QTN_CHECK_HIP(hipHostFree(pinned));
+      pinned = nullptr;
+      pinned_bytes = 0;
+    }
+    size_t want = bytes < 4096 ? 4096 : bytes;
+    QTN_CHECK_HIP(hipHostMalloc(&pinned, want, hipHostMallocDefault));
+    pinned_bytes = want;
+  }
+  *out = pinned;
+  return QTN_OK;
+}
+
+extern "C" {
+
+const char* qtn_last_error(void) { return qtn::g_last_error.c_str(); }
+
+const char* qtn_version(void) { return "libqtn 0.1.0 (gfx950)"; }
+
+int32_t qtn_ctx_create(int32_t device, qtn_ctx** ctx) {
+  qtn::clear_error();
+  QTN_ARG(ctx != nullptr, "qtn_ctx_create: ctx is NULL");
+  *ctx = nullptr;
+  int ndev = 0;
+  QTN_CHECK_HIP(hipGetDeviceCount(&ndev));
+  QTN_ARG(device >= 0 && device < ndev,
+          "qtn_ctx_create: device " + std::to_string(device) +
+              " out of range (" + std::to_string(ndev) + " devices)");
+  QTN_CHECK_HIP(hipSetDevice(device));
+  qtn_ctx* c = new (std::nothrow) qtn_ctx();
+  if (!c) {
+    qtn::set_error("qtn_ctx_create: host allocation failed");
+    return QTN_ERR_OOM;
+  }
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    qtn::set_error(std::string("hipStreamCreate failed: ") +
+                   hipGetErrorString(e));
+    return QTN_ERR_HIP;
+  }
+  *ctx = c;
+  return QTN_OK;
+}
+
+int32_t qtn_ctx_destroy(qtn_ctx* ctx) {
+  qtn::clear_error();
+  if (!ctx) return QTN_OK;
+  QTN_CHECK_HIP(hipSetDevice(ctx->device));
+  QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  for (auto& b : ctx->bufs)
+    if (b.ptr) QTN_CHECK_HIP(hipFree(b.ptr));
+  if (ctx->pinned) QTN_CHECK_HIP(hipHostFree(ctx->pinned));
+  QTN_CHECK_HIP(hipStreamDestroy(ctx->stream));
+  delete ctx;
+  return QTN_OK;
+}
+
+int32_t qtn_ctx_sync(qtn_ctx* ctx) {
+  qtn::clear_error();
+  QTN_ARG(ctx != nullptr, "qtn_ctx_sync: ctx is NULL");
+  QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return QTN_OK;
+}
+
+int32_t qtn_ctx_stream(qtn_ctx* ctx, void** stream) {
+  qtn::clear_error();
+  QTN_ARG(ctx != nullptr && stream != nullptr,
+          "qtn_ctx_stream: NULL argument");
+  *stream = (void*)ctx->stream;
+  return QTN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,105 @@
+// Internal declarations shared by the libqtn translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "qtn.h"
+
+namespace qtn {
+
+void set_error(const std::string& msg);
+void clear_error();
+
+#define QTN_CHECK_HIP(call)                                                  \
+  do {                                                                       \
+    hipError_t qtn_e_ = (call);                                              \
+    if (qtn_e_ != hipSuccess) {                                              \
+      ::qtn::set_error(std::string(#call) + " failed: " +                    \
+                       hipGetErrorString(qtn_e_));                           \
+      return QTN_ERR_HIP;                                                    \
+    }                                                                        \
+  } while (0)
+
+#define QTN_TRY(expr)                                                        \
+  do {                                                                       \
+    int32_t qtn_s_ = (expr);                                                 \
+    if (qtn_s_ != QTN_OK) return qtn_s_;                                     \
+  } while (0)
+
+#define QTN_ARG(cond, msg)                                                   \
+  do {                                                                       \
+    if (!(cond)) {                                                           \
+      ::qtn::set_error(msg);                                                 \
+      return QTN_ERR_ARG;                                                    \
+    }                                                                        \
+  } while (0)
+
+// A growable device buffer owned by the context.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+// Complex GEMM launch (device pointers, element counts; complex elements).
+// op: 0 = N, 1 = T, 2 = C.  Returns QTN_OK or an error status.
+int32_t launch_zgemm(hipStream_t s, int opA, int opB, int64_t m, int64_t n,
+                     int64_t k, double2 alpha, const double2* A, int64_t lda,
+                     int64_t sA, const double2* B, int64_t ldb, int64_t sB,
+                     double2 beta, double2* C, int64_t ldc, int64_t sC,
+                     int64_t batch);
+int32_t launch_cgemm(hipStream_t s, int opA, int opB, int64_t m, int64_t n,
+                     int64_t k, float2 alpha, const float2* A, int64_t lda,
+                     int64_t sA, const float2* B, int64_t ldb, int64_t sB,
+                     float2 beta, float2* C, int64_t ldc, int64_t sC,
+                     int64_t batch);
+// Per-problem descriptor of a grouped (uneven batched) complex GEMM.
+template <typename R>
+struct CType;
+template <>
+struct CType<double> {
+  typedef double2 type;
+};
+template <>
+struct CType<float> {
+  typedef float2 type;
+};
+template <typename R>
+struct GemmDesc {
+  int64_t M, N, K;
+  typename CType<R>::type alpha, beta;
+  const typename CType<R>::type* A;
+  int64_t lda;
+  const typename CType<R>::type* B;
+  int64_t ldb;
+  typename CType<R>::type* C;
+  int64_t ldc;
+};
+// d_desc: device array of `count` descriptors; maxM / maxN bound every
+// problem's M / N (they size the grid).
+int32_t launch_zgemm_grouped(hipStream_t s, int opA, int opB,
+                             const GemmDesc<double>* d_desc, int count,
+                             int64_t maxM, int64_t maxN);
+int32_t launch_cgemm_grouped(hipStream_t s, int opA, int opB,
+                             const GemmDesc<float>* d_desc, int count,
+                             int64_t maxM, int64_t maxN);
+int32_t launch_permute(hipStream_t s, const double2* in, int nd,
+                       const int64_t* dims, const int* perm0, double2* out);
+
+}  // namespace qtn
+
+struct qtn_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // Named scratch buffers grown on demand (never shrunk until destroy).
+  std::vector<qtn::DevBuf> bufs;
+  // Small pinned host staging buffer for scalar read-backs.
+  void* pinned = nullptr;
+  size_t pinned_bytes = 0;
+  // Returns a device buffer of at least `bytes` for slot `slot`.
+  int32_t buffer(int slot, size_t bytes, void** out);
+  int32_t pinned_buffer(size_t bytes, void** out);
+};

@@ -1,0 +1,727 @@
+// Device-resident matrix-product states and the two-site truncated update.
+//
+// Gauge: right-canonical site tensors B_j (chi_{j-1}, 2, chi_j) with the bond
+// spectra lambda_j kept beside them (the "Vidal-free" TEBD form of Hastings,
+// J. Math. Phys. 50, 095207 (2009)).  A gate on sites (i, i+1) is
+//   Phi   = g . (B_i B_{i+1})                    rows (a,s'), cols (t',b)
+//   theta = diag(lambda_{i-1} (x) 1_2) Phi
+//   theta = U S V^H, k = truncation rule (src/svd.jl:29-33, relative er)
+//   B_{i+1} <- V_k^H,  B_i <- Phi V_k / |S_k|,  lambda_i <- S_k / |S_k|
+// so no division by small Schmidt values is ever needed and the gates of a
+// brick-wall layer are independent (one batched pass per layer).  The SVD is
+// taken of X = theta^H: its orthogonalised columns are V_k S_k directly.
+// Site-index conventions follow switch! (src/switch.jl:26-55): theta rows are
+// (a, s) with a fastest, columns (t, b) with t fastest, column-major.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "qtn_internal.hpp"
+#include "qtn_svd.hpp"
+
+struct qtn_mps {
+  qtn_ctx* ctx = nullptr;
+  int n = 0;
+  int64_t cap = 0;
+  std::vector<int64_t> dims;     // chi_0..chi_n
+  std::vector<int64_t> maxb;     // capacity of every bond
+  std::vector<double2*> site;    // device site tensors (0-based site index)
+  std::vector<double*> lam;      // device bond spectra, lam[j] for bond j
+  void* arena = nullptr;         // one allocation for all of the above
+};
+
+namespace qtn {
+namespace {
+
+constexpr int kSlotPhi = 30, kSlotXm = 31, kSlotVk = 32, kSlotDesc = 33,
+              kSlotGates = 34, kSlotEnv = 35, kSlotT1 = 36, kSlotT2 = 37,
+              kSlotRed = 38, kSlotBits = 39, kSlotAmp = 40;
+
+unsigned grid_for(int64_t n) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256,
+                                                          8192));
+}
+
+int64_t pow2_cap(int e, int64_t cap) {
+  if (e >= 62) return cap;
+  return std::min<int64_t>(cap, (int64_t)1 << e);
+}
+
+// Per-gate data of a batched two-site update.
+struct GateDesc {
+  double2* Bl;        // site i tensor
+  double2* Br;        // site i+1 tensor
+  const double* lamL; // lambda_{i-1}
+  double* lamM;       // lambda_i (written)
+  double2* phi;       // Phi (M x N)
+  double2* X;         // X = theta^H (rows_p x cols_p)
+  double2* Vk;        // V_k (N x k)
+  const double2* g;   // 4x4 gate
+  int64_t chiL, chiM, chiR;
+};
+
+// Phi <- g . Phi in place and X = conj(lambda_a Phi)^T; one thread per
+// (a, b) handles the four (s, t) entries.  grid (tiles, batch).
+__global__ void gate_epilogue(const GateDesc* __restrict__ gd,
+                              int64_t rows_p) {
+  const GateDesc d = gd[blockIdx.y];
+  const int64_t chiL = d.chiL, chiR = d.chiR;
+  const int64_t M = 2 * chiL;
+  const int64_t total = chiL * chiR;
+  double2 g[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) g[i] = d.g[i];
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+       idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t a = idx % chiL, b = idx / chiL;
+    double2 in[4], out[4];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+        in[s + 2 * t] = d.phi[(a + chiL * s) + M * (t + 2 * b)];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double re = 0.0, im = 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const double2 gv = g[r + 4 * c];
+        re += gv.x * in[c].x - gv.y * in[c].y;
+        im += gv.x * in[c].y + gv.y * in[c].x;
+      }
+      out[r] = make_double2(re, im);
+    }
+    const double la = d.lamL[a];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const double2 v = out[s + 2 * t];
+        const int64_t row = a + chiL * s, col = t + 2 * b;
+        d.phi[row + M * col] = v;
+        d.X[col + rows_p * row] = make_double2(la * v.x, -la * v.y);
+      }
+  }
+}
+
+// V_k[:, j] = X[:, perm[j]] / sig[j] (N rows) and
+// B_{i+1}[j, c] = conj(V_k[c, j]) (k x N, ld k); lambda_i = sig / |S_k|.
+__global__ void split_right(const GateDesc* __restrict__ gd,
+                            const double* __restrict__ sig,
+                            const int* __restrict__ perm,
+                            const int64_t* __restrict__ kept,
+                            const double* __restrict__ keptw, int64_t rows_p,
+                            int64_t cols_p) {
+  const int bz = blockIdx.y;
+  const GateDesc d = gd[bz];
+  const int64_t k = kept[bz];
+  const int64_t N = 2 * d.chiR;
+  const double* s = sig + (int64_t)bz * cols_p;
+  const int* pm = perm + (int64_t)bz * cols_p;
+  const int64_t total = N * k;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+       idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = idx % N, j = idx / N;
+    const double sj = s[j];
+    const double inv = sj > 0.0 ? 1.0 / sj : 0.0;
+    const double2 x = d.X[c + rows_p * (int64_t)pm[j]];
+    const double2 v = make_double2(x.x * inv, x.y * inv);
+    d.Vk[c + N * j] = v;
+    d.Br[j + k * c] = make_double2(v.x, -v.y);
+  }
+  if (blockIdx.x == 0) {
+    const double nrm = sqrt(keptw[bz]);
+    for (int64_t j = threadIdx.x; j < k; j += blockDim.x)
+      d.lamM[j] = s[j] / nrm;
+  }
+}
+
+// Single-qubit gate on a site tensor in place: B[a, s', b] = u[s', s] B[a,s,b]
+__global__ void apply_1q_kernel(double2* __restrict__ B, int64_t chiL,
+                                int64_t chiR, const double2* __restrict__ u) {
+  const double2 u00 = u[0], u10 = u[1], u01 = u[2], u11 = u[3];
+  const int64_t total = chiL * chiR;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+       idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t a = idx % chiL, b = idx / chiL;
+    double2* p0 = B + a + chiL * (0 + 2 * b);
+    double2* p1 = B + a + chiL * (1 + 2 * b);
+    const double2 x0 = *p0, x1 = *p1;
+    *p0 = make_double2(u00.x * x0.x - u00.y * x0.y + u01.x * x1.x - u01.y * x1.y,
+                       u00.x * x0.y + u00.y * x0.x + u01.x * x1.y + u01.y * x1.x);
+    *p1 = make_double2(u10.x * x0.x - u10.y * x0.y + u11.x * x1.x - u11.y * x1.y,
+                       u10.x * x0.y + u10.y * x0.x + u11.x * x1.y + u11.y * x1.x);
+  }
+}
+
+// Amplitude contraction step: V'[i][c] = sum_a V[i][a] B[a, s_i, c].
+// V is nb x chiL (ld nb), V' is nb x chiR.
+__global__ void amp_step(const double2* __restrict__ V, int64_t nb,
+                         int64_t chiL, int64_t chiR,
+                         const double2* __restrict__ B,
+                         const uint8_t* __restrict__ bits, int nsites,
+                         int site0, double2* __restrict__ Vn) {
+  const int64_t total = nb * chiR;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+       idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = idx % nb, c = idx / nb;
+    const int s = bits[i * nsites + site0] ? 1 : 0;
+    double re = 0.0, im = 0.0;
+    const double2* col = B + chiL * (s + 2 * c);
+    for (int64_t a = 0; a < chiL; ++a) {
+      const double2 v = V[i + nb * a];
+      const double2 b = col[a];
+      re += v.x * b.x - v.y * b.y;
+      im += v.x * b.y + v.y * b.x;
+    }
+    Vn[idx] = make_double2(re, im);
+  }
+}
+
+// sum over (a, s, b) of w_s conj(B[(a,s),b]) Y[(a,s),b] for w = (1,1) and
+// w = (1,-1): out[0..1] = norm (complex), out[2..3] = Z moment (complex).
+__global__ __launch_bounds__(256) void env_reduce(
+    const double2* __restrict__ B, const double2* __restrict__ Y,
+    int64_t chiL, int64_t chiR, double* __restrict__ out) {
+  const int64_t total = 2 * chiL * chiR;
+  double nr = 0, ni = 0, zr = 0, zi = 0;
+  for (int64_t idx = threadIdx.x; idx < total; idx += 256) {
+    const int64_t row = idx % (2 * chiL);
+    const int s = row >= chiL ? 1 : 0;
+    const double2 b = B[idx], y = Y[idx];
+    const double pr = b.x * y.x + b.y * y.y;  // conj(b) * y
+    const double pi = b.x * y.y - b.y * y.x;
+    nr += pr;
+    ni += pi;
+    zr += s ? -pr : pr;
+    zi += s ? -pi : pi;
+  }
+  __shared__ double red[4][256];
+  red[0][threadIdx.x] = nr;
+  red[1][threadIdx.x] = ni;
+  red[2][threadIdx.x] = zr;
+  red[3][threadIdx.x] = zi;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void fill_one(double2* p) { *p = make_double2(1.0, 0.0); }
+__global__ void fill_lambda_one(double* p) { *p = 1.0; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Batched two-site update of disjoint gates (host orchestration).
+int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
+                        const double* gates, double cutoff, int64_t chimax,
+                        int64_t* kept_out, double* disc_out, double* spec_out,
+                        int64_t* spec_len) {
+  qtn_ctx* ctx = m->ctx;
+  hipStream_t s = ctx->stream;
+  std::vector<char> used(m->n + 1, 0);
+  int64_t maxM = 0, maxN = 0, maxK = 0;
+  for (int g = 0; g < ng; ++g) {
+    const int i = sites[g];
+    QTN_ARG(i >= 1 && i < m->n, "apply_2q: site out of range");
+    QTN_ARG(!used[i] && !used[i + 1], "apply_layer: gates overlap");
+    used[i] = used[i + 1] = 1;
+    maxM = std::max(maxM, 2 * m->dims[i - 1]);
+    maxN = std::max(maxN, 2 * m->dims[i + 1]);
+    maxK = std::max(maxK, m->dims[i]);
+  }
+  const int64_t rows_p = round_up64(maxN), cols_p = round_up64(maxM);
+  const int64_t phi_sz = maxM * maxN, x_sz = rows_p * cols_p;
+  void *dphi, *dX, *dVk, *ddesc, *dg;
+  QTN_TRY(ctx->buffer(kSlotPhi, (size_t)ng * phi_sz * sizeof(double2), &dphi));
+  QTN_TRY(ctx->buffer(kSlotXm, (size_t)ng * x_sz * sizeof(double2), &dX));
+  QTN_TRY(ctx->buffer(kSlotVk, (size_t)ng * maxN * std::max(maxM, maxN) *
+                                   sizeof(double2),
+                      &dVk));
+  const size_t desc_bytes = (size_t)ng * (sizeof(GateDesc) +
+                                          2 * sizeof(GemmDesc<double>));
+  QTN_TRY(ctx->buffer(kSlotDesc, desc_bytes, &ddesc));
+  QTN_TRY(ctx->buffer(kSlotGates, (size_t)ng * 16 * sizeof(double2), &dg));
+  std::vector<GateDesc> gd(ng);
+  std::vector<GemmDesc<double>> g1(ng);
+  std::vector<int64_t> nsv(ng);
+  for (int g = 0; g < ng; ++g) {
+    const int i = sites[g];
+    GateDesc& d = gd[g];
+    d.chiL = m->dims[i - 1];
+    d.chiM = m->dims[i];
+    d.chiR = m->dims[i + 1];
+    d.Bl = m->site[i - 1];
+    d.Br = m->site[i];
+    d.lamL = m->lam[i - 1];
+    d.lamM = m->lam[i];
+    d.phi = (double2*)dphi + (size_t)g * phi_sz;
+    d.X = (double2*)dX + (size_t)g * x_sz;
+    d.Vk = (double2*)dVk + (size_t)g * maxN * std::max(maxM, maxN);
+    d.g = (const double2*)dg + 16 * g;
+    const int64_t M = 2 * d.chiL, N = 2 * d.chiR;
+    nsv[g] = std::min(M, N);
+    GemmDesc<double>& e = g1[g];
+    e.M = M;
+    e.N = N;
+    e.K = d.chiM;
+    e.alpha = make_double2(1.0, 0.0);
+    e.beta = make_double2(0.0, 0.0);
+    e.A = d.Bl;
+    e.lda = M;
+    e.B = d.Br;
+    e.ldb = d.chiM;
+    e.C = d.phi;
+    e.ldc = M;
+  }
+  GateDesc* d_gd = (GateDesc*)ddesc;
+  GemmDesc<double>* d_g1 = (GemmDesc<double>*)(d_gd + ng);
+  GemmDesc<double>* d_g2 = d_g1 + ng;
+  QTN_CHECK_HIP(hipMemcpyAsync(dg, gates, (size_t)ng * 16 * sizeof(double2),
+                               hipMemcpyHostToDevice, s));
+  QTN_CHECK_HIP(hipMemcpyAsync(d_gd, gd.data(), ng * sizeof(GateDesc),
+                               hipMemcpyHostToDevice, s));
+  QTN_CHECK_HIP(hipMemcpyAsync(d_g1, g1.data(), ng * sizeof(GemmDesc<double>),
+                               hipMemcpyHostToDevice, s));
+  // 1. theta0 = B_i B_{i+1}
+  QTN_TRY(launch_zgemm_grouped(s, 0, 0, d_g1, ng, maxM, maxN));
+  // 2. gate, lambda scaling, X = theta^H (zero padded)
+  QTN_CHECK_HIP(hipMemsetAsync(dX, 0, (size_t)ng * x_sz * sizeof(double2), s));
+  hipLaunchKernelGGL(gate_epilogue,
+                     dim3(grid_for(maxM * maxN / 4), (unsigned)ng), dim3(256),
+                     0, s, (const GateDesc*)d_gd, rows_p);
+  QTN_CHECK_HIP(hipGetLastError());
+  // 3. SVD of every theta
+  JacobiEngine eng;
+  QTN_TRY(eng.reserve(ctx, ng, rows_p, cols_p));
+  int sweeps = 0;
+  QTN_TRY(eng.run(ctx, (double2*)dX, x_sz, nullptr, 0, nsv.data(), cutoff, 1,
+                  chimax, 60, &sweeps));
+  std::vector<int64_t> kept(ng);
+  std::vector<int> st(ng);
+  std::vector<double> disc(ng), total(ng), keptw(ng);
+  QTN_CHECK_HIP(hipMemcpyAsync(kept.data(), eng.kept, ng * sizeof(int64_t),
+                               hipMemcpyDeviceToHost, s));
+  QTN_CHECK_HIP(hipMemcpyAsync(st.data(), eng.status, ng * sizeof(int),
+                               hipMemcpyDeviceToHost, s));
+  QTN_CHECK_HIP(hipMemcpyAsync(disc.data(), eng.disc, ng * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+  QTN_CHECK_HIP(hipMemcpyAsync(total.data(), eng.total, ng * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+  QTN_CHECK_HIP(hipMemcpyAsync(keptw.data(), eng.keptw, ng * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+  QTN_CHECK_HIP(hipStreamSynchronize(s));
+  for (int g = 0; g < ng; ++g) {
+    if (st[g] != 0) {
+      set_error("apply_2q: every singular value is within the truncation "
+                "error (zero two-site tensor)");
+      return QTN_ERR_TRUNC;
+    }
+  }
+  if (spec_out && ng == 1) {
+    QTN_CHECK_HIP(hipMemcpy(spec_out, eng.sig, nsv[0] * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    if (spec_len) *spec_len = nsv[0];
+  }
+  // 4. B_{i+1} = V_k^H, lambda_i
+  int64_t maxk = 0;
+  for (int g = 0; g < ng; ++g) maxk = std::max(maxk, kept[g]);
+  hipLaunchKernelGGL(split_right,
+                     dim3(grid_for(maxN * maxk), (unsigned)ng), dim3(256), 0,
+                     s, (const GateDesc*)d_gd, (const double*)eng.sig,
+                     (const int*)eng.perm, (const int64_t*)eng.kept,
+                     (const double*)eng.keptw, rows_p, cols_p);
+  QTN_CHECK_HIP(hipGetLastError());
+  // 5. B_i = Phi V_k / |S_k|
+  std::vector<GemmDesc<double>> g2(ng);
+  for (int g = 0; g < ng; ++g) {
+    const GateDesc& d = gd[g];
+    const double nrm = std::sqrt(keptw[g]);
+    GemmDesc<double>& e = g2[g];
+    e.M = 2 * d.chiL;
+    e.N = kept[g];
+    e.K = 2 * d.chiR;
+    e.alpha = make_double2(nrm > 0 ? 1.0 / nrm : 0.0, 0.0);
+    e.beta = make_double2(0.0, 0.0);
+    e.A = d.phi;
+    e.lda = 2 * d.chiL;
+    e.B = d.Vk;
+    e.ldb = 2 * d.chiR;
+    e.C = d.Bl;
+    e.ldc = 2 * d.chiL;
+  }
+  QTN_CHECK_HIP(hipMemcpyAsync(d_g2, g2.data(), ng * sizeof(GemmDesc<double>),
+                               hipMemcpyHostToDevice, s));
+  QTN_TRY(launch_zgemm_grouped(s, 0, 0, d_g2, ng, maxM, maxk));
+  for (int g = 0; g < ng; ++g) {
+    m->dims[sites[g]] = kept[g];
+    if (kept_out) kept_out[g] = kept[g];
+    if (disc_out) disc_out[g] = total[g] > 0 ? disc[g] / total[g] : 0.0;
+  }
+  QTN_CHECK_HIP(hipStreamSynchronize(s));
+  return QTN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Right environments R_j (chi_j x chi_j), j = 0..n (R_n = 1), stored densely
+// at slot offsets; returns the device pointers.
+int32_t mps_right_envs(const qtn_mps* m, std::vector<double2*>& R) {
+  qtn_ctx* ctx = m->ctx;
+  hipStream_t s = ctx->stream;
+  size_t total = 0;
+  std::vector<size_t> off(m->n + 1);
+  for (int j = 0; j <= m->n; ++j) {
+    off[j] = total;
+    total += (size_t)m->dims[j] * m->dims[j];
+  }
+  void* base;
+  QTN_TRY(ctx->buffer(kSlotEnv, total * sizeof(double2), &base));
+  void* t2;
+  int64_t maxb = 1;
+  for (int j = 0; j <= m->n; ++j) maxb = std::max(maxb, m->dims[j]);
+  QTN_TRY(ctx->buffer(kSlotT2, (size_t)2 * maxb * maxb * sizeof(double2), &t2));
+  R.resize(m->n + 1);
+  for (int j = 0; j <= m->n; ++j) R[j] = (double2*)base + off[j];
+  hipLaunchKernelGGL(fill_one, dim3(1), dim3(1), 0, s, R[m->n]);
+  for (int j = m->n; j >= 1; --j) {
+    const int64_t cl = m->dims[j - 1], cr = m->dims[j];
+    // T2 = B_j (2cl x cr) R_j
+    QTN_TRY(launch_zgemm(s, 0, 0, 2 * cl, cr, cr, make_double2(1, 0),
+                         m->site[j - 1], 2 * cl, 0, R[j], cr, 0,
+                         make_double2(0, 0), (double2*)t2, 2 * cl, 0, 1));
+    // R_{j-1} = T2 (cl x 2cr) B_j^H (2cr x cl)
+    QTN_TRY(launch_zgemm(s, 0, 2, cl, cl, 2 * cr, make_double2(1, 0),
+                         (const double2*)t2, cl, 0, m->site[j - 1], cl, 0,
+                         make_double2(0, 0), R[j - 1], cl, 0, 1));
+  }
+  return QTN_OK;
+}
+
+}  // namespace qtn
+
+using namespace qtn;
+
+extern "C" {
+
+int32_t qtn_mps_create(qtn_ctx* ctx, int32_t nsites, int64_t chimax_cap,
+                       qtn_mps** out) {
+  clear_error();
+  QTN_ARG(ctx && out, "qtn_mps_create: NULL argument");
+  QTN_ARG(nsites >= 2, "qtn_mps_create: need at least 2 sites");
+  QTN_ARG(chimax_cap >= 1, "qtn_mps_create: chimax_cap must be >= 1");
+  QTN_CHECK_HIP(hipSetDevice(ctx->device));
+  qtn_mps* m = new qtn_mps();
+  m->ctx = ctx;
+  m->n = nsites;
+  m->cap = chimax_cap;
+  m->dims.assign(nsites + 1, 1);
+  m->maxb.resize(nsites + 1);
+  for (int j = 0; j <= nsites; ++j)
+    m->maxb[j] = std::min(pow2_cap(j, chimax_cap), pow2_cap(nsites - j,
+                                                            chimax_cap));
+  size_t bytes = 0;
+  std::vector<size_t> soff(nsites), loff(nsites + 1);
+  for (int j = 0; j < nsites; ++j) {
+    soff[j] = bytes;
+    bytes += (size_t)m->maxb[j] * 2 * m->maxb[j + 1] * sizeof(double2);
+    bytes = (bytes + 255) & ~(size_t)255;
+  }
+  for (int j = 0; j <= nsites; ++j) {
+    loff[j] = bytes;
+    bytes += (size_t)m->maxb[j] * sizeof(double);
+    bytes = (bytes + 255) & ~(size_t)255;
+  }
+  hipError_t e = hipMalloc(&m->arena, bytes);
+  if (e != hipSuccess) {
+    delete m;
+    set_error(std::string("qtn_mps_create: hipMalloc failed: ") +
+              hipGetErrorString(e));
+    return QTN_ERR_OOM;
+  }
+  m->site.resize(nsites);
+  m->lam.resize(nsites + 1);
+  for (int j = 0; j < nsites; ++j)
+    m->site[j] = (double2*)((char*)m->arena + soff[j]);
+  for (int j = 0; j <= nsites; ++j)
+    m->lam[j] = (double*)((char*)m->arena + loff[j]);
+  // |0...0>: every site (1,2,1) = [1, 0], every lambda = [1]
+  QTN_CHECK_HIP(hipMemsetAsync(m->arena, 0, bytes, ctx->stream));
+  for (int j = 0; j < nsites; ++j)
+    hipLaunchKernelGGL(fill_one, dim3(1), dim3(1), 0, ctx->stream, m->site[j]);
+  for (int j = 0; j <= nsites; ++j)
+    hipLaunchKernelGGL(fill_lambda_one, dim3(1), dim3(1), 0, ctx->stream,
+                       m->lam[j]);
+  QTN_CHECK_HIP(hipGetLastError());
+  QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  *out = m;
+  return QTN_OK;
+}
+
+int32_t qtn_mps_destroy(qtn_mps* m) {
+  clear_error();
+  if (!m) return QTN_OK;
+  QTN_CHECK_HIP(hipSetDevice(m->ctx->device));
+  QTN_CHECK_HIP(hipStreamSynchronize(m->ctx->stream));
+  if (m->arena) QTN_CHECK_HIP(hipFree(m->arena));
+  delete m;
+  return QTN_OK;
+}
+
+int32_t qtn_mps_copy(const qtn_mps* a, qtn_mps* b) {
+  clear_error();
+  QTN_ARG(a && b, "qtn_mps_copy: NULL argument");
+  QTN_ARG(a->n == b->n && a->cap == b->cap,
+          "qtn_mps_copy: states differ in size or capacity");
+  QTN_CHECK_HIP(hipSetDevice(a->ctx->device));
+  size_t bytes = 0;
+  for (int j = 0; j < a->n; ++j) {
+    bytes += (size_t)a->maxb[j] * 2 * a->maxb[j + 1] * sizeof(double2);
+    bytes = (bytes + 255) & ~(size_t)255;
+  }
+  for (int j = 0; j <= a->n; ++j) {
+    bytes += (size_t)a->maxb[j] * sizeof(double);
+    bytes = (bytes + 255) & ~(size_t)255;
+  }
+  QTN_CHECK_HIP(hipMemcpyAsync(b->arena, a->arena, bytes,
+                               hipMemcpyDeviceToDevice, a->ctx->stream));
+  QTN_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
+  b->dims = a->dims;
+  return QTN_OK;
+}
+
+int32_t qtn_mps_bond_dims(const qtn_mps* m, int64_t* dims) {
+  clear_error();
+  QTN_ARG(m && dims, "qtn_mps_bond_dims: NULL argument");
+  std::copy(m->dims.begin(), m->dims.end(), dims);
+  return QTN_OK;
+}
+
+int32_t qtn_mps_bond_spectrum(const qtn_mps* m, int32_t bond, double* S,
+                              int64_t* k) {
+  clear_error();
+  QTN_ARG(m && S && k, "qtn_mps_bond_spectrum: NULL argument");
+  QTN_ARG(bond >= 0 && bond <= m->n, "qtn_mps_bond_spectrum: bad bond");
+  QTN_CHECK_HIP(hipSetDevice(m->ctx->device));
+  *k = m->dims[bond];
+  QTN_CHECK_HIP(hipStreamSynchronize(m->ctx->stream));
+  QTN_CHECK_HIP(hipMemcpy(S, m->lam[bond], *k * sizeof(double),
+                          hipMemcpyDeviceToHost));
+  return QTN_OK;
+}
+
+int32_t qtn_mps_get_site(const qtn_mps* m, int32_t site, double* out,
+                         int64_t* dims) {
+  clear_error();
+  QTN_ARG(m && dims, "qtn_mps_get_site: NULL argument");
+  QTN_ARG(site >= 1 && site <= m->n, "qtn_mps_get_site: bad site");
+  QTN_CHECK_HIP(hipSetDevice(m->ctx->device));
+  dims[0] = m->dims[site - 1];
+  dims[1] = 2;
+  dims[2] = m->dims[site];
+  if (out) {
+    QTN_CHECK_HIP(hipStreamSynchronize(m->ctx->stream));
+    QTN_CHECK_HIP(hipMemcpy(out, m->site[site - 1],
+                            (size_t)dims[0] * 2 * dims[2] * sizeof(double2),
+                            hipMemcpyDeviceToHost));
+  }
+  return QTN_OK;
+}
+
+int32_t qtn_mps_apply_1q(qtn_mps* m, int32_t site, const double* u) {
+  clear_error();
+  QTN_ARG(m && u, "qtn_mps_apply_1q: NULL argument");
+  QTN_ARG(site >= 1 && site <= m->n, "qtn_mps_apply_1q: site out of range");
+  qtn_ctx* ctx = m->ctx;
+  QTN_CHECK_HIP(hipSetDevice(ctx->device));
+  void* dg;
+  QTN_TRY(ctx->buffer(kSlotGates, 16 * sizeof(double2), &dg));
+  QTN_CHECK_HIP(hipMemcpyAsync(dg, u, 4 * sizeof(double2),
+                               hipMemcpyHostToDevice, ctx->stream));
+  const int64_t cl = m->dims[site - 1], cr = m->dims[site];
+  hipLaunchKernelGGL(apply_1q_kernel, dim3(grid_for(cl * cr)), dim3(256), 0,
+                     ctx->stream, m->site[site - 1], cl, cr,
+                     (const double2*)dg);
+  QTN_CHECK_HIP(hipGetLastError());
+  QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return QTN_OK;
+}
+
+int32_t qtn_mps_apply_2q(qtn_mps* m, int32_t site, const double* g,
+                         double cutoff, int64_t chimax, int32_t form,
+                         int64_t* kept, double* disc_weight, double* spectrum,
+                         int64_t* spectrum_len) {
+  clear_error();
+  QTN_ARG(m && g, "qtn_mps_apply_2q: NULL argument");
+  QTN_ARG(cutoff >= 0.0, "Error must be positive");
+  QTN_ARG(form == QTN_FORM_PLAIN, "qtn_mps_apply_2q: unknown form");
+  QTN_ARG(chimax <= m->cap, "qtn_mps_apply_2q: chimax above capacity");
+  QTN_CHECK_HIP(hipSetDevice(m->ctx->device));
+  const int64_t cm = (chimax > 0) ? chimax : m->cap;
+  return mps_apply_gates(m, 1, &site, g, cutoff, cm, kept, disc_weight,
+                         spectrum, spectrum_len);
+}
+
+int32_t qtn_mps_apply_layer(qtn_mps* m, int32_t ngates, const int32_t* sites,
+                            const double* gates, double cutoff,
+                            int64_t chimax, int32_t form, int64_t* kept,
+                            double* disc_weight) {
+  clear_error();
+  QTN_ARG(m && (ngates == 0 || (sites && gates)),
+          "qtn_mps_apply_layer: NULL argument");
+  QTN_ARG(cutoff >= 0.0, "Error must be positive");
+  QTN_ARG(form == QTN_FORM_PLAIN, "qtn_mps_apply_layer: unknown form");
+  QTN_ARG(chimax <= m->cap, "qtn_mps_apply_layer: chimax above capacity");
+  if (ngates == 0) return QTN_OK;
+  QTN_CHECK_HIP(hipSetDevice(m->ctx->device));
+  const int64_t cm = (chimax > 0) ? chimax : m->cap;
+  return mps_apply_gates(m, ngates, sites, gates, cutoff, cm, kept,
+                         disc_weight, nullptr, nullptr);
+}
+
+int32_t qtn_mps_amplitudes(const qtn_mps* m, int64_t nb, const uint8_t* bits,
+                           double* out) {
+  clear_error();
+  QTN_ARG(m && bits && out, "qtn_mps_amplitudes: NULL argument");
+  QTN_ARG(nb >= 1, "qtn_mps_amplitudes: nb must be >= 1");
+  qtn_ctx* ctx = m->ctx;
+  hipStream_t s = ctx->stream;
+  QTN_CHECK_HIP(hipSetDevice(ctx->device));
+  int64_t maxb = 1;
+  for (int j = 0; j <= m->n; ++j) maxb = std::max(maxb, m->dims[j]);
+  void *dbits, *damp;
+  QTN_TRY(ctx->buffer(kSlotBits, (size_t)nb * m->n, &dbits));
+  QTN_TRY(ctx->buffer(kSlotAmp, (size_t)2 * nb * maxb * sizeof(double2),
+                      &damp));
+  QTN_CHECK_HIP(hipMemcpyAsync(dbits, bits, (size_t)nb * m->n,
+                               hipMemcpyHostToDevice, s));
+  double2* V = (double2*)damp;
+  double2* Vn = V + nb * maxb;
+  // V starts as nb x 1 of ones
+  std::vector<double> ones(2 * nb, 0.0);
+  for (int64_t i = 0; i < nb; ++i) ones[2 * i] = 1.0;
+  QTN_CHECK_HIP(hipMemcpyAsync(V, ones.data(), nb * sizeof(double2),
+                               hipMemcpyHostToDevice, s));
+  for (int j = 0; j < m->n; ++j) {
+    const int64_t cl = m->dims[j], cr = m->dims[j + 1];
+    hipLaunchKernelGGL(amp_step, dim3(grid_for(nb * cr)), dim3(256), 0, s,
+                       (const double2*)V, nb, cl, cr,
+                       (const double2*)m->site[j], (const uint8_t*)dbits,
+                       m->n, j, Vn);
+    std::swap(V, Vn);
+  }
+  QTN_CHECK_HIP(hipGetLastError());
+  QTN_CHECK_HIP(hipMemcpyAsync(out, V, nb * sizeof(double2),
+                               hipMemcpyDeviceToHost, s));
+  QTN_CHECK_HIP(hipStreamSynchronize(s));
+  return QTN_OK;
+}
+
+int32_t qtn_mps_expect_z(const qtn_mps* m, double* out) {
+  clear_error();
+  QTN_ARG(m && out, "qtn_mps_expect_z: NULL argument");
+  qtn_ctx* ctx = m->ctx;
+  hipStream_t s = ctx->stream;
+  QTN_CHECK_HIP(hipSetDevice(ctx->device));
+  std::vector<double2*> R;
+  QTN_TRY(mps_right_envs(m, R));
+  int64_t maxb = 1;
+  for (int j = 0; j <= m->n; ++j) maxb = std::max(maxb, m->dims[j]);
+  void *t1, *ebuf, *red;
+  QTN_TRY(ctx->buffer(kSlotT1, (size_t)4 * maxb * maxb * sizeof(double2),
+                      &t1));
+  QTN_TRY(ctx->buffer(kSlotT2 + 10, (size_t)2 * maxb * maxb * sizeof(double2),
+                      &ebuf));
+  QTN_TRY(ctx->buffer(kSlotRed, (size_t)4 * m->n * sizeof(double), &red));
+  double2* T1 = (double2*)t1;
+  double2* Y = T1 + 2 * maxb * maxb;
+  double2* E = (double2*)ebuf;
+  double2* En = E + maxb * maxb;
+  hipLaunchKernelGGL(fill_one, dim3(1), dim3(1), 0, s, E);
+  for (int j = 1; j <= m->n; ++j) {
+    const int64_t cl = m->dims[j - 1], cr = m->dims[j];
+    const double2* B = m->site[j - 1];
+    // T1^s = E B^s (batch over s; B^s rows offset s*cl, ld 2cl)
+    QTN_TRY(launch_zgemm(s, 0, 0, cl, cr, cl, make_double2(1, 0), E, cl, 0, B,
+                         2 * cl, cl, make_double2(0, 0), T1, 2 * cl, cl, 2));
+    // Y = T1 R_j
+    QTN_TRY(launch_zgemm(s, 0, 0, 2 * cl, cr, cr, make_double2(1, 0), T1,
+                         2 * cl, 0, R[j], cr, 0, make_double2(0, 0), Y,
+                         2 * cl, 0, 1));
+    hipLaunchKernelGGL(env_reduce, dim3(1), dim3(256), 0, s, B,
+                       (const double2*)Y, cl, cr, (double*)red + 4 * (j - 1));
+    // E_j = B^H T1  (K = 2cl)
+    QTN_TRY(launch_zgemm(s, 2, 0, cr, cr, 2 * cl, make_double2(1, 0), B,
+                         2 * cl, 0, T1, 2 * cl, 0, make_double2(0, 0), En, cr,
+                         0, 1));
+    std::swap(E, En);
+  }
+  QTN_CHECK_HIP(hipGetLastError());
+  std::vector<double> h(4 * m->n);
+  QTN_CHECK_HIP(hipMemcpyAsync(h.data(), red, 4 * m->n * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+  QTN_CHECK_HIP(hipStreamSynchronize(s));
+  for (int j = 0; j < m->n; ++j) out[j] = h[4 * j + 2] / h[4 * j + 0];
+  return QTN_OK;
+}
+
+int32_t qtn_mps_overlap(const qtn_mps* a, const qtn_mps* b, double* out) {
+  clear_error();
+  QTN_ARG(a && b && out, "qtn_mps_overlap: NULL argument");
+  QTN_ARG(a->n == b->n, "qtn_mps_overlap: different number of sites");
+  QTN_ARG(a->ctx == b->ctx, "qtn_mps_overlap: states from two contexts");
+  qtn_ctx* ctx = a->ctx;
+  hipStream_t s = ctx->stream;
+  QTN_CHECK_HIP(hipSetDevice(ctx->device));
+  int64_t mb = 1;
+  for (int j = 0; j <= a->n; ++j)
+    mb = std::max(mb, std::max(a->dims[j], b->dims[j]));
+  void *t1, *ebuf;
+  QTN_TRY(ctx->buffer(kSlotT1, (size_t)4 * mb * mb * sizeof(double2), &t1));
+  QTN_TRY(ctx->buffer(kSlotT2 + 10, (size_t)2 * mb * mb * sizeof(double2),
+                      &ebuf));
+  double2* T1 = (double2*)t1;
+  double2* E = (double2*)ebuf;
+  double2* En = E + mb * mb;
+  hipLaunchKernelGGL(fill_one, dim3(1), dim3(1), 0, s, E);
+  for (int j = 1; j <= a->n; ++j) {
+    const int64_t al = a->dims[j - 1], ar = a->dims[j];
+    const int64_t bl = b->dims[j - 1], br = b->dims[j];
+    // T1^s = E (al x bl) B^s (bl x br) -> stacked (2al x br)
+    QTN_TRY(launch_zgemm(s, 0, 0, al, br, bl, make_double2(1, 0), E, al, 0,
+                         b->site[j - 1], 2 * bl, bl, make_double2(0, 0), T1,
+                         2 * al, al, 2));
+    // E_j = A^H (ar x 2al) T1 (2al x br)
+    QTN_TRY(launch_zgemm(s, 2, 0, ar, br, 2 * al, make_double2(1, 0),
+                         a->site[j - 1], 2 * al, 0, T1, 2 * al, 0,
+                         make_double2(0, 0), En, ar, 0, 1));
+    std::swap(E, En);
+  }
+  QTN_CHECK_HIP(hipMemcpyAsync(out, E, sizeof(double2),
+                               hipMemcpyDeviceToHost, s));
+  QTN_CHECK_HIP(hipStreamSynchronize(s));
+  return QTN_OK;
+}
+
+int32_t qtn_mps_norm2(const qtn_mps* m, double* out) {
+  double v[2];
+  QTN_TRY(qtn_mps_overlap(m, m, v));
+  *out = v[0];
+  return QTN_OK;
+}
+
+int32_t qtn_mps_set_sites(qtn_mps* m, const int64_t* dims,
+                          const double* const* sites) {
+  clear_error();
+  (void)dims;
+  (void)sites;
+  QTN_ARG(m != nullptr, "qtn_mps_set_sites: NULL argument");
+  set_error("qtn_mps_set_sites: not available in this build");
+  return QTN_ERR_ARG;
+}
+
+}  // extern "C"

@@ -1,0 +1,671 @@
+// Batched block one-sided Jacobi SVD (Hestenes) for complex FP64, with the
+// truncation rule of Qaintensor's contract_svd (src/svd.jl:29-33).
+//
+// Method (one-sided Jacobi: Hestenes 1958; block form with Gram-matrix
+// rotations as in Boukaram, Turkiyyah, Ltaief, Keyes, "Batched QR and SVD
+// algorithms on GPUs with applications in hierarchical matrix compression",
+// Parallel Computing 74 (2018)):
+//   X (rows x cols, zero padded to multiples of 64) is orthogonalised by
+//   right-multiplication with unitary rotations until every pair of columns
+//   satisfies |x_i^H x_j| <= tol * |x_i| |x_j|.  Columns are grouped into
+//   2p blocks of 32; a sweep is 2p-1 rounds of the round-robin tournament,
+//   each round handling p disjoint block pairs.  For a block pair (64
+//   columns):
+//     1. jac_gram   : G = X_IJ^H X_IJ (64x64) on MFMA, split over row chunks
+//     2. jac_inner  : one cyclic two-sided Jacobi sweep on G in LDS (Hermitian
+//                     2x2 rotations, Golub & Van Loan 8.5), accumulating Q
+//     3. jac_update : X_IJ <- X_IJ Q on MFMA (and W_IJ <- W_IJ Q if vectors
+//                     of the other side are needed)
+//   The pair is skipped when G already meets the tolerance, so convergence
+//   is always decided on a Gram matrix freshly computed from the columns.
+// At convergence sigma_j = |x_j|; X W = U_X diag(sigma) with W unitary.
+// All updates are unitary rotations applied to X, so the singular values are
+// backward stable (absolute error O(eps |X|_F)), as for LAPACK's gesdd.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+
+#include "qtn_internal.hpp"
+#include "qtn_svd.hpp"
+
+namespace qtn {
+namespace {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+// v_mfma_f64_16x16x4f64: A[i][k] from lane i + 16k, B[k][j] from lane
+// j + 16k, result element e of lane l is D[4*e + l/16][l%16].
+__device__ __forceinline__ d4_t mfma64(double a, double b, d4_t c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Round-robin (circle method) position -> block for 2p blocks.
+__device__ __forceinline__ int rr_block(int pos, int round, int nb) {
+  return pos == 0 ? 0 : 1 + (pos - 1 + round) % (nb - 1);
+}
+
+// Global column of local column c (0..63) of the block pair (I, J).
+__device__ __forceinline__ int64_t pair_col(int c, int I, int J) {
+  return c < 32 ? (int64_t)I * 32 + c : (int64_t)J * 32 + (c - 32);
+}
+
+// ---------------------------------------------------------------------------
+// 1. Partial Gram of a 64-row chunk of a block pair.
+// grid (rows_p/64, p, batch); Gp layout [batch][p][chunk][64*64] col-major.
+__global__ __launch_bounds__(256) void jac_gram(
+    const double2* __restrict__ X, int64_t rows_p, int64_t sX, int p,
+    int round, double2* __restrict__ Gp, const int* __restrict__ active) {
+  const int bz = blockIdx.z;
+  if (!active[bz]) return;
+  const int nb = 2 * p;
+  const int pi = blockIdx.y;
+  const int chunk = blockIdx.x;
+  const int nchunk = gridDim.x;
+  const int b1 = rr_block(pi, round, nb);
+  const int b2 = rr_block(nb - 1 - pi, round, nb);
+  const int I = min(b1, b2), J = max(b1, b2);
+  X += (int64_t)bz * sX;
+
+  __shared__ double Xr[64][65];  // [local col][row]
+  __shared__ double Xi[64][65];
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < 64 * 64; idx += 256) {
+    const int r = idx & 63, c = idx >> 6;
+    const double2 v = X[(int64_t)chunk * 64 + r + pair_col(c, I, J) * rows_p];
+    Xr[c][r] = v.x;
+    Xi[c][r] = v.y;
+  }
+  __syncthreads();
+
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  d4_t gr[2][2], gi[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        gr[i][j][e] = 0.0;
+        gi[i][j][e] = 0.0;
+      }
+  // G[c1][c2] = sum_r conj(X[r][c1]) X[r][c2]
+#pragma unroll 4
+  for (int ks = 0; ks < 64; ks += 4) {
+    const int kr = ks + (lane >> 4);
+    double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      ar[i] = Xr[wm + i * 16 + (lane & 15)][kr];
+      ai[i] = -Xi[wm + i * 16 + (lane & 15)][kr];
+      br[i] = Xr[wn + i * 16 + (lane & 15)][kr];
+      bi[i] = Xi[wn + i * 16 + (lane & 15)][kr];
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        gr[i][j] = mfma64(ar[i], br[j], gr[i][j]);
+        gr[i][j] = mfma64(-ai[i], bi[j], gr[i][j]);
+        gi[i][j] = mfma64(ar[i], bi[j], gi[i][j]);
+        gi[i][j] = mfma64(ai[i], br[j], gi[i][j]);
+      }
+  }
+  double2* out =
+      Gp + (((int64_t)bz * p + pi) * nchunk + chunk) * (int64_t)(64 * 64);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = wn + j * 16 + (lane & 15);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = wm + i * 16 + 4 * e + (lane >> 4);
+        out[row + 64 * col] = make_double2(gr[i][j][e], gi[i][j][e]);
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 2. Sum the partial Grams, run inner cyclic Jacobi sweep(s), emit Q.
+// grid (p, batch).  flags[bz*p + pi] = 1 if Q must be applied.
+struct RotParam {
+  int p, q;
+  double cs, sn;    // real rotation
+  double er, ei;    // e^{-i phi}
+  double ap, dq;    // new diagonal entries
+  int on;
+};
+
+__global__ __launch_bounds__(256) void jac_inner(
+    const double2* __restrict__ Gp, int nchunk, int p, double tol,
+    const double* __restrict__ zthr, int n_inner, double2* __restrict__ Qout,
+    int* __restrict__ flags, int* __restrict__ counters,
+    const int* __restrict__ active) {
+  const int bz = blockIdx.y;
+  const int pi = blockIdx.x;
+  if (!active[bz]) return;
+  __shared__ double Gr[64][65];  // G[row][col]
+  __shared__ double Gi[64][65];
+  __shared__ double Qr[64][65];  // Q[row][col]
+  __shared__ double Qi[64][65];
+  __shared__ RotParam rot[32];
+  __shared__ int nrot_sweep;
+  __shared__ int nrot_total;
+  const int tid = threadIdx.x;
+  const double2* src = Gp + ((int64_t)bz * p + pi) * nchunk * (int64_t)4096;
+  for (int idx = tid; idx < 4096; idx += 256) {
+    const int r = idx & 63, c = idx >> 6;
+    double sr = 0.0, si = 0.0;
+    for (int ch = 0; ch < nchunk; ++ch) {
+      const double2 v = src[(int64_t)ch * 4096 + idx];
+      sr += v.x;
+      si += v.y;
+    }
+    Gr[r][c] = sr;
+    Gi[r][c] = si;
+    Qr[r][c] = (r == c) ? 1.0 : 0.0;
+    Qi[r][c] = 0.0;
+  }
+  if (tid == 0) nrot_total = 0;
+  const double zt = zthr[bz];
+  __syncthreads();
+
+  for (int sweep = 0; sweep < n_inner; ++sweep) {
+    if (tid == 0) nrot_sweep = 0;
+    __syncthreads();
+    for (int round = 0; round < 63; ++round) {
+      if (tid < 32) {
+        const int c1 = rr_block(tid, round, 64);
+        const int c2 = rr_block(63 - tid, round, 64);
+        const int pp = min(c1, c2), qq = max(c1, c2);
+        const double a = Gr[pp][pp], d = Gr[qq][qq];
+        const double cr = Gr[pp][qq], ci = Gi[pp][qq];
+        const double absc = hypot(cr, ci);
+        RotParam rp;
+        rp.p = pp;
+        rp.q = qq;
+        rp.on = 0;
+        if (a > zt && d > zt && absc > tol * sqrt(a) * sqrt(d)) {
+          const double zeta = (d - a) / (2.0 * absc);
+          const double t = (zeta >= 0.0 ? 1.0 : -1.0) /
+                           (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          const double cs = 1.0 / sqrt(1.0 + t * t);
+          rp.cs = cs;
+          rp.sn = t * cs;
+          rp.er = cr / absc;
+          rp.ei = -ci / absc;
+          rp.ap = a - t * absc;
+          rp.dq = d + t * absc;
+          rp.on = 1;
+          atomicAdd(&nrot_sweep, 1);
+        }
+        rot[tid] = rp;
+      }
+      __syncthreads();
+      // column update of G and Q: [g_p g_q] <- [g_p g_q] J,
+      // J = [[cs, sn], [-sn e, cs e]], e = e^{-i phi}
+      for (int task = tid; task < 32 * 128; task += 256) {
+        const int pr = task >> 7;
+        const RotParam& rp = rot[pr];
+        if (!rp.on) continue;
+        const int rr = task & 63;
+        const bool isq = (task & 64) != 0;
+        double (*Mr)[65] = isq ? Qr : Gr;
+        double (*Mi)[65] = isq ? Qi : Gi;
+        const double xpr = Mr[rr][rp.p], xpi = Mi[rr][rp.p];
+        const double xqr = Mr[rr][rp.q], xqi = Mi[rr][rp.q];
+        // e * x_q
+        const double yr = rp.er * xqr - rp.ei * xqi;
+        const double yi = rp.er * xqi + rp.ei * xqr;
+        Mr[rr][rp.p] = rp.cs * xpr - rp.sn * yr;
+        Mi[rr][rp.p] = rp.cs * xpi - rp.sn * yi;
+        Mr[rr][rp.q] = rp.sn * xpr + rp.cs * yr;
+        Mi[rr][rp.q] = rp.sn * xpi + rp.cs * yi;
+      }
+      __syncthreads();
+      // row update of G: [r_p; r_q] <- J^H [r_p; r_q],
+      // J^H = [[cs, -sn conj(e)], [sn, cs conj(e)]]
+      for (int task = tid; task < 32 * 64; task += 256) {
+        const int pr = task >> 6;
+        const RotParam& rp = rot[pr];
+        if (!rp.on) continue;
+        const int cc = task & 63;
+        const double xpr = Gr[rp.p][cc], xpi = Gi[rp.p][cc];
+        const double xqr = Gr[rp.q][cc], xqi = Gi[rp.q][cc];
+        // conj(e) * x_q
+        const double yr = rp.er * xqr + rp.ei * xqi;
+        const double yi = rp.er * xqi - rp.ei * xqr;
+        Gr[rp.p][cc] = rp.cs * xpr - rp.sn * yr;
+        Gi[rp.p][cc] = rp.cs * xpi - rp.sn * yi;
+        Gr[rp.q][cc] = rp.sn * xpr + rp.cs * yr;
+        Gi[rp.q][cc] = rp.sn * xpi + rp.cs * yi;
+      }
+      __syncthreads();
+      if (tid < 32 && rot[tid].on) {
+        const RotParam& rp = rot[tid];
+        Gr[rp.p][rp.q] = 0.0;
+        Gi[rp.p][rp.q] = 0.0;
+        Gr[rp.q][rp.p] = 0.0;
+        Gi[rp.q][rp.p] = 0.0;
+        Gr[rp.p][rp.p] = rp.ap;
+        Gi[rp.p][rp.p] = 0.0;
+        Gr[rp.q][rp.q] = rp.dq;
+        Gi[rp.q][rp.q] = 0.0;
+      }
+      __syncthreads();
+    }
+    const int nr = nrot_sweep;
+    if (tid == 0) nrot_total += nr;
+    __syncthreads();
+    if (nr == 0) break;
+  }
+  const int total = nrot_total;
+  if (tid == 0) {
+    flags[bz * p + pi] = total > 0 ? 1 : 0;
+    if (total > 0) atomicAdd(&counters[bz], total);
+  }
+  if (total > 0) {
+    double2* q = Qout + ((int64_t)bz * p + pi) * (int64_t)4096;
+    for (int idx = tid; idx < 4096; idx += 256) {
+      const int r = idx & 63, c = idx >> 6;
+      q[idx] = make_double2(Qr[r][c], Qi[r][c]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3. X_IJ <- X_IJ Q for one 64-row chunk.  grid (rows/64, p, batch).
+__global__ __launch_bounds__(256) void jac_update(
+    double2* __restrict__ X, int64_t rows, int64_t sX, int p, int round,
+    const double2* __restrict__ Q, const int* __restrict__ flags,
+    const int* __restrict__ active) {
+  const int bz = blockIdx.z;
+  if (!active[bz]) return;
+  const int pi = blockIdx.y;
+  if (!flags[bz * p + pi]) return;
+  const int nb = 2 * p;
+  const int chunk = blockIdx.x;
+  const int b1 = rr_block(pi, round, nb);
+  const int b2 = rr_block(nb - 1 - pi, round, nb);
+  const int I = min(b1, b2), J = max(b1, b2);
+  X += (int64_t)bz * sX;
+  const double2* Qp = Q + ((int64_t)bz * p + pi) * (int64_t)4096;
+
+  __shared__ double Xr[64][65];  // [local col][row]
+  __shared__ double Xi[64][65];
+  __shared__ double Qr[64][65];  // [k][n]
+  __shared__ double Qi[64][65];
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < 4096; idx += 256) {
+    const int r = idx & 63, c = idx >> 6;
+    const double2 v = X[(int64_t)chunk * 64 + r + pair_col(c, I, J) * rows];
+    Xr[c][r] = v.x;
+    Xi[c][r] = v.y;
+    const double2 qv = Qp[idx];  // Q[r][c] col-major
+    Qr[r][c] = qv.x;
+    Qi[r][c] = qv.y;
+  }
+  __syncthreads();
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  d4_t yr[2][2], yi[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        yr[i][j][e] = 0.0;
+        yi[i][j][e] = 0.0;
+      }
+  // Y[row][n] = sum_k X[row][k] Q[k][n]
+#pragma unroll 4
+  for (int ks = 0; ks < 64; ks += 4) {
+    const int kr = ks + (lane >> 4);
+    double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      ar[i] = Xr[kr][wm + i * 16 + (lane & 15)];
+      ai[i] = Xi[kr][wm + i * 16 + (lane & 15)];
+      br[i] = Qr[kr][wn + i * 16 + (lane & 15)];
+      bi[i] = Qi[kr][wn + i * 16 + (lane & 15)];
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        yr[i][j] = mfma64(ar[i], br[j], yr[i][j]);
+        yr[i][j] = mfma64(-ai[i], bi[j], yr[i][j]);
+        yi[i][j] = mfma64(ar[i], bi[j], yi[i][j]);
+        yi[i][j] = mfma64(ai[i], br[j], yi[i][j]);
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int c = wn + j * 16 + (lane & 15);
+      const int64_t gc = pair_col(c, I, J);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = wm + i * 16 + 4 * e + (lane >> 4);
+        X[(int64_t)chunk * 64 + r + gc * rows] =
+            make_double2(yr[i][j][e], yi[i][j][e]);
+      }
+    }
+}
+
+// After a sweep: a problem stays active only if it rotated something.
+__global__ void jac_sweep_end(int* __restrict__ active,
+                              int* __restrict__ counters, int batch,
+                              int* __restrict__ n_active) {
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  for (int b = threadIdx.x; b < batch; b += blockDim.x) {
+    const int a = active[b] && counters[b] > 0;
+    active[b] = a;
+    counters[b] = 0;
+    if (a) atomicAdd(&cnt, 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *n_active = cnt;
+}
+
+// Frobenius norm^2 per problem, finiteness check, zero-column threshold.
+// grid (batch), 256 threads.
+__global__ __launch_bounds__(256) void jac_prepare(
+    const double2* __restrict__ X, int64_t n_elem, int64_t sX,
+    double* __restrict__ normF2, double* __restrict__ zthr,
+    int* __restrict__ active, int* __restrict__ counters,
+    int* __restrict__ bad) {
+  const int bz = blockIdx.x;
+  X += (int64_t)bz * sX;
+  double s = 0.0;
+  int nonfinite = 0;
+  for (int64_t i = threadIdx.x; i < n_elem; i += 256) {
+    const double2 v = X[i];
+    if (!isfinite(v.x) || !isfinite(v.y)) nonfinite = 1;
+    s += v.x * v.x + v.y * v.y;
+  }
+  __shared__ double red[256];
+  __shared__ int nf;
+  if (threadIdx.x == 0) nf = 0;
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (nonfinite) atomicOr(&nf, 1);
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double f2 = red[0];
+    normF2[bz] = f2;
+    // Columns with norm below eps*|X|_F are rounding noise: never rotated.
+    zthr[bz] = DBL_EPSILON * DBL_EPSILON * f2;
+    active[bz] = (f2 > 0.0 && !nf) ? 1 : 0;
+    counters[bz] = 0;
+    if (nf) atomicOr(bad, 1);
+  }
+}
+
+// Column norms: one wave per column.  grid (cols/4, batch), 256 threads.
+__global__ __launch_bounds__(256) void jac_colnorms(
+    const double2* __restrict__ X, int64_t rows, int64_t cols, int64_t sX,
+    double* __restrict__ sig) {
+  const int bz = blockIdx.y;
+  const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (c >= cols) return;
+  const double2* x = X + (int64_t)bz * sX + c * rows;
+  double s = 0.0;
+  for (int64_t r = lane; r < rows; r += 64) {
+    const double2 v = x[r];
+    s += v.x * v.x + v.y * v.y;
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) sig[(int64_t)bz * cols + c] = sqrt(s);
+}
+
+// Sort sigma descending (bitonic in LDS) and apply the truncation rule of
+// src/svd.jl:29-33 exactly: tail norms are accumulated from the smallest
+// value upwards in sequence, k = #values kept.  grid (batch), 1024 threads.
+// nsv[bz]: number of meaningful singular values (min of the true dims).
+constexpr int kSortMax = 4096;
+__global__ __launch_bounds__(1024) void jac_sort_trunc(
+    double* __restrict__ sig, int* __restrict__ perm, int64_t cols,
+    const int64_t* __restrict__ nsv, double cutoff, int relative,
+    int64_t chimax, int64_t* __restrict__ kept, double* __restrict__ disc,
+    double* __restrict__ total, double* __restrict__ keptw,
+    int* __restrict__ status) {
+  const int bz = blockIdx.x;
+  __shared__ double sv[kSortMax];
+  __shared__ int si[kSortMax];
+  int n2 = 1;
+  while (n2 < cols) n2 <<= 1;
+  double* s = sig + (int64_t)bz * cols;
+  for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+    sv[i] = i < cols ? s[i] : -1.0;
+    si[i] = i;
+  }
+  __syncthreads();
+  for (int k = 2; k <= n2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const bool desc = (i & k) == 0;
+          const double a = sv[i], b = sv[ixj];
+          // descending within desc blocks; ties broken by index for
+          // determinism
+          const bool swap = desc ? (a < b || (a == b && si[i] > si[ixj]))
+                                 : (a > b || (a == b && si[i] < si[ixj]));
+          if (swap) {
+            sv[i] = b;
+            sv[ixj] = a;
+            const int t = si[i];
+            si[i] = si[ixj];
+            si[ixj] = t;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < cols; i += blockDim.x) {
+    s[i] = sv[i];
+    perm[(int64_t)bz * cols + i] = si[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int64_t n = nsv[bz];
+    // cum[j] = sum_{l >= j} s_l^2, accumulated from the smallest upwards
+    double c2 = 0.0;
+    for (int64_t j = n - 1; j >= 0; --j) c2 += sv[j] * sv[j];
+    const double tot = c2;
+    const double er = relative ? cutoff * sqrt(tot) : cutoff;
+    c2 = 0.0;
+    int64_t k = -1;
+    double disc_w = 0.0;
+    for (int64_t j = n - 1; j >= 0; --j) {
+      c2 += sv[j] * sv[j];
+      if (sqrt(c2) > er) {
+        k = j + 1;
+        break;
+      }
+    }
+    int st = 0;
+    if (k < 0) {
+      st = QTN_ERR_TRUNC;
+      k = 0;
+    }
+    if (chimax > 0 && k > chimax) k = chimax;
+    // discarded weight: sum of squares of s[k..n-1], smallest first
+    disc_w = 0.0;
+    for (int64_t j = n - 1; j >= k; --j) disc_w += sv[j] * sv[j];
+    double kw = 0.0;
+    for (int64_t j = k - 1; j >= 0; --j) kw += sv[j] * sv[j];
+    kept[bz] = k;
+    disc[bz] = disc_w;
+    total[bz] = tot;
+    keptw[bz] = kw;
+    status[bz] = st;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
+                              int64_t cols_p_) {
+  batch = batch_;
+  rows_p = rows_p_;
+  cols_p = cols_p_;
+  p = (int)(cols_p / 64);
+  const int64_t nchunk = std::max(rows_p, cols_p) / 64;
+  void* ptr;
+  QTN_TRY(ctx->buffer(kSlotGram,
+                      (size_t)batch * p * nchunk * 4096 * sizeof(double2),
+                      &ptr));
+  Gp = (double2*)ptr;
+  QTN_TRY(ctx->buffer(kSlotQ, (size_t)batch * p * 4096 * sizeof(double2),
+                      &ptr));
+  Q = (double2*)ptr;
+  // small per-problem arrays packed in one slot
+  const size_t small =
+      (size_t)batch * (p * sizeof(int) + 4 * sizeof(int) + 5 * sizeof(double) +
+                       2 * sizeof(int64_t)) +
+      1024;
+  QTN_TRY(ctx->buffer(kSlotSmall, small, &ptr));
+  char* c = (char*)ptr;
+  normF2 = (double*)c;
+  c += batch * sizeof(double);
+  zthr = (double*)c;
+  c += batch * sizeof(double);
+  disc = (double*)c;
+  c += batch * sizeof(double);
+  total = (double*)c;
+  c += batch * sizeof(double);
+  keptw = (double*)c;
+  c += batch * sizeof(double);
+  kept = (int64_t*)c;
+  c += batch * sizeof(int64_t);
+  nsv = (int64_t*)c;
+  c += batch * sizeof(int64_t);
+  flags = (int*)c;
+  c += (size_t)batch * p * sizeof(int);
+  counters = (int*)c;
+  c += batch * sizeof(int);
+  active = (int*)c;
+  c += batch * sizeof(int);
+  status = (int*)c;
+  c += batch * sizeof(int);
+  misc = (int*)c;  // [0] = n_active, [1] = nonfinite flag
+  QTN_TRY(ctx->buffer(kSlotSig, (size_t)batch * cols_p * sizeof(double),
+                      &ptr));
+  sig = (double*)ptr;
+  QTN_TRY(ctx->buffer(kSlotPerm, (size_t)batch * cols_p * sizeof(int), &ptr));
+  perm = (int*)ptr;
+  return QTN_OK;
+}
+
+int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t sX, double2* W,
+                          int64_t sW, const int64_t* nsv_host, double cutoff,
+                          int relative, int64_t chimax, int max_sweeps,
+                          int* sweeps_done) {
+  hipStream_t s = ctx->stream;
+  const double tol = std::sqrt((double)rows_p) * (DBL_EPSILON * 0.5);
+  const int nb = 2 * p;
+  const int rounds = nb - 1;
+  const int n_inner = (p == 1) ? 40 : 1;
+  const int nchunk_x = (int)(rows_p / 64);
+  const int nchunk_w = (int)(cols_p / 64);
+
+  QTN_CHECK_HIP(hipMemcpyAsync(nsv, nsv_host, batch * sizeof(int64_t),
+                               hipMemcpyHostToDevice, s));
+  QTN_CHECK_HIP(hipMemsetAsync(misc, 0, 2 * sizeof(int), s));
+  hipLaunchKernelGGL(jac_prepare, dim3(batch), dim3(256), 0, s, X,
+                     rows_p * cols_p, sX, normF2, zthr, active, counters,
+                     misc + 1);
+  QTN_CHECK_HIP(hipGetLastError());
+  if (W) {
+    // W starts as the identity on the padded columns.
+    QTN_TRY(launch_identity(s, W, cols_p, sW, batch));
+  }
+  int* host_flags = nullptr;
+  QTN_TRY(ctx->pinned_buffer(4 * sizeof(int), (void**)&host_flags));
+  int sweep = 0;
+  bool converged = false;
+  for (; sweep < max_sweeps; ++sweep) {
+    for (int r = 0; r < rounds; ++r) {
+      hipLaunchKernelGGL(jac_gram, dim3(nchunk_x, p, batch), dim3(256), 0, s,
+                         (const double2*)X, rows_p, sX, p, r, Gp, active);
+      hipLaunchKernelGGL(jac_inner, dim3(p, batch), dim3(256), 0, s,
+                         (const double2*)Gp, nchunk_x, p, tol, zthr, n_inner,
+                         Q, flags, counters, active);
+      hipLaunchKernelGGL(jac_update, dim3(nchunk_x, p, batch), dim3(256), 0, s,
+                         X, rows_p, sX, p, r, (const double2*)Q, flags,
+                         active);
+      if (W)
+        hipLaunchKernelGGL(jac_update, dim3(nchunk_w, p, batch), dim3(256), 0,
+                           s, W, cols_p, sW, p, r, (const double2*)Q, flags,
+                           active);
+    }
+    QTN_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(jac_sweep_end, dim3(1), dim3(256), 0, s, active,
+                       counters, batch, misc);
+    QTN_CHECK_HIP(hipMemcpyAsync(host_flags, misc, 2 * sizeof(int),
+                                 hipMemcpyDeviceToHost, s));
+    QTN_CHECK_HIP(hipStreamSynchronize(s));
+    if (host_flags[1]) {
+      set_error("svd: input contains NaN or Inf");
+      return QTN_ERR_NOTFINITE;
+    }
+    if (host_flags[0] == 0) {
+      converged = true;
+      ++sweep;
+      break;
+    }
+  }
+  if (sweeps_done) *sweeps_done = sweep;
+  if (!converged) {
+    set_error("svd: one-sided Jacobi did not converge in " +
+              std::to_string(max_sweeps) + " sweeps");
+    return QTN_ERR_NOCONV;
+  }
+  hipLaunchKernelGGL(jac_colnorms, dim3((unsigned)((cols_p + 3) / 4), batch),
+                     dim3(256), 0, s, (const double2*)X, rows_p, cols_p, sX,
+                     sig);
+  if (cols_p > kSortMax) {
+    set_error("svd: more than 4096 columns are not supported");
+    return QTN_ERR_ARG;
+  }
+  hipLaunchKernelGGL(jac_sort_trunc, dim3(batch), dim3(1024), 0, s, sig, perm,
+                     cols_p, (const int64_t*)nsv, cutoff, relative, chimax,
+                     kept, disc, total, keptw, status);
+  QTN_CHECK_HIP(hipGetLastError());
+  return QTN_OK;
+}
+
+namespace {
+__global__ void identity_kernel(double2* W, int64_t n, int64_t sW) {
+  const int64_t bz = blockIdx.y;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i % n, c = i / n;
+    W[bz * sW + i] = make_double2(r == c ? 1.0 : 0.0, 0.0);
+  }
+}
+}  // namespace
+
+int32_t launch_identity(hipStream_t s, double2* W, int64_t n, int64_t sW,
+                        int batch) {
+  const int64_t blocks = std::min<int64_t>((n * n + 255) / 256, 4096);
+  hipLaunchKernelGGL(identity_kernel, dim3((unsigned)blocks, batch),
+                     dim3(256), 0, s, W, n, sW);
+  QTN_CHECK_HIP(hipGetLastError());
+  return QTN_OK;
+}
+
+}  // namespace qtn

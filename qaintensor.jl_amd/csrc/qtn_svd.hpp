@@ -1,0 +1,56 @@
+// Batched block one-sided Jacobi SVD engine (device side lives in
+// qtn_svd.hip).  The caller owns X (and optionally W); the engine owns the
+// Gram partials, rotation blocks and the per-problem result arrays, all in
+// context scratch slots.
+#pragma once
+#include "qtn_internal.hpp"
+
+namespace qtn {
+
+// Context scratch slots used by the engine (callers use other slots).
+constexpr int kSlotGram = 20;
+constexpr int kSlotQ = 21;
+constexpr int kSlotSmall = 22;
+constexpr int kSlotSig = 23;
+constexpr int kSlotPerm = 24;
+
+struct JacobiEngine {
+  int batch = 0;
+  int64_t rows_p = 0;  // padded rows of X (multiple of 64)
+  int64_t cols_p = 0;  // padded columns of X (multiple of 64)
+  int p = 0;           // block pairs per round (= cols_p / 64)
+  // device arrays (valid after reserve)
+  double2* Gp = nullptr;
+  double2* Q = nullptr;
+  double* normF2 = nullptr;
+  double* zthr = nullptr;
+  double* disc = nullptr;   // discarded weight per problem
+  double* total = nullptr;  // sum of all sigma^2 per problem
+  double* keptw = nullptr;  // sum of the kept sigma^2 per problem
+  int64_t* kept = nullptr;
+  int64_t* nsv = nullptr;
+  int* flags = nullptr;
+  int* counters = nullptr;
+  int* active = nullptr;
+  int* status = nullptr;
+  int* misc = nullptr;
+  double* sig = nullptr;  // [batch][cols_p], sorted descending after run
+  int* perm = nullptr;    // [batch][cols_p], column of X for each sigma
+
+  int32_t reserve(qtn_ctx* ctx, int batch, int64_t rows_p, int64_t cols_p);
+  // Orthogonalise the columns of every X (rows_p x cols_p, ld rows_p,
+  // batch stride sX); if W is given (cols_p x cols_p, stride sW) it
+  // accumulates the rotations starting from the identity.  Afterwards sig /
+  // perm hold the sorted column norms and kept / disc / total / status the
+  // truncation decision for nsv_host[b] meaningful values.
+  int32_t run(qtn_ctx* ctx, double2* X, int64_t sX, double2* W, int64_t sW,
+              const int64_t* nsv_host, double cutoff, int relative,
+              int64_t chimax, int max_sweeps, int* sweeps_done);
+};
+
+int32_t launch_identity(hipStream_t s, double2* W, int64_t n, int64_t sW,
+                        int batch);
+
+inline int64_t round_up64(int64_t x) { return ((x + 63) / 64) * 64; }
+
+}  // namespace qtn

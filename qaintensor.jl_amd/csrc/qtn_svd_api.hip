@@ -1,0 +1,219 @@
+// Host-pointer truncated SVD entry points (qtn_svd_trunc, qtn_svd_values)
+// on top of the batched Jacobi engine.  They replace LinearAlgebra.svd plus
+// the truncation of contract_svd (src/svd.jl:26-33).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "qtn_internal.hpp"
+#include "qtn_svd.hpp"
+
+namespace qtn {
+namespace {
+
+// X (rows_p x cols_p, zero padded) from A (m x n, ld lda):
+// trans == 0: X = A; trans == 1: X = A^H.
+__global__ void build_x(const double2* __restrict__ A, int64_t m, int64_t n,
+                        int64_t lda, int trans, double2* __restrict__ X,
+                        int64_t rows_p, int64_t cols_p) {
+  const int64_t total = rows_p * cols_p;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+       idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = idx % rows_p, c = idx / rows_p;
+    double2 v = make_double2(0.0, 0.0);
+    if (!trans) {
+      if (r < m && c < n) v = A[r + c * lda];
+    } else {
+      if (r < n && c < m) {
+        const double2 a = A[c + r * lda];
+        v = make_double2(a.x, -a.y);
+      }
+    }
+    X[idx] = v;
+  }
+}
+
+// dst column j (j < k) = src column perm[j] (first `rows` rows) scaled by
+// 1/sig[j] when scale != 0.  Column-major dst with ld = rows.
+__global__ void gather_cols(const double2* __restrict__ src, int64_t ld_src,
+                            int64_t rows, const int* __restrict__ perm,
+                            const double* __restrict__ sig, int scale,
+                            int64_t k, double2* __restrict__ dst) {
+  const int64_t total = rows * k;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+       idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = idx % rows, j = idx / rows;
+    double2 v = src[r + (int64_t)perm[j] * ld_src];
+    if (scale) {
+      const double s = sig[j];
+      const double inv = s > 0.0 ? 1.0 / s : 0.0;
+      v.x *= inv;
+      v.y *= inv;
+    }
+    dst[idx] = v;
+  }
+}
+
+// dst (k x cols, ld k) row j = conj(src column perm[j])^T, optionally / sig.
+__global__ void gather_rows_conj(const double2* __restrict__ src,
+                                 int64_t ld_src, int64_t cols,
+                                 const int* __restrict__ perm,
+                                 const double* __restrict__ sig, int scale,
+                                 int64_t k, double2* __restrict__ dst) {
+  const int64_t total = cols * k;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+       idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = idx % k, c = idx / k;
+    double2 v = src[c + (int64_t)perm[j] * ld_src];
+    v.y = -v.y;
+    if (scale) {
+      const double s = sig[j];
+      const double inv = s > 0.0 ? 1.0 / s : 0.0;
+      v.x *= inv;
+      v.y *= inv;
+    }
+    dst[idx] = v;
+  }
+}
+
+unsigned grid_for(int64_t n) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256,
+                                                          8192));
+}
+
+constexpr int kSlotA = 11, kSlotX = 12, kSlotW = 13, kSlotU = 14, kSlotV = 15;
+
+struct HostSvdOut {
+  int64_t k = 0;
+  double disc = 0.0;
+  std::vector<double> sig;
+};
+
+int32_t svd_common(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
+                   int64_t lda, double cutoff, int relative, int64_t chimax,
+                   bool want_vectors, int64_t* k_out, double* U, double* S,
+                   double* Vh, double* disc_out, bool values_only) {
+  hipStream_t s = ctx->stream;
+  const int trans = (m < n) ? 1 : 0;  // columns of X = min(m, n)
+  const int64_t xr = trans ? n : m;
+  const int64_t xc = trans ? m : n;
+  const int64_t rows_p = round_up64(xr), cols_p = round_up64(xc);
+  if (cols_p > 4096) {
+    set_error("svd: min(m, n) above 4096 is not supported");
+    return QTN_ERR_ARG;
+  }
+  void *dA, *dX, *dW = nullptr;
+  QTN_TRY(ctx->buffer(kSlotA, (size_t)m * n * sizeof(double2), &dA));
+  QTN_TRY(ctx->buffer(kSlotX, (size_t)rows_p * cols_p * sizeof(double2),
+                      &dX));
+  if (want_vectors)
+    QTN_TRY(ctx->buffer(kSlotW, (size_t)cols_p * cols_p * sizeof(double2),
+                        &dW));
+  QTN_CHECK_HIP(hipMemcpy2DAsync(dA, m * sizeof(double2), A,
+                                 lda * sizeof(double2), m * sizeof(double2), n,
+                                 hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(build_x, dim3(grid_for(rows_p * cols_p)), dim3(256), 0,
+                     s, (const double2*)dA, m, n, m, trans, (double2*)dX,
+                     rows_p, cols_p);
+  QTN_CHECK_HIP(hipGetLastError());
+  JacobiEngine eng;
+  QTN_TRY(eng.reserve(ctx, 1, rows_p, cols_p));
+  const int64_t nsv = std::min(m, n);
+  int sweeps = 0;
+  QTN_TRY(eng.run(ctx, (double2*)dX, rows_p * cols_p, (double2*)dW,
+                  cols_p * cols_p, &nsv, cutoff, relative, chimax, 60,
+                  &sweeps));
+  // read back the decision
+  int64_t k = 0;
+  int st = 0;
+  double disc = 0.0;
+  QTN_CHECK_HIP(hipMemcpyAsync(&k, eng.kept, sizeof(int64_t),
+                               hipMemcpyDeviceToHost, s));
+  QTN_CHECK_HIP(hipMemcpyAsync(&st, eng.status, sizeof(int),
+                               hipMemcpyDeviceToHost, s));
+  QTN_CHECK_HIP(hipMemcpyAsync(&disc, eng.disc, sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+  QTN_CHECK_HIP(hipStreamSynchronize(s));
+  if (values_only) {
+    QTN_CHECK_HIP(hipMemcpy(S, eng.sig, nsv * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    return QTN_OK;
+  }
+  if (st != 0) {
+    set_error("svd: every singular value is within the truncation error");
+    return QTN_ERR_TRUNC;
+  }
+  if (k_out) *k_out = k;
+  if (disc_out) *disc_out = disc;
+  if (S && k > 0)
+    QTN_CHECK_HIP(hipMemcpy(S, eng.sig, k * sizeof(double),
+                            hipMemcpyDeviceToHost));
+  if (k == 0) return QTN_OK;
+  void *dU = nullptr, *dV = nullptr;
+  if (U) QTN_TRY(ctx->buffer(kSlotU, (size_t)m * k * sizeof(double2), &dU));
+  if (Vh) QTN_TRY(ctx->buffer(kSlotV, (size_t)k * n * sizeof(double2), &dV));
+  if (!trans) {
+    // X = A: A W = U diag(s), V = W
+    if (U)
+      hipLaunchKernelGGL(gather_cols, dim3(grid_for(m * k)), dim3(256), 0, s,
+                         (const double2*)dX, rows_p, m, eng.perm, eng.sig, 1,
+                         k, (double2*)dU);
+    if (Vh)
+      hipLaunchKernelGGL(gather_rows_conj, dim3(grid_for(n * k)), dim3(256),
+                         0, s, (const double2*)dW, cols_p, n, eng.perm,
+                         eng.sig, 0, k, (double2*)dV);
+  } else {
+    // X = A^H: A^H W = V diag(s), U = W
+    if (U)
+      hipLaunchKernelGGL(gather_cols, dim3(grid_for(m * k)), dim3(256), 0, s,
+                         (const double2*)dW, cols_p, m, eng.perm, eng.sig, 0,
+                         k, (double2*)dU);
+    if (Vh)
+      hipLaunchKernelGGL(gather_rows_conj, dim3(grid_for(n * k)), dim3(256),
+                         0, s, (const double2*)dX, rows_p, n, eng.perm,
+                         eng.sig, 1, k, (double2*)dV);
+  }
+  QTN_CHECK_HIP(hipGetLastError());
+  if (U)
+    QTN_CHECK_HIP(hipMemcpyAsync(U, dU, (size_t)m * k * sizeof(double2),
+                                 hipMemcpyDeviceToHost, s));
+  if (Vh)
+    QTN_CHECK_HIP(hipMemcpyAsync(Vh, dV, (size_t)k * n * sizeof(double2),
+                                 hipMemcpyDeviceToHost, s));
+  QTN_CHECK_HIP(hipStreamSynchronize(s));
+  return QTN_OK;
+}
+
+}  // namespace
+}  // namespace qtn
+
+extern "C" {
+
+int32_t qtn_svd_trunc(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
+                      int64_t lda, double cutoff, int32_t relative,
+                      int64_t chimax, int64_t* k, double* U, double* S,
+                      double* Vh, double* disc_weight) {
+  qtn::clear_error();
+  QTN_ARG(ctx != nullptr && A != nullptr, "qtn_svd_trunc: NULL argument");
+  QTN_ARG(m >= 1 && n >= 1, "qtn_svd_trunc: empty matrix");
+  QTN_ARG(lda >= m, "qtn_svd_trunc: lda < m");
+  QTN_ARG(cutoff >= 0.0, "Error must be positive");
+  QTN_CHECK_HIP(hipSetDevice(ctx->device));
+  return qtn::svd_common(ctx, m, n, A, lda, cutoff, relative, chimax,
+                         U != nullptr || Vh != nullptr, k, U, S, Vh,
+                         disc_weight, false);
+}
+
+int32_t qtn_svd_values(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
+                       int64_t lda, double* S) {
+  qtn::clear_error();
+  QTN_ARG(ctx != nullptr && A != nullptr && S != nullptr,
+          "qtn_svd_values: NULL argument");
+  QTN_ARG(m >= 1 && n >= 1, "qtn_svd_values: empty matrix");
+  QTN_ARG(lda >= m, "qtn_svd_values: lda < m");
+  QTN_CHECK_HIP(hipSetDevice(ctx->device));
+  return qtn::svd_common(ctx, m, n, A, lda, 0.0, 0, 0, false, nullptr,
+                         nullptr, S, nullptr, nullptr, true);
+}
+
+}  // extern "C"

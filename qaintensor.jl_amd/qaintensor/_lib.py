@@ -1,0 +1,214 @@
+"""ctypes binding of libqtn (the gfx950 C-ABI declared in include/qtn.h).
+
+The library is loaded from this package directory.  There is no fallback:
+if libqtn.so is missing or cannot be loaded, every GPU entry point raises
+QtnLibraryError.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqtn.so")
+
+QTN_OK = 0
+QTN_ERR_ARG = -1
+QTN_ERR_HIP = -2
+QTN_ERR_OOM = -3
+QTN_ERR_NOCONV = -4
+QTN_ERR_TRUNC = -5
+QTN_ERR_NOTFINITE = -6
+
+QTN_FORM_PLAIN = 0
+QTN_FORM_CHOSEN = 1
+
+# Every symbol include/qtn.h declares (checked by the CPU test suite).
+EXPORTS = [
+    "qtn_ctx_create", "qtn_ctx_destroy", "qtn_ctx_sync", "qtn_ctx_stream",
+    "qtn_last_error", "qtn_version",
+    "qtn_contract_pair", "qtn_contract_pair_dev", "qtn_permute",
+    "qtn_permute_dev", "qtn_zgemm_dev", "qtn_cgemm_dev",
+    "qtn_svd_trunc", "qtn_svd_values",
+    "qtn_mps_create", "qtn_mps_destroy", "qtn_mps_copy", "qtn_mps_set_sites",
+    "qtn_mps_bond_dims", "qtn_mps_bond_spectrum", "qtn_mps_get_site",
+    "qtn_mps_apply_1q", "qtn_mps_apply_2q", "qtn_mps_apply_layer",
+    "qtn_mps_amplitudes", "qtn_mps_expect_z", "qtn_mps_overlap",
+    "qtn_mps_norm2",
+]
+
+
+class QtnLibraryError(RuntimeError):
+    """libqtn.so is missing or failed to load."""
+
+
+class QtnError(RuntimeError):
+    """A libqtn call returned a non-zero status."""
+
+    def __init__(self, status, message):
+        super().__init__("%s (status %d)" % (message, status))
+        self.status = status
+        self.message = message
+
+
+_lib = None
+_lock = threading.Lock()
+
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_dbl = ctypes.c_double
+_vp = ctypes.c_void_p
+_pi64 = ctypes.POINTER(ctypes.c_int64)
+_pi32 = ctypes.POINTER(ctypes.c_int32)
+_pd = ctypes.POINTER(ctypes.c_double)
+_pu8 = ctypes.POINTER(ctypes.c_uint8)
+
+
+def _declare(lib):
+    sig = {
+        "qtn_ctx_create": (_i32, [_i32, ctypes.POINTER(_vp)]),
+        "qtn_ctx_destroy": (_i32, [_vp]),
+        "qtn_ctx_sync": (_i32, [_vp]),
+        "qtn_ctx_stream": (_i32, [_vp, ctypes.POINTER(_vp)]),
+        "qtn_last_error": (ctypes.c_char_p, []),
+        "qtn_version": (ctypes.c_char_p, []),
+        "qtn_contract_pair": (_i32, [_vp, _pd, _i32, _pi64, _pd, _i32, _pi64,
+                                     _i32, _pi32, _pi32, _pi32, _pd]),
+        "qtn_contract_pair_dev": (_i32, [_vp, _vp, _i32, _pi64, _vp, _i32,
+                                         _pi64, _i32, _pi32, _pi32, _pi32,
+                                         _vp]),
+        "qtn_permute": (_i32, [_vp, _pd, _i32, _pi64, _pi32, _pd]),
+        "qtn_permute_dev": (_i32, [_vp, _vp, _i32, _pi64, _pi32, _vp]),
+        "qtn_zgemm_dev": (_i32, [_vp, ctypes.c_char, ctypes.c_char, _i64,
+                                 _i64, _i64, _pd, _vp, _i64, _i64, _vp, _i64,
+                                 _i64, _pd, _vp, _i64, _i64, _i64]),
+        "qtn_cgemm_dev": (_i32, [_vp, ctypes.c_char, ctypes.c_char, _i64,
+                                 _i64, _i64, ctypes.POINTER(ctypes.c_float),
+                                 _vp, _i64, _i64, _vp, _i64, _i64,
+                                 ctypes.POINTER(ctypes.c_float), _vp, _i64,
+                                 _i64, _i64]),
+        "qtn_svd_trunc": (_i32, [_vp, _i64, _i64, _pd, _i64, _dbl, _i32, _i64,
+                                 _pi64, _pd, _pd, _pd, _pd]),
+        "qtn_svd_values": (_i32, [_vp, _i64, _i64, _pd, _i64, _pd]),
+        "qtn_mps_create": (_i32, [_vp, _i32, _i64, ctypes.POINTER(_vp)]),
+        "qtn_mps_destroy": (_i32, [_vp]),
+        "qtn_mps_copy": (_i32, [_vp, _vp]),
+        "qtn_mps_set_sites": (_i32, [_vp, _pi64, ctypes.POINTER(_pd)]),
+        "qtn_mps_bond_dims": (_i32, [_vp, _pi64]),
+        "qtn_mps_bond_spectrum": (_i32, [_vp, _i32, _pd, _pi64]),
+        "qtn_mps_get_site": (_i32, [_vp, _i32, _pd, _pi64]),
+        "qtn_mps_apply_1q": (_i32, [_vp, _i32, _pd]),
+        "qtn_mps_apply_2q": (_i32, [_vp, _i32, _pd, _dbl, _i64, _i32, _pi64,
+                                    _pd, _pd, _pi64]),
+        "qtn_mps_apply_layer": (_i32, [_vp, _i32, _pi32, _pd, _dbl, _i64,
+                                       _i32, _pi64, _pd]),
+        "qtn_mps_amplitudes": (_i32, [_vp, _i64, _pu8, _pd]),
+        "qtn_mps_expect_z": (_i32, [_vp, _pd]),
+        "qtn_mps_overlap": (_i32, [_vp, _vp, _pd]),
+        "qtn_mps_norm2": (_i32, [_vp, _pd]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def lib_path():
+    return LIB_PATH
+
+
+def load():
+    """Load libqtn.so (once).  Raises QtnLibraryError if it is absent."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise QtnLibraryError(
+                "libqtn.so not found at %s; build it with "
+                "`python qaintensor.jl_amd/build_lib.py` (or "
+                "__graft_entry__.build())" % LIB_PATH)
+        try:
+            lib = ctypes.CDLL(LIB_PATH)
+        except OSError as e:
+            raise QtnLibraryError("cannot load %s: %s" % (LIB_PATH, e))
+        _declare(lib)
+        _lib = lib
+        return lib
+
+
+def check(status):
+    """Raise QtnError for a non-zero libqtn status."""
+    if status != QTN_OK:
+        msg = load().qtn_last_error().decode("utf-8", "replace")
+        raise QtnError(status, msg)
+
+
+def dptr(a):
+    """Pointer to a C-contiguous numpy array as double*."""
+    return a.ctypes.data_as(_pd)
+
+
+def i64ptr(a):
+    return a.ctypes.data_as(_pi64)
+
+
+def i32ptr(a):
+    return a.ctypes.data_as(_pi32)
+
+
+def as_c128(a, order="F"):
+    """complex128 array in the requested memory order (Julia = 'F')."""
+    return np.asarray(a, dtype=np.complex128, order=order)
+
+
+class Context:
+    """Owns one qtn_ctx (device, stream, scratch)."""
+
+    def __init__(self, device=0):
+        lib = load()
+        h = _vp()
+        check(lib.qtn_ctx_create(int(device), ctypes.byref(h)))
+        self._h = h
+        self.device = device
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise QtnError(QTN_ERR_ARG, "context already destroyed")
+        return self._h
+
+    def sync(self):
+        check(load().qtn_ctx_sync(self.handle))
+
+    def stream(self):
+        s = _vp()
+        check(load().qtn_ctx_stream(self.handle, ctypes.byref(s)))
+        return s.value
+
+    def close(self):
+        if self._h is not None:
+            check(load().qtn_ctx_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            if self._h is not None and _lib is not None:
+                _lib.qtn_ctx_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    """Process-wide context on device 0 (created on first use)."""
+    global _default_ctx
+    with _lock:
+        pass
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
