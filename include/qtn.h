@@ -61,6 +61,12 @@ int32_t qtn_ctx_sync(qtn_ctx* ctx);
 /* Returns the context's hipStream_t (as void*) so callers can time with
  * events on the stream the kernels run on.                                 */
 int32_t qtn_ctx_stream(qtn_ctx* ctx, void** stream);
+/* Work counters of the context since creation (or the last reset):
+ * out[0] SVD problems solved, out[1] Jacobi sweeps summed over problems,
+ * out[2] largest sweep count of one problem, out[3] batched SVD launches.
+ * n = number of slots to write (<= 4).  reset != 0 zeroes them after.     */
+int32_t qtn_ctx_counters(qtn_ctx* ctx, int64_t* out, int32_t n,
+                         int32_t reset);
 /* Message of the last failing call on this thread ("" if none).            */
 const char* qtn_last_error(void);
 /* Library version string. */

@@ -1,0 +1,450 @@
+"""FP64 CPU oracle for the contraction & truncation hot path.
+
+THIS IS TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py may import it, and only as the checker (or the
+timed CPU baseline) -- never as the product path.  The product is libqtn
+(qaintensor.jl_amd/csrc) driven through the qaintensor host package.
+
+It restates, in numpy on LAPACK (zgesdd through numpy.linalg.svd, the same
+driver Julia's default svd calls), the semantics of Qaintensor.jl:
+  - truncation rule            src/svd.jl:29-33
+  - contract_svd               src/svd.jl:7-38
+  - ncon labelling of contract src/contract.jl:39-60, 242-264
+  - MPS(psi) SVD sweep         src/mps.jl:55-89
+  - switch! two-site split     src/switch.jl:18-56
+  - MPO(m) / decompose!        src/mpo.jl:27-90, src/decompose.jl:6-52
+  - extend_MPO / apply_MPO     src/mpo.jl:122-157, 184-252
+and the build's new operations that the reference only has as primitives
+(two-site gate application with cutoff + chi_max truncation, amplitudes,
+<Z>, overlaps), in the right-canonical Hastings gauge used on the GPU.
+
+Index convention: arrays use the reference's (Julia) index order; Julia's
+column-major reshape is reproduced with order='F'.  Wire w <-> bit w-1 of
+the state index (least significant first); 2-qubit gate matrices act on
+index s_w + 2 s_{w+1} (src/mpo.jl:199-214: apply_MPO contracts the state
+with the gate's column legs).
+
+Pinning status of each function is listed in PINNING below; every entry is
+exercised by tests/test_oracle.py.
+"""
+import numpy as np
+
+PINNING = {
+    "truncation_rank": "KAT s=exp(-(0:99)), er=1e-10 -> k=24 "
+                       "(test/test_svd.jl:53-83); edge cases er=0 / all<=er",
+    "truncate": "relative-er wrapper of truncation_rank; brute force over "
+                "all k on small spectra",
+    "contract_svd": "closed form t1*t2 at er=0 (test/test_svd.jl:7-11); "
+                    "equals einsum on 4-leg tensors (test_svd.jl:13-25); "
+                    "error bound of test_svd.jl:75-81",
+    "ncon": "einsum definition; brute-force loops on tiny tensors",
+    "mps_from_state": "round trip MPS(psi) -> contract == psi "
+                      "(test/test_mps.jl:34-49)",
+    "mpo_from_matrix": "contract(MPO(U)) == U as a (2,..,2) tensor; "
+                       "operator-Schmidt closed forms (SWAP, CNOT)",
+    "apply_mpo_dense": "dense kron/matvec (test/test_mpo.jl:62-121)",
+    "TebdState": "untruncated runs equal the dense statevector simulator; "
+                 "product-state closed forms",
+    "StateVector": "closed forms: X/H/CNOT on basis states, unitarity",
+}
+
+
+# ---------------------------------------------------------------------------
+# truncation (src/svd.jl:29-33)
+class TruncationError(ValueError):
+    """Every singular value lies within the truncation error (the reference
+    errors here: findfirst returns nothing)."""
+
+
+def truncation_rank(s, er):
+    """Number of leading singular values kept by contract_svd's rule.
+
+    c = sqrt.(cumsum(reverse(S.^2))); k = length(S) - findfirst(c .> er) + 1
+    (src/svd.jl:29-32).  s must be sorted descending.
+    """
+    s = np.asarray(s, dtype=np.float64)
+    c = np.sqrt(np.cumsum(s[::-1] ** 2))
+    hit = np.nonzero(c > er)[0]
+    if hit.size == 0:
+        raise TruncationError("every singular value is within er")
+    k_rev = int(hit[0]) + 1
+    return s.size - k_rev + 1
+
+
+def truncate(s, cutoff, relative=True, chimax=0):
+    """(k, discarded weight, total weight) for the build's truncation.
+
+    er = cutoff * ||s||_2 when relative (||s||^2 summed from the smallest
+    value up, the same order as the tail sums), else er = cutoff; then
+    k = min(k, chimax) when chimax > 0.  The discarded weight is the sum of
+    the squares of the dropped values, again from the smallest up.
+    """
+    s = np.asarray(s, dtype=np.float64)
+    if cutoff < 0:
+        raise ValueError("Error must be positive")
+    cs = np.cumsum(s[::-1] ** 2)
+    total = float(cs[-1]) if s.size else 0.0
+    er = cutoff * np.sqrt(total) if relative else cutoff
+    k = truncation_rank(s, er)
+    if chimax and chimax > 0:
+        k = min(k, int(chimax))
+    n = s.size
+    disc = float(cs[n - k - 1]) if k < n else 0.0
+    return k, disc, total
+
+
+# ---------------------------------------------------------------------------
+# contraction semantics
+def ncon(tensors, indexlists):
+    """Full contraction with TensorOperations.ncon labelling as produced by
+    contract_rep (src/contract.jl:39-60): positive labels are summed,
+    negative labels -1, -2, ... give the output axes in that order.
+    Evaluated by einsum (a closed form of the definition)."""
+    letters = {}
+    alphabet = [chr(c) for c in range(ord("a"), ord("z") + 1)] + \
+        [chr(c) for c in range(ord("A"), ord("Z") + 1)]
+
+    def let(lbl):
+        if lbl not in letters:
+            letters[lbl] = alphabet[len(letters)]
+        return letters[lbl]
+
+    subs = []
+    for lst in indexlists:
+        subs.append("".join(let(l) for l in lst))
+    outs = sorted({l for lst in indexlists for l in lst if l < 0},
+                  reverse=True)
+    out = "".join(let(l) for l in outs)
+    expr = ",".join(subs) + "->" + out
+    return np.einsum(expr, *[np.asarray(t) for t in tensors], optimize=True)
+
+
+def contract_network(tensors, contractions, openidx):
+    """contract(net) of src/contract.jl:242-264 (non-optimised order).
+
+    contractions: list of lists of (tensor, leg) pairs, 1-based;
+    openidx: list of (tensor, leg), 1-based.  Output axes follow openidx.
+    """
+    if len(tensors) == 1:
+        perm = [leg - 1 for (_, leg) in openidx]
+        return np.transpose(np.asarray(tensors[0]), perm)
+    index = [[0] * np.ndim(t) for t in tensors]
+    for i, summ in enumerate(contractions, start=1):
+        for (t, leg) in summ:
+            index[t - 1][leg - 1] = i
+    J = len(contractions)
+    for i, (t, leg) in enumerate(openidx, start=1):
+        assert index[t - 1][leg - 1] == 0
+        index[t - 1][leg - 1] = -i - J
+    for lst in index:
+        assert all(l != 0 for l in lst)
+    # relabel open legs to -1, -2, ... preserving order
+    return ncon(tensors, [[(l + J if l < 0 else l) for l in lst]
+                          for lst in index])
+
+
+def contract_pair(A, B, legsA, legsB, outperm=None):
+    """One pairwise ncon step: contract legsA of A with legsB of B (1-based);
+    natural output = free legs of A then free legs of B; then
+    permutedims(natural, outperm)."""
+    la = [l - 1 for l in legsA]
+    lb = [l - 1 for l in legsB]
+    C = np.tensordot(np.asarray(A), np.asarray(B), axes=(la, lb))
+    if outperm is not None:
+        C = np.transpose(C, [p - 1 for p in outperm])
+    return C
+
+
+def contract_svd(T1, T2, indx, er=0.0):
+    """src/svd.jl:7-38: SVD both operands, truncate each with the absolute
+    tail-norm rule, multiply back."""
+    if er < 0:
+        raise ValueError("Error must be positive")
+    i1, i2 = indx
+    T1 = np.asarray(T1)
+    T2 = np.asarray(T2)
+    n1, n2 = T1.ndim, T2.ndim
+    # Julia's size(A, d) is 1 for d > ndims(A)
+    D1 = T1.shape[i1 - 1] if i1 <= n1 else 1
+    D2 = T2.shape[i2 - 1] if i2 <= n2 else 1
+    if D1 != D2:
+        raise ValueError("Dimensions of contraction legs do not match")
+    s1, s2 = T1.shape, T2.shape
+    newdim = list(s1[:i1 - 1]) + list(s1[i1:]) + list(s2[:i2 - 1]) + \
+        list(s2[i2:])
+    p1 = list(range(i1 - 1)) + list(range(i1, n1)) + [i1 - 1]
+    p2 = [i2 - 1] + list(range(i2 - 1)) + list(range(i2, n2))
+    T1p = np.transpose(T1, p1).reshape((-1, D1), order="F")
+    T2p = np.transpose(T2, p2).reshape((D1, -1), order="F")
+    U1, S1, V1h = np.linalg.svd(T1p, full_matrices=False)
+    U2, S2, V2h = np.linalg.svd(T2p, full_matrices=False)
+    k1 = truncation_rank(S1, er)
+    k2 = truncation_rank(S2, er)
+    T = (U1[:, :k1] * S1[:k1]) @ V1h[:k1] @ (U2[:, :k2] * S2[:k2]) @ V2h[:k2]
+    return T.reshape(newdim, order="F")
+
+
+# ---------------------------------------------------------------------------
+# MPS(psi) and MPO(m) (exact SVD sweeps)
+def mps_from_state(psi):
+    """src/mps.jl:55-89: statevector -> list of site tensors (first (2,chi),
+    middle (chi,2,chi'), last (chi,2)), no truncation."""
+    psi = np.asarray(psi, dtype=np.complex128)
+    n = psi.size
+    if n == 0 or (n & (n - 1)) != 0:
+        raise ValueError("Input state must have length 2^N")
+    M = int(np.log2(n))
+    tensors = []
+    m = psi.reshape((2, 2 ** (M - 1)), order="F")
+    U, S, Vh = np.linalg.svd(m, full_matrices=False)
+    tensors.append(U)
+    lbond = S.size
+    m = S[:, None] * Vh
+    for bit in range(2, M):
+        m = m.reshape((lbond * 2, 2 ** (M - bit)), order="F")
+        U, S, Vh = np.linalg.svd(m, full_matrices=False)
+        rbond = S.size
+        m = S[:, None] * Vh
+        tensors.append(U.reshape((lbond, 2, rbond), order="F"))
+        lbond = rbond
+    tensors.append(m)
+    return tensors
+
+
+def mps_to_state(tensors):
+    """Contract a closed MPS (first (2,chi), last (chi,2)) to the vector."""
+    v = np.asarray(tensors[0])  # (2, chi)
+    for t in tensors[1:-1]:
+        # v: (2^k, chi) ; t: (chi, 2, chi')
+        v = np.einsum("xa,asb->xsb", v, t)
+        v = v.reshape((-1, t.shape[2]), order="F")
+    v = np.einsum("xa,as->xs", v, np.asarray(tensors[-1]))
+    return v.reshape(-1, order="F")
+
+
+def mpo_from_matrix(m):
+    """src/mpo.jl:27-90: operator-Schmidt / TT split of a 2^M x 2^M matrix.
+    Returns (tensors, singular values of every split).  Tensors are
+    (row, col, chi), (chi, row, col, chi), ..., (chi, row, col)."""
+    m = np.asarray(m, dtype=np.complex128)
+    M = int(round(np.log2(m.shape[0])))
+    if M < 1:
+        raise ValueError("Need at least one qubit to act on.")
+    if M == 1:
+        return [m.reshape((2, 2), order="F")], []
+    t = m.reshape((2,) * (2 * M), order="F")
+    dims = []
+    for i in range(M):
+        dims += [i, i + M]
+    t = np.transpose(t, dims)
+    bond = 1
+    mm = t.reshape((4 * bond, -1), order="F")
+    U, S, Vh = np.linalg.svd(mm, full_matrices=False)
+    spectra = [S]
+    bond = S.size
+    mm = S[:, None] * Vh
+    out = [U.reshape((2, 2, bond), order="F")]
+    for i in range(2, M):
+        mm = mm.reshape((bond * 4, -1), order="F")
+        U, S, Vh = np.linalg.svd(mm, full_matrices=False)
+        spectra.append(S)
+        nb = S.size
+        mm = S[:, None] * Vh
+        out.append(U.reshape((bond, 2, 2, nb), order="F"))
+        bond = nb
+    out.append(mm.reshape((bond, 2, 2), order="F"))
+    return out, spectra
+
+
+def operator_schmidt(g):
+    """Singular values of the first split of MPO(g) for a 4x4 gate g (the
+    operator-Schmidt coefficients, src/decompose.jl:16-29)."""
+    _, spectra = mpo_from_matrix(g)
+    return spectra[0]
+
+
+def apply_mpo_dense(psi, m, iwire):
+    """Dense apply_MPO(psi, m, iwire) (src/mpo.jl:228-252): the M-qubit
+    matrix m acts on wires iwire (1-based, any order); gate-matrix bit k-1
+    is wire iwire[k]."""
+    psi = np.asarray(psi, dtype=np.complex128)
+    n = int(round(np.log2(psi.size)))
+    M = len(iwire)
+    t = psi.reshape((2,) * n, order="F")  # axis w-1 = wire w
+    g = np.asarray(m).reshape((2,) * (2 * M), order="F")
+    # g axes: out bits 0..M-1 then in bits 0..M-1
+    in_axes = [w - 1 for w in iwire]
+    out = np.tensordot(g, t, axes=(list(range(M, 2 * M)), in_axes))
+    # out axes: gate out bits (M) then remaining state axes in order
+    rest = [a for a in range(n) if a not in in_axes]
+    order = [0] * n
+    for k, w in enumerate(in_axes):
+        order[w] = k
+    for j, a in enumerate(rest):
+        order[a] = M + j
+    out = np.transpose(out, order)
+    return out.reshape(-1, order="F")
+
+
+# ---------------------------------------------------------------------------
+# dense statevector (independent check of the MPS path)
+class StateVector:
+    def __init__(self, n):
+        self.n = n
+        self.psi = np.zeros(2 ** n, dtype=np.complex128)
+        self.psi[0] = 1.0
+
+    def apply_1q(self, w, u):
+        self.psi = apply_mpo_dense(self.psi, u, (w,))
+
+    def apply_2q(self, w, g):
+        self.psi = apply_mpo_dense(self.psi, g, (w, w + 1))
+
+    def amplitudes(self, bits):
+        bits = np.asarray(bits, dtype=np.int64)
+        idx = (bits * (1 << np.arange(self.n))).sum(axis=1)
+        return self.psi[idx]
+
+    def expect_z(self):
+        p = np.abs(self.psi) ** 2
+        idx = np.arange(p.size)
+        return np.array([np.sum(p * (1 - 2 * ((idx >> w) & 1)))
+                         for w in range(self.n)]) / p.sum()
+
+
+# ---------------------------------------------------------------------------
+# TEBD in the right-canonical (Hastings) gauge
+class TebdState:
+    """|0...0> of n qubits; B[j] has shape (chi_{j-1}, 2, chi_j) (0-based
+    list index j-1), lam[j] is the spectrum of bond j (lam[0] = lam[n] =
+    [1]).  Same algebra as libqtn's qtn_mps_apply_2q."""
+
+    def __init__(self, n):
+        self.n = n
+        self.B = []
+        for _ in range(n):
+            t = np.zeros((1, 2, 1), dtype=np.complex128)
+            t[0, 0, 0] = 1.0
+            self.B.append(t)
+        self.lam = [np.ones(1) for _ in range(n + 1)]
+
+    def copy(self):
+        c = TebdState.__new__(TebdState)
+        c.n = self.n
+        c.B = [b.copy() for b in self.B]
+        c.lam = [l.copy() for l in self.lam]
+        return c
+
+    def bond_dims(self):
+        return [1] + [b.shape[2] for b in self.B]
+
+    def apply_1q(self, w, u):
+        """u acts on the physical leg of site w (1-based)."""
+        self.B[w - 1] = np.einsum("uv,avb->aub", np.asarray(u),
+                                  self.B[w - 1])
+
+    def theta(self, i, g):
+        """(Phi, theta) matrices for gate g on sites (i, i+1)."""
+        Bl, Br = self.B[i - 1], self.B[i]
+        chiL, chiM, chiR = Bl.shape[0], Bl.shape[2], Br.shape[2]
+        t0 = Bl.reshape((2 * chiL, chiM), order="F") @ \
+            Br.reshape((chiM, 2 * chiR), order="F")
+        t4 = t0.reshape((chiL, 2, 2, chiR), order="F")
+        G = np.asarray(g).reshape((2, 2, 2, 2), order="F")  # [s',t',s,t]
+        p4 = np.einsum("uvst,astb->auvb", G, t4)
+        phi = p4.reshape((2 * chiL, 2 * chiR), order="F")
+        lamL = np.repeat(self.lam[i - 1][None, :], 2, axis=0).reshape(-1)
+        theta = lamL[:, None] * phi
+        return phi, theta
+
+    def apply_2q(self, i, g, cutoff, chimax, svd=None):
+        """Gate g (4x4, index s_i + 2 s_{i+1}) on sites (i, i+1).
+        Returns (k, discarded weight / total, full spectrum)."""
+        Bl, Br = self.B[i - 1], self.B[i]
+        chiL, chiR = Bl.shape[0], Br.shape[2]
+        phi, theta = self.theta(i, g)
+        if svd is None:
+            _, S, Vh = np.linalg.svd(theta, full_matrices=False)
+        else:
+            _, S, Vh = svd(theta)
+        k, disc, total = truncate(S, cutoff, True, chimax)
+        kw = float(np.sum(S[:k][::-1] ** 2))
+        nrm = np.sqrt(kw)
+        Vk = Vh[:k].conj().T
+        self.B[i] = Vh[:k].reshape((k, 2, chiR), order="F")
+        self.B[i - 1] = ((phi @ Vk) / nrm).reshape((chiL, 2, k), order="F")
+        self.lam[i] = S[:k] / nrm
+        return k, (disc / total if total > 0 else 0.0), S
+
+    def amplitudes(self, bits):
+        bits = np.asarray(bits, dtype=np.int64)
+        nb = bits.shape[0]
+        v = np.ones((nb, 1), dtype=np.complex128)
+        for j, B in enumerate(self.B):
+            s = bits[:, j]
+            v = np.einsum("ia,iab->ib", v, B[:, s, :].transpose(1, 0, 2))
+        return v[:, 0]
+
+    def _right_envs(self):
+        # R_{j-1}[a, d] = sum_{s,b,c} B[a,s,b] R_j[b,c] conj(B[d,s,c])
+        R = [None] * (self.n + 1)
+        R[self.n] = np.ones((1, 1), dtype=np.complex128)
+        for j in range(self.n, 0, -1):
+            B = self.B[j - 1]
+            cl, cr = B.shape[0], B.shape[2]
+            T = (B.reshape((2 * cl, cr), order="F") @ R[j]).reshape(
+                (cl, 2 * cr), order="F")
+            R[j - 1] = T @ B.reshape((cl, 2 * cr), order="F").conj().T
+        return R
+
+    def expect_z(self):
+        R = self._right_envs()
+        E = np.ones((1, 1), dtype=np.complex128)
+        out = np.zeros(self.n)
+        for j in range(1, self.n + 1):
+            B = self.B[j - 1]
+            cl, cr = B.shape[0], B.shape[2]
+            # T[a, s, b] = sum_d E[a, d] B[d, s, b]
+            T = np.stack([E @ B[:, 0, :], E @ B[:, 1, :]], axis=1)
+            Y = T.reshape((2 * cl, cr), order="F") @ R[j]
+            Bm = B.reshape((2 * cl, cr), order="F").conj()
+            prod = Bm * Y
+            num = prod[:cl].sum() - prod[cl:].sum()
+            nrm = prod.sum()
+            out[j - 1] = (num / nrm).real
+            E = Bm.T @ T.reshape((2 * cl, cr), order="F")
+        return out
+
+    def overlap(self, other):
+        """<self|other>."""
+        E = np.ones((1, 1), dtype=np.complex128)
+        for A, B in zip(self.B, other.B):
+            al, ar = A.shape[0], A.shape[2]
+            bl, br = B.shape[0], B.shape[2]
+            T = np.concatenate([E @ B[:, 0, :], E @ B[:, 1, :]], axis=0)
+            E = A.reshape((2 * al, ar), order="F").conj().T @ T
+        return complex(E[0, 0])
+
+    def norm2(self):
+        return self.overlap(self).real
+
+    def to_statevector(self):
+        v = np.ones((1, 1), dtype=np.complex128)  # (2^k, chi)
+        for B in self.B:
+            v = np.einsum("xa,asb->xsb", v, B).reshape((-1, B.shape[2]),
+                                                      order="F")
+        return v[:, 0]
+
+
+def run_layers(state, layers, cutoff, chimax, record=None):
+    """Apply brick-wall layers ([(singles, pairs)], see
+    qaintensor.circuits) to a TebdState; optionally record per-gate
+    (site, k, disc, spectrum)."""
+    for singles, pairs in layers:
+        for (w, u) in singles:
+            state.apply_1q(w, u)
+        for (w, g) in pairs:
+            k, disc, S = state.apply_2q(w, g, cutoff, chimax)
+            if record is not None:
+                record.append((w, k, disc, S))
+    return state

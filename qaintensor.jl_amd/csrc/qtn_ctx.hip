@@ -111,6 +111,19 @@ int32_t qtn_ctx_sync(qtn_ctx* ctx) {
   return QTN_OK;
 }
 
+int32_t qtn_ctx_counters(qtn_ctx* ctx, int64_t* out, int32_t n,
+                         int32_t reset) {
+  qtn::clear_error();
+  QTN_ARG(ctx != nullptr && (n == 0 || out != nullptr),
+          "qtn_ctx_counters: NULL argument");
+  QTN_ARG(n >= 0 && n <= 4, "qtn_ctx_counters: n must be 0..4");
+  const int64_t v[4] = {ctx->n_svd, ctx->n_sweeps, ctx->max_sweeps,
+                        ctx->n_batches};
+  for (int i = 0; i < n; ++i) out[i] = v[i];
+  if (reset) ctx->n_svd = ctx->n_sweeps = ctx->max_sweeps = ctx->n_batches = 0;
+  return QTN_OK;
+}
+
 int32_t qtn_ctx_stream(qtn_ctx* ctx, void** stream) {
   qtn::clear_error();
   QTN_ARG(ctx != nullptr && stream != nullptr,

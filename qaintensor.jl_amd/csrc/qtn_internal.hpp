@@ -96,6 +96,8 @@ struct qtn_ctx {
   hipStream_t stream = nullptr;
   // Named scratch buffers grown on demand (never shrunk until destroy).
   std::vector<qtn::DevBuf> bufs;
+  // work counters (see qtn_ctx_counters)
+  int64_t n_svd = 0, n_sweeps = 0, max_sweeps = 0, n_batches = 0;
   // Small pinned host staging buffer for scalar read-backs.
   void* pinned = nullptr;
   size_t pinned_bytes = 0;

@@ -36,7 +36,7 @@ namespace {
 
 constexpr int kSlotPhi = 30, kSlotXm = 31, kSlotVk = 32, kSlotDesc = 33,
               kSlotGates = 34, kSlotEnv = 35, kSlotT1 = 36, kSlotT2 = 37,
-              kSlotRed = 38, kSlotBits = 39, kSlotAmp = 40;
+              kSlotRed = 38, kSlotBits = 39, kSlotAmp = 40, kSlotW2 = 41;
 
 unsigned grid_for(int64_t n) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256,
@@ -55,14 +55,17 @@ struct GateDesc {
   const double* lamL; // lambda_{i-1}
   double* lamM;       // lambda_i (written)
   double2* phi;       // Phi (M x N)
-  double2* X;         // X = theta^H (rows_p x cols_p)
+  double2* X;         // theta^H (trans) or theta, zero padded
+  double2* W;         // accumulated rotations when !trans (V = W)
   double2* Vk;        // V_k (N x k)
   const double2* g;   // 4x4 gate
   int64_t chiL, chiM, chiR;
+  int trans;          // 1: X = theta^H (M <= N), 0: X = theta (M > N)
 };
 
-// Phi <- g . Phi in place and X = conj(lambda_a Phi)^T; one thread per
-// (a, b) handles the four (s, t) entries.  grid (tiles, batch).
+// Phi <- g . Phi in place and X = (lambda_a Phi)^H (or lambda_a Phi when the
+// gate's X is not transposed); one thread per (a, b) handles the four (s, t)
+// entries.  grid (tiles, batch).
 __global__ void gate_epilogue(const GateDesc* __restrict__ gd,
                               int64_t rows_p) {
   const GateDesc d = gd[blockIdx.y];
@@ -100,13 +103,17 @@ __global__ void gate_epilogue(const GateDesc* __restrict__ gd,
         const double2 v = out[s + 2 * t];
         const int64_t row = a + chiL * s, col = t + 2 * b;
         d.phi[row + M * col] = v;
-        d.X[col + rows_p * row] = make_double2(la * v.x, -la * v.y);
+        if (d.trans)
+          d.X[col + rows_p * row] = make_double2(la * v.x, -la * v.y);
+        else
+          d.X[row + rows_p * col] = make_double2(la * v.x, la * v.y);
       }
   }
 }
 
-// V_k[:, j] = X[:, perm[j]] / sig[j] (N rows) and
-// B_{i+1}[j, c] = conj(V_k[c, j]) (k x N, ld k); lambda_i = sig / |S_k|.
+// V_k[:, j] = X[:, perm[j]] / sig[j] (X = theta^H) or W[:, perm[j]]
+// (X = theta), N rows; B_{i+1}[j, c] = conj(V_k[c, j]) (k x N, ld k);
+// lambda_i = sig / |S_k|.
 __global__ void split_right(const GateDesc* __restrict__ gd,
                             const double* __restrict__ sig,
                             const int* __restrict__ perm,
@@ -123,10 +130,15 @@ __global__ void split_right(const GateDesc* __restrict__ gd,
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
        idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
     const int64_t c = idx % N, j = idx / N;
-    const double sj = s[j];
-    const double inv = sj > 0.0 ? 1.0 / sj : 0.0;
-    const double2 x = d.X[c + rows_p * (int64_t)pm[j]];
-    const double2 v = make_double2(x.x * inv, x.y * inv);
+    double2 v;
+    if (d.trans) {
+      const double sj = s[j];
+      const double inv = sj > 0.0 ? 1.0 / sj : 0.0;
+      const double2 x = d.X[c + rows_p * (int64_t)pm[j]];
+      v = make_double2(x.x * inv, x.y * inv);
+    } else {
+      v = d.W[c + cols_p * (int64_t)pm[j]];
+    }
     d.Vk[c + N * j] = v;
     d.Br[j + k * c] = make_double2(v.x, -v.y);
   }
@@ -225,19 +237,30 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
   qtn_ctx* ctx = m->ctx;
   hipStream_t s = ctx->stream;
   std::vector<char> used(m->n + 1, 0);
-  int64_t maxM = 0, maxN = 0, maxK = 0;
+  int64_t maxM = 0, maxN = 0, maxXr = 0, maxXc = 0;
+  bool any_w = false;
+  std::vector<int> trans(ng);
   for (int g = 0; g < ng; ++g) {
     const int i = sites[g];
     QTN_ARG(i >= 1 && i < m->n, "apply_2q: site out of range");
     QTN_ARG(!used[i] && !used[i + 1], "apply_layer: gates overlap");
     used[i] = used[i + 1] = 1;
-    maxM = std::max(maxM, 2 * m->dims[i - 1]);
-    maxN = std::max(maxN, 2 * m->dims[i + 1]);
-    maxK = std::max(maxK, m->dims[i]);
+    const int64_t M = 2 * m->dims[i - 1], N = 2 * m->dims[i + 1];
+    maxM = std::max(maxM, M);
+    maxN = std::max(maxN, N);
+    // orthogonalise min(M, N) columns: theta^H when M <= N, else theta
+    // with the right vectors accumulated in W
+    trans[g] = (M <= N) ? 1 : 0;
+    maxXr = std::max(maxXr, trans[g] ? N : M);
+    maxXc = std::max(maxXc, trans[g] ? M : N);
+    if (!trans[g]) any_w = true;
   }
-  const int64_t rows_p = round_up64(maxN), cols_p = round_up64(maxM);
+  const int64_t rows_p = round_up64(maxXr), cols_p = round_up64(maxXc);
   const int64_t phi_sz = maxM * maxN, x_sz = rows_p * cols_p;
-  void *dphi, *dX, *dVk, *ddesc, *dg;
+  const int64_t w_sz = cols_p * cols_p;
+  void *dphi, *dX, *dVk, *ddesc, *dg, *dW = nullptr;
+  if (any_w)
+    QTN_TRY(ctx->buffer(kSlotW2, (size_t)ng * w_sz * sizeof(double2), &dW));
   QTN_TRY(ctx->buffer(kSlotPhi, (size_t)ng * phi_sz * sizeof(double2), &dphi));
   QTN_TRY(ctx->buffer(kSlotXm, (size_t)ng * x_sz * sizeof(double2), &dX));
   QTN_TRY(ctx->buffer(kSlotVk, (size_t)ng * maxN * std::max(maxM, maxN) *
@@ -262,6 +285,8 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
     d.lamM = m->lam[i];
     d.phi = (double2*)dphi + (size_t)g * phi_sz;
     d.X = (double2*)dX + (size_t)g * x_sz;
+    d.W = dW ? (double2*)dW + (size_t)g * w_sz : nullptr;
+    d.trans = trans[g];
     d.Vk = (double2*)dVk + (size_t)g * maxN * std::max(maxM, maxN);
     d.g = (const double2*)dg + 16 * g;
     const int64_t M = 2 * d.chiL, N = 2 * d.chiR;
@@ -299,8 +324,11 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
   // 3. SVD of every theta
   JacobiEngine eng;
   QTN_TRY(eng.reserve(ctx, ng, rows_p, cols_p));
+  std::vector<int> wantw(ng);
+  for (int g = 0; g < ng; ++g) wantw[g] = trans[g] ? 0 : 1;
   int sweeps = 0;
-  QTN_TRY(eng.run(ctx, (double2*)dX, x_sz, nullptr, 0, nsv.data(), cutoff, 1,
+  QTN_TRY(eng.run(ctx, (double2*)dX, x_sz, (double2*)dW, w_sz,
+                  any_w ? wantw.data() : nullptr, nsv.data(), cutoff, 1,
                   chimax, 60, &sweeps));
   std::vector<int64_t> kept(ng);
   std::vector<int> st(ng);

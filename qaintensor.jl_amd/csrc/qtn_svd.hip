@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 
 #include "qtn_internal.hpp"
 #include "qtn_svd.hpp"
@@ -167,9 +168,30 @@ __global__ __launch_bounds__(256) void jac_inner(
     Qr[r][c] = (r == c) ? 1.0 : 0.0;
     Qi[r][c] = 0.0;
   }
-  if (tid == 0) nrot_total = 0;
+  __shared__ int any_off;
+  if (tid == 0) {
+    nrot_total = 0;
+    any_off = 0;
+  }
   const double zt = zthr[bz];
   __syncthreads();
+  // Nothing to do when every pair already meets the tolerance (the common
+  // case in late sweeps): skip the 63 rounds.
+  for (int idx = tid; idx < 4096; idx += 256) {
+    const int r = idx & 63, c = idx >> 6;
+    if (r < c) {
+      const double a = Gr[r][r], d = Gr[c][c];
+      if (a > zt && d > zt) {
+        const double o2 = Gr[r][c] * Gr[r][c] + Gi[r][c] * Gi[r][c];
+        if (o2 > tol * tol * a * d) atomicOr(&any_off, 1);
+      }
+    }
+  }
+  __syncthreads();
+  if (!any_off) {
+    if (tid == 0) flags[bz * p + pi] = 0;
+    return;
+  }
 
   for (int sweep = 0; sweep < n_inner; ++sweep) {
     if (tid == 0) nrot_sweep = 0;
@@ -279,9 +301,10 @@ __global__ __launch_bounds__(256) void jac_inner(
 __global__ __launch_bounds__(256) void jac_update(
     double2* __restrict__ X, int64_t rows, int64_t sX, int p, int round,
     const double2* __restrict__ Q, const int* __restrict__ flags,
-    const int* __restrict__ active) {
+    const int* __restrict__ active, const int* __restrict__ want) {
   const int bz = blockIdx.z;
   if (!active[bz]) return;
+  if (want && !want[bz]) return;
   const int pi = blockIdx.y;
   if (!flags[bz * p + pi]) return;
   const int nb = 2 * p;
@@ -376,7 +399,7 @@ __global__ void jac_sweep_end(int* __restrict__ active,
 // Frobenius norm^2 per problem, finiteness check, zero-column threshold.
 // grid (batch), 256 threads.
 __global__ __launch_bounds__(256) void jac_prepare(
-    const double2* __restrict__ X, int64_t n_elem, int64_t sX,
+    const double2* __restrict__ X, int64_t n_elem, int64_t sX, double zrel,
     double* __restrict__ normF2, double* __restrict__ zthr,
     int* __restrict__ active, int* __restrict__ counters,
     int* __restrict__ bad) {
@@ -402,8 +425,9 @@ __global__ __launch_bounds__(256) void jac_prepare(
   if (threadIdx.x == 0) {
     const double f2 = red[0];
     normF2[bz] = f2;
-    // Columns with norm below eps*|X|_F are rounding noise: never rotated.
-    zthr[bz] = DBL_EPSILON * DBL_EPSILON * f2;
+    // Columns with norm below rows*eps*|X|_F are rounding noise (the
+    // backward error of the rotations themselves): never rotated.
+    zthr[bz] = zrel * zrel * f2;
     active[bz] = (f2 > 0.0 && !nf) ? 1 : 0;
     counters[bz] = 0;
     if (nf) atomicOr(bad, 1);
@@ -534,7 +558,7 @@ int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
   Q = (double2*)ptr;
   // small per-problem arrays packed in one slot
   const size_t small =
-      (size_t)batch * (p * sizeof(int) + 4 * sizeof(int) + 5 * sizeof(double) +
+      (size_t)batch * (p * sizeof(int) + 5 * sizeof(int) + 5 * sizeof(double) +
                        2 * sizeof(int64_t)) +
       1024;
   QTN_TRY(ctx->buffer(kSlotSmall, small, &ptr));
@@ -561,6 +585,8 @@ int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
   c += batch * sizeof(int);
   status = (int*)c;
   c += batch * sizeof(int);
+  want_w = (int*)c;
+  c += batch * sizeof(int);
   misc = (int*)c;  // [0] = n_active, [1] = nonfinite flag
   QTN_TRY(ctx->buffer(kSlotSig, (size_t)batch * cols_p * sizeof(double),
                       &ptr));
@@ -571,23 +597,31 @@ int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
 }
 
 int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t sX, double2* W,
-                          int64_t sW, const int64_t* nsv_host, double cutoff,
+                          int64_t sW, const int* want_w_host,
+                          const int64_t* nsv_host, double cutoff,
                           int relative, int64_t chimax, int max_sweeps,
                           int* sweeps_done) {
   hipStream_t s = ctx->stream;
   const double tol = std::sqrt((double)rows_p) * (DBL_EPSILON * 0.5);
   const int nb = 2 * p;
   const int rounds = nb - 1;
-  const int n_inner = (p == 1) ? 40 : 1;
+  // inner sweeps per block pair: the whole problem when it is one block
+  // pair, otherwise QTN_JACOBI_INNER (default 1)
+  int n_inner_big = 1;
+  if (const char* e = getenv("QTN_JACOBI_INNER")) n_inner_big = std::max(1, atoi(e));
+  const int n_inner = (p == 1) ? 40 : n_inner_big;
   const int nchunk_x = (int)(rows_p / 64);
   const int nchunk_w = (int)(cols_p / 64);
 
   QTN_CHECK_HIP(hipMemcpyAsync(nsv, nsv_host, batch * sizeof(int64_t),
                                hipMemcpyHostToDevice, s));
+  if (want_w_host)
+    QTN_CHECK_HIP(hipMemcpyAsync(want_w, want_w_host, batch * sizeof(int),
+                                 hipMemcpyHostToDevice, s));
   QTN_CHECK_HIP(hipMemsetAsync(misc, 0, 2 * sizeof(int), s));
   hipLaunchKernelGGL(jac_prepare, dim3(batch), dim3(256), 0, s, X,
-                     rows_p * cols_p, sX, normF2, zthr, active, counters,
-                     misc + 1);
+                     rows_p * cols_p, sX, (double)rows_p * DBL_EPSILON,
+                     normF2, zthr, active, counters, misc + 1);
   QTN_CHECK_HIP(hipGetLastError());
   if (W) {
     // W starts as the identity on the padded columns.
@@ -606,11 +640,12 @@ int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t sX, double2* W,
                          Q, flags, counters, active);
       hipLaunchKernelGGL(jac_update, dim3(nchunk_x, p, batch), dim3(256), 0, s,
                          X, rows_p, sX, p, r, (const double2*)Q, flags,
-                         active);
+                         active, (const int*)nullptr);
       if (W)
         hipLaunchKernelGGL(jac_update, dim3(nchunk_w, p, batch), dim3(256), 0,
                            s, W, cols_p, sW, p, r, (const double2*)Q, flags,
-                           active);
+                           active, (const int*)(want_w_host ? want_w
+                                                            : nullptr));
     }
     QTN_CHECK_HIP(hipGetLastError());
     hipLaunchKernelGGL(jac_sweep_end, dim3(1), dim3(256), 0, s, active,
@@ -629,6 +664,10 @@ int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t sX, double2* W,
     }
   }
   if (sweeps_done) *sweeps_done = sweep;
+  ctx->n_svd += batch;
+  ctx->n_sweeps += (int64_t)sweep * batch;
+  ctx->max_sweeps = std::max<int64_t>(ctx->max_sweeps, sweep);
+  ctx->n_batches += 1;
   if (!converged) {
     set_error("svd: one-sided Jacobi did not converge in " +
               std::to_string(max_sweeps) + " sweeps");

@@ -33,6 +33,7 @@ struct JacobiEngine {
   int* counters = nullptr;
   int* active = nullptr;
   int* status = nullptr;
+  int* want_w = nullptr;
   int* misc = nullptr;
   double* sig = nullptr;  // [batch][cols_p], sorted descending after run
   int* perm = nullptr;    // [batch][cols_p], column of X for each sigma
@@ -40,12 +41,14 @@ struct JacobiEngine {
   int32_t reserve(qtn_ctx* ctx, int batch, int64_t rows_p, int64_t cols_p);
   // Orthogonalise the columns of every X (rows_p x cols_p, ld rows_p,
   // batch stride sX); if W is given (cols_p x cols_p, stride sW) it
-  // accumulates the rotations starting from the identity.  Afterwards sig /
-  // perm hold the sorted column norms and kept / disc / total / status the
-  // truncation decision for nsv_host[b] meaningful values.
+  // accumulates the rotations starting from the identity, for the problems
+  // with want_w_host[b] != 0 (all of them when want_w_host is NULL).
+  // Afterwards sig / perm hold the sorted column norms and kept / disc /
+  // total / status the truncation decision for nsv_host[b] meaningful
+  // values.
   int32_t run(qtn_ctx* ctx, double2* X, int64_t sX, double2* W, int64_t sW,
-              const int64_t* nsv_host, double cutoff, int relative,
-              int64_t chimax, int max_sweeps, int* sweeps_done);
+              const int* want_w_host, const int64_t* nsv_host, double cutoff,
+              int relative, int64_t chimax, int max_sweeps, int* sweeps_done);
 };
 
 int32_t launch_identity(hipStream_t s, double2* W, int64_t n, int64_t sW,

@@ -121,7 +121,7 @@ int32_t svd_common(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
   const int64_t nsv = std::min(m, n);
   int sweeps = 0;
   QTN_TRY(eng.run(ctx, (double2*)dX, rows_p * cols_p, (double2*)dW,
-                  cols_p * cols_p, &nsv, cutoff, relative, chimax, 60,
+                  cols_p * cols_p, nullptr, &nsv, cutoff, relative, chimax, 60,
                   &sweeps));
   // read back the decision
   int64_t k = 0;

@@ -27,6 +27,7 @@ QTN_FORM_CHOSEN = 1
 # Every symbol include/qtn.h declares (checked by the CPU test suite).
 EXPORTS = [
     "qtn_ctx_create", "qtn_ctx_destroy", "qtn_ctx_sync", "qtn_ctx_stream",
+    "qtn_ctx_counters",
     "qtn_last_error", "qtn_version",
     "qtn_contract_pair", "qtn_contract_pair_dev", "qtn_permute",
     "qtn_permute_dev", "qtn_zgemm_dev", "qtn_cgemm_dev",
@@ -71,6 +72,7 @@ def _declare(lib):
         "qtn_ctx_destroy": (_i32, [_vp]),
         "qtn_ctx_sync": (_i32, [_vp]),
         "qtn_ctx_stream": (_i32, [_vp, ctypes.POINTER(_vp)]),
+        "qtn_ctx_counters": (_i32, [_vp, _pi64, _i32, _i32]),
         "qtn_last_error": (ctypes.c_char_p, []),
         "qtn_version": (ctypes.c_char_p, []),
         "qtn_contract_pair": (_i32, [_vp, _pd, _i32, _pi64, _pd, _i32, _pi64,
@@ -186,6 +188,14 @@ class Context:
         s = _vp()
         check(load().qtn_ctx_stream(self.handle, ctypes.byref(s)))
         return s.value
+
+    def counters(self, reset=False):
+        """dict of the context's work counters (qtn_ctx_counters)."""
+        out = np.zeros(4, dtype=np.int64)
+        check(load().qtn_ctx_counters(self.handle, i64ptr(out), 4,
+                                      int(bool(reset))))
+        return dict(svd=int(out[0]), sweeps=int(out[1]),
+                    max_sweeps=int(out[2]), batches=int(out[3]))
 
     def close(self):
         if self._h is not None:

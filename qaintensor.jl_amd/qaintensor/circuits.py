@@ -1,0 +1,57 @@
+"""Seeded Haar-random brick-wall circuits (the synthetic inputs of every
+benchmark configuration).
+
+Haar unitaries follow Mezzadri, "How to generate random matrices from the
+classical compact groups", Notices AMS 54 (2007): Q R = Z for a complex
+Ginibre matrix Z = (X + iY)/sqrt(2), then U = Q diag(r_ii / |r_ii|).
+
+Layer l (0-based) applies a Haar U(2) on every wire 1..n, then a Haar U(4) on
+the pairs (1,2),(3,4),... when l is even and (2,3),(4,5),... when l is odd.
+Draw order per layer: the n single-qubit gates (wire order), then the
+two-qubit gates (pair order), all from one numpy PCG64 generator seeded
+once.  Each single-qubit gate is fused into the two-qubit gate that follows
+it on its wire; unpaired wires keep their single-qubit gate.  A 4x4 gate
+acts on the index s_w + 2 s_{w+1} (wire w <-> bit w-1), so the fused gate
+is U4 @ kron(u_{w+1}, u_w).
+"""
+import numpy as np
+
+
+def haar_unitary(rng, d):
+    z = (rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))) \
+        / np.sqrt(2.0)
+    q, r = np.linalg.qr(z)
+    ph = np.diag(r) / np.abs(np.diag(r))
+    return q * ph[None, :]
+
+
+def brickwall_layers(n, depth, seed):
+    """[(singles, pairs)] per layer: singles = [(wire, u 2x2)] for wires not
+    covered by a pair, pairs = [(wire, fused 4x4 gate on (wire, wire+1))]."""
+    rng = np.random.default_rng(seed)
+    layers = []
+    for layer in range(depth):
+        us = [haar_unitary(rng, 2) for _ in range(n)]
+        start = 1 if layer % 2 == 0 else 2
+        pair_wires = list(range(start, n, 2))
+        gs = [haar_unitary(rng, 4) for _ in pair_wires]
+        covered = set()
+        pairs = []
+        for w, g in zip(pair_wires, gs):
+            pairs.append((w, g @ np.kron(us[w], us[w - 1])))
+            covered.add(w)
+            covered.add(w + 1)
+        singles = [(w, us[w - 1]) for w in range(1, n + 1)
+                   if w not in covered]
+        layers.append((singles, pairs))
+    return layers
+
+
+def count_2q(layers):
+    return sum(len(p) for _, p in layers)
+
+
+def random_bitstrings(n, count, seed):
+    """count x n array of 0/1 (uint8); column w-1 is wire w."""
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, 2, size=(count, n), dtype=np.uint8)

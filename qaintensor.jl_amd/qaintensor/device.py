@@ -1,0 +1,174 @@
+"""Device-resident MPS driven through libqtn (qtn_mps_* in include/qtn.h).
+
+DeviceMPS keeps the state on the GPU in the right-canonical gauge with the
+bond spectra beside the site tensors.  Only 2x2 / 4x4 gates and scalars
+cross the ABI per gate.  This is the hot path behind the build's
+apply!/expect/amplitude methods (the reference has the two-site split only
+as switch!, src/switch.jl:18-56, and no truncating update).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+FORM_PLAIN = L.QTN_FORM_PLAIN
+FORM_CHOSEN = L.QTN_FORM_CHOSEN
+
+
+class DeviceMPS:
+    """|0...0> of `nsites` qubits with per-bond capacity `chimax_cap`."""
+
+    def __init__(self, nsites, chimax_cap, ctx=None):
+        self.ctx = ctx if ctx is not None else L.default_context()
+        self.n = int(nsites)
+        self.cap = int(chimax_cap)
+        h = ctypes.c_void_p()
+        L.check(L.load().qtn_mps_create(self.ctx.handle, self.n, self.cap,
+                                        ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise L.QtnError(L.QTN_ERR_ARG, "DeviceMPS already freed")
+        return self._h
+
+    def free(self):
+        if self._h is not None:
+            L.check(L.load().qtn_mps_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            if self._h is not None and L._lib is not None:
+                L._lib.qtn_mps_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def copy_from(self, other):
+        L.check(L.load().qtn_mps_copy(other.handle, self.handle))
+
+    # -- gates -------------------------------------------------------------
+    def apply_1q(self, site, u):
+        u = L.as_c128(u, "F")
+        if u.shape != (2, 2):
+            raise ValueError("single-qubit gate must be 2x2")
+        L.check(L.load().qtn_mps_apply_1q(self.handle, int(site), L.dptr(u)))
+
+    def apply_2q(self, site, g, cutoff, chimax, form=FORM_PLAIN,
+                 want_spectrum=False):
+        """Gate g (4x4, index s_site + 2 s_{site+1}) on (site, site+1).
+        Returns (kept rank, discarded weight[, full spectrum])."""
+        g = L.as_c128(g, "F")
+        if g.shape != (4, 4):
+            raise ValueError("two-qubit gate must be 4x4")
+        kept = ctypes.c_int64(0)
+        disc = ctypes.c_double(0.0)
+        spec = None
+        slen = ctypes.c_int64(0)
+        if want_spectrum:
+            spec = np.zeros(2 * self.cap + 2, dtype=np.float64)
+        L.check(L.load().qtn_mps_apply_2q(
+            self.handle, int(site), L.dptr(g), float(cutoff), int(chimax),
+            int(form), ctypes.byref(kept), ctypes.byref(disc),
+            L.dptr(spec) if spec is not None else None,
+            ctypes.byref(slen)))
+        if want_spectrum:
+            return kept.value, disc.value, spec[:slen.value].copy()
+        return kept.value, disc.value
+
+    def apply_layer(self, sites, gates, cutoff, chimax, form=FORM_PLAIN):
+        """Gates on disjoint pairs in one batched pass.  Returns (kept[],
+        disc[])."""
+        ng = len(sites)
+        if ng == 0:
+            return np.zeros(0, np.int64), np.zeros(0)
+        s = np.ascontiguousarray(sites, dtype=np.int32)
+        g = np.empty((ng, 16), dtype=np.complex128)
+        for i, gi in enumerate(gates):
+            gi = np.asarray(gi, dtype=np.complex128)
+            if gi.shape != (4, 4):
+                raise ValueError("two-qubit gate must be 4x4")
+            g[i] = gi.reshape(-1, order="F")
+        kept = np.zeros(ng, dtype=np.int64)
+        disc = np.zeros(ng, dtype=np.float64)
+        L.check(L.load().qtn_mps_apply_layer(
+            self.handle, ng, L.i32ptr(s), L.dptr(g), float(cutoff),
+            int(chimax), int(form), L.i64ptr(kept), L.dptr(disc)))
+        return kept, disc
+
+    def run_layers(self, layers, cutoff, chimax, form=FORM_PLAIN,
+                   batched=True, record=None):
+        """Apply brick-wall layers ([(singles, pairs)]).  batched=False
+        applies the two-qubit gates one at a time (the plain per-gate
+        form)."""
+        for singles, pairs in layers:
+            for (w, u) in singles:
+                self.apply_1q(w, u)
+            if not pairs:
+                continue
+            if batched:
+                k, d = self.apply_layer([w for w, _ in pairs],
+                                        [g for _, g in pairs], cutoff,
+                                        chimax, form)
+                if record is not None:
+                    for (w, _), ki, di in zip(pairs, k, d):
+                        record.append((w, int(ki), float(di)))
+            else:
+                for (w, g) in pairs:
+                    ki, di = self.apply_2q(w, g, cutoff, chimax, form)
+                    if record is not None:
+                        record.append((w, ki, di))
+        return self
+
+    # -- observables -------------------------------------------------------
+    def bond_dims(self):
+        d = np.zeros(self.n + 1, dtype=np.int64)
+        L.check(L.load().qtn_mps_bond_dims(self.handle, L.i64ptr(d)))
+        return d
+
+    def bond_spectrum(self, bond):
+        S = np.zeros(self.cap + 1, dtype=np.float64)
+        k = ctypes.c_int64(0)
+        L.check(L.load().qtn_mps_bond_spectrum(self.handle, int(bond),
+                                               L.dptr(S), ctypes.byref(k)))
+        return S[:k.value].copy()
+
+    def site(self, j):
+        dims = np.zeros(3, dtype=np.int64)
+        L.check(L.load().qtn_mps_get_site(self.handle, int(j), None,
+                                          L.i64ptr(dims)))
+        out = np.zeros(int(np.prod(dims)), dtype=np.complex128)
+        L.check(L.load().qtn_mps_get_site(self.handle, int(j), L.dptr(out),
+                                          L.i64ptr(dims)))
+        return out.reshape(tuple(dims), order="F")
+
+    def amplitudes(self, bits):
+        bits = np.ascontiguousarray(bits, dtype=np.uint8)
+        if bits.ndim != 2 or bits.shape[1] != self.n:
+            raise ValueError("bits must be (count, nsites)")
+        out = np.zeros(bits.shape[0], dtype=np.complex128)
+        L.check(L.load().qtn_mps_amplitudes(
+            self.handle, bits.shape[0],
+            bits.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+            L.dptr(out)))
+        return out
+
+    def expect_z(self):
+        out = np.zeros(self.n, dtype=np.float64)
+        L.check(L.load().qtn_mps_expect_z(self.handle, L.dptr(out)))
+        return out
+
+    def overlap(self, other):
+        """<self|other>."""
+        out = np.zeros(1, dtype=np.complex128)
+        L.check(L.load().qtn_mps_overlap(self.handle, other.handle,
+                                         L.dptr(out)))
+        return complex(out[0])
+
+    def norm2(self):
+        out = ctypes.c_double(0.0)
+        L.check(L.load().qtn_mps_norm2(self.handle, ctypes.byref(out)))
+        return out.value

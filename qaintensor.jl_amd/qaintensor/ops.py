@@ -1,0 +1,103 @@
+"""Array-level wrappers of libqtn's compute entry points.
+
+Arrays follow the reference's (Julia) index order and are passed to the
+library in column-major (Fortran) memory order, so dims go across the ABI in
+Julia order unchanged.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _ctx(ctx):
+    return ctx if ctx is not None else L.default_context()
+
+
+def _dims(a):
+    return np.ascontiguousarray(a.shape, dtype=np.int64)
+
+
+def contract_pair(A, B, legsA, legsB, outperm=None, ctx=None):
+    """One pairwise ncon step on the GPU (qtn_contract_pair): contract leg
+    legsA[i] of A with legsB[i] of B (1-based); output axes are the free
+    legs of A then of B, permuted by outperm (1-based) if given."""
+    ctx = _ctx(ctx)
+    A = L.as_c128(A, "F")
+    B = L.as_c128(B, "F")
+    la = np.ascontiguousarray(legsA, dtype=np.int32)
+    lb = np.ascontiguousarray(legsB, dtype=np.int32)
+    if la.size != lb.size:
+        raise ValueError("contract_pair: legsA and legsB differ in length")
+    fa = [A.shape[i] for i in range(A.ndim) if (i + 1) not in set(la)]
+    fb = [B.shape[i] for i in range(B.ndim) if (i + 1) not in set(lb)]
+    nat = fa + fb
+    if outperm is not None:
+        op = np.ascontiguousarray(outperm, dtype=np.int32)
+        if sorted(op.tolist()) != list(range(1, len(nat) + 1)):
+            raise ValueError("contract_pair: outperm is not a permutation")
+        out_shape = [nat[p - 1] for p in op]
+        op_ptr = L.i32ptr(op)
+    else:
+        out_shape = nat
+        op_ptr = None
+    C = np.zeros(out_shape, dtype=np.complex128, order="F")
+    dA, dB = _dims(A), _dims(B)
+    L.check(L.load().qtn_contract_pair(
+        ctx.handle, L.dptr(A), A.ndim, L.i64ptr(dA), L.dptr(B), B.ndim,
+        L.i64ptr(dB), la.size, L.i32ptr(la), L.i32ptr(lb), op_ptr,
+        L.dptr(C)))
+    return C
+
+
+def permute(A, perm, ctx=None):
+    """permutedims(A, perm) (perm 1-based) on the GPU."""
+    ctx = _ctx(ctx)
+    A = L.as_c128(A, "F")
+    p = np.ascontiguousarray(perm, dtype=np.int32)
+    if sorted(p.tolist()) != list(range(1, A.ndim + 1)):
+        raise ValueError("permute: perm is not a permutation")
+    out = np.zeros([A.shape[i - 1] for i in p], dtype=np.complex128,
+                   order="F")
+    dA = _dims(A)
+    L.check(L.load().qtn_permute(ctx.handle, L.dptr(A), A.ndim,
+                                 L.i64ptr(dA), L.i32ptr(p), L.dptr(out)))
+    return out
+
+
+def svd_trunc(A, cutoff=0.0, relative=False, chimax=0, ctx=None):
+    """Truncated SVD on the GPU: (U, S, Vh, discarded weight) with the
+    truncation rule of src/svd.jl:29-33 (er = cutoff, or cutoff*||s|| when
+    relative), then k <= chimax when chimax > 0."""
+    ctx = _ctx(ctx)
+    A = L.as_c128(A, "F")
+    if A.ndim != 2:
+        raise ValueError("svd_trunc: matrix expected")
+    m, n = A.shape
+    kk = min(m, n)
+    U = np.zeros((m, kk), dtype=np.complex128, order="F")
+    S = np.zeros(kk, dtype=np.float64)
+    Vh = np.zeros((kk, n), dtype=np.complex128, order="F")
+    k = ctypes.c_int64(0)
+    disc = ctypes.c_double(0.0)
+    L.check(L.load().qtn_svd_trunc(
+        ctx.handle, m, n, L.dptr(A), m, float(cutoff), int(bool(relative)),
+        int(chimax), ctypes.byref(k), L.dptr(U), L.dptr(S), L.dptr(Vh),
+        ctypes.byref(disc)))
+    k = k.value
+    # the library writes U with ld m and Vh with ld k
+    Vh_k = np.asfortranarray(
+        Vh.reshape(-1, order="F")[:k * n].reshape((k, n), order="F"))
+    return U[:, :k].copy(order="F"), S[:k].copy(), Vh_k, disc.value
+
+
+def svd_values(A, ctx=None):
+    """All min(m, n) singular values of A (descending) on the GPU."""
+    ctx = _ctx(ctx)
+    A = L.as_c128(A, "F")
+    m, n = A.shape
+    S = np.zeros(min(m, n), dtype=np.float64)
+    L.check(L.load().qtn_svd_values(ctx.handle, m, n, L.dptr(A), m,
+                                    L.dptr(S)))
+    return S
