@@ -129,15 +129,11 @@ __global__ __launch_bounds__(256) void jac_gram(
 
 // ---------------------------------------------------------------------------
 // 2. Sum the partial Grams, run inner cyclic Jacobi sweep(s), emit Q.
-// grid (p, batch).  flags[bz*p + pi] = 1 if Q must be applied.
-struct RotParam {
-  int p, q;
-  double cs, sn;    // real rotation
-  double er, ei;    // e^{-i phi}
-  double ap, dq;    // new diagonal entries
-  int on;
-};
-
+// grid (p, batch), 256 threads.  flags[bz*p + pi] = 1 if Q must be applied.
+// Each of the 32 rotation pairs of a round is owned by 8 threads, which all
+// derive the rotation from G_pp, G_qq, G_pq themselves; thread t of a pair
+// updates rows / columns t, t+8, ..., t+56.  Two barriers per round: after
+// the column update (G J, Q J) and after the row update (J^H G).
 __global__ __launch_bounds__(256) void jac_inner(
     const double2* __restrict__ Gp, int nchunk, int p, double tol,
     const double* __restrict__ zthr, int n_inner, double2* __restrict__ Qout,
@@ -146,13 +142,10 @@ __global__ __launch_bounds__(256) void jac_inner(
   const int bz = blockIdx.y;
   const int pi = blockIdx.x;
   if (!active[bz]) return;
-  __shared__ double Gr[64][65];  // G[row][col]
-  __shared__ double Gi[64][65];
-  __shared__ double Qr[64][65];  // Q[row][col]
-  __shared__ double Qi[64][65];
-  __shared__ RotParam rot[32];
-  __shared__ int nrot_sweep;
-  __shared__ int nrot_total;
+  __shared__ double2 G[64][65];  // G[row][col]
+  __shared__ double2 Q[64][65];  // Q[row][col]
+  __shared__ int any_off;
+  __shared__ int nrot;
   const int tid = threadIdx.x;
   const double2* src = Gp + ((int64_t)bz * p + pi) * nchunk * (int64_t)4096;
   for (int idx = tid; idx < 4096; idx += 256) {
@@ -163,27 +156,25 @@ __global__ __launch_bounds__(256) void jac_inner(
       sr += v.x;
       si += v.y;
     }
-    Gr[r][c] = sr;
-    Gi[r][c] = si;
-    Qr[r][c] = (r == c) ? 1.0 : 0.0;
-    Qi[r][c] = 0.0;
+    G[r][c] = make_double2(sr, si);
+    Q[r][c] = make_double2(r == c ? 1.0 : 0.0, 0.0);
   }
-  __shared__ int any_off;
   if (tid == 0) {
-    nrot_total = 0;
     any_off = 0;
+    nrot = 0;
   }
   const double zt = zthr[bz];
+  const double tol2 = tol * tol;
   __syncthreads();
   // Nothing to do when every pair already meets the tolerance (the common
-  // case in late sweeps): skip the 63 rounds.
+  // case in late sweeps): skip the rounds.
   for (int idx = tid; idx < 4096; idx += 256) {
     const int r = idx & 63, c = idx >> 6;
     if (r < c) {
-      const double a = Gr[r][r], d = Gr[c][c];
+      const double a = G[r][r].x, d = G[c][c].x;
       if (a > zt && d > zt) {
-        const double o2 = Gr[r][c] * Gr[r][c] + Gi[r][c] * Gi[r][c];
-        if (o2 > tol * tol * a * d) atomicOr(&any_off, 1);
+        const double2 g = G[r][c];
+        if (g.x * g.x + g.y * g.y > tol2 * a * d) atomicOr(&any_off, 1);
       }
     }
   }
@@ -192,97 +183,91 @@ __global__ __launch_bounds__(256) void jac_inner(
     if (tid == 0) flags[bz * p + pi] = 0;
     return;
   }
-
+  const int pr = tid >> 3;  // rotation pair of this thread
+  const int t = tid & 7;
   for (int sweep = 0; sweep < n_inner; ++sweep) {
-    if (tid == 0) nrot_sweep = 0;
-    __syncthreads();
+    int sweep_rot = 0;
     for (int round = 0; round < 63; ++round) {
-      if (tid < 32) {
-        const int c1 = rr_block(tid, round, 64);
-        const int c2 = rr_block(63 - tid, round, 64);
-        const int pp = min(c1, c2), qq = max(c1, c2);
-        const double a = Gr[pp][pp], d = Gr[qq][qq];
-        const double cr = Gr[pp][qq], ci = Gi[pp][qq];
-        const double absc = hypot(cr, ci);
-        RotParam rp;
-        rp.p = pp;
-        rp.q = qq;
-        rp.on = 0;
-        if (a > zt && d > zt && absc > tol * sqrt(a) * sqrt(d)) {
-          const double zeta = (d - a) / (2.0 * absc);
-          const double t = (zeta >= 0.0 ? 1.0 : -1.0) /
-                           (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-          const double cs = 1.0 / sqrt(1.0 + t * t);
-          rp.cs = cs;
-          rp.sn = t * cs;
-          rp.er = cr / absc;
-          rp.ei = -ci / absc;
-          rp.ap = a - t * absc;
-          rp.dq = d + t * absc;
-          rp.on = 1;
-          atomicAdd(&nrot_sweep, 1);
+      const int c1 = rr_block(pr, round, 64);
+      const int c2 = rr_block(63 - pr, round, 64);
+      const int pp = min(c1, c2), qq = max(c1, c2);
+      const double a = G[pp][pp].x, d = G[qq][qq].x;
+      const double2 g = G[pp][qq];
+      const double absc2 = g.x * g.x + g.y * g.y;
+      const bool on = (a > zt && d > zt && absc2 > tol2 * a * d);
+      double cs = 1.0, sn = 0.0, er = 1.0, ei = 0.0, ap = a, dq = d;
+      if (on) {
+        const double absc = sqrt(absc2);
+        const double zeta = (d - a) / (2.0 * absc);
+        const double tt = (zeta >= 0.0 ? 1.0 : -1.0) /
+                          (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        cs = 1.0 / sqrt(1.0 + tt * tt);
+        sn = tt * cs;
+        er = g.x / absc;   // e = e^{-i phi} = conj(g) / |g|
+        ei = -g.y / absc;
+        ap = a - tt * absc;
+        dq = d + tt * absc;
+        sweep_rot += (t == 0);
+        // column update: [x_p x_q] <- [x_p x_q] J,
+        // J = [[cs, sn], [-sn e, cs e]]
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int r = t + 8 * j;
+          {
+            const double2 xp = G[r][pp], xq = G[r][qq];
+            const double yr = er * xq.x - ei * xq.y;
+            const double yi = er * xq.y + ei * xq.x;
+            G[r][pp] = make_double2(cs * xp.x - sn * yr, cs * xp.y - sn * yi);
+            G[r][qq] = make_double2(sn * xp.x + cs * yr, sn * xp.y + cs * yi);
+          }
+          {
+            const double2 xp = Q[r][pp], xq = Q[r][qq];
+            const double yr = er * xq.x - ei * xq.y;
+            const double yi = er * xq.y + ei * xq.x;
+            Q[r][pp] = make_double2(cs * xp.x - sn * yr, cs * xp.y - sn * yi);
+            Q[r][qq] = make_double2(sn * xp.x + cs * yr, sn * xp.y + cs * yi);
+          }
         }
-        rot[tid] = rp;
       }
       __syncthreads();
-      // column update of G and Q: [g_p g_q] <- [g_p g_q] J,
-      // J = [[cs, sn], [-sn e, cs e]], e = e^{-i phi}
-      for (int task = tid; task < 32 * 128; task += 256) {
-        const int pr = task >> 7;
-        const RotParam& rp = rot[pr];
-        if (!rp.on) continue;
-        const int rr = task & 63;
-        const bool isq = (task & 64) != 0;
-        double (*Mr)[65] = isq ? Qr : Gr;
-        double (*Mi)[65] = isq ? Qi : Gi;
-        const double xpr = Mr[rr][rp.p], xpi = Mi[rr][rp.p];
-        const double xqr = Mr[rr][rp.q], xqi = Mi[rr][rp.q];
-        // e * x_q
-        const double yr = rp.er * xqr - rp.ei * xqi;
-        const double yi = rp.er * xqi + rp.ei * xqr;
-        Mr[rr][rp.p] = rp.cs * xpr - rp.sn * yr;
-        Mi[rr][rp.p] = rp.cs * xpi - rp.sn * yi;
-        Mr[rr][rp.q] = rp.sn * xpr + rp.cs * yr;
-        Mi[rr][rp.q] = rp.sn * xpi + rp.cs * yi;
-      }
-      __syncthreads();
-      // row update of G: [r_p; r_q] <- J^H [r_p; r_q],
-      // J^H = [[cs, -sn conj(e)], [sn, cs conj(e)]]
-      for (int task = tid; task < 32 * 64; task += 256) {
-        const int pr = task >> 6;
-        const RotParam& rp = rot[pr];
-        if (!rp.on) continue;
-        const int cc = task & 63;
-        const double xpr = Gr[rp.p][cc], xpi = Gi[rp.p][cc];
-        const double xqr = Gr[rp.q][cc], xqi = Gi[rp.q][cc];
-        // conj(e) * x_q
-        const double yr = rp.er * xqr + rp.ei * xqi;
-        const double yi = rp.er * xqi - rp.ei * xqr;
-        Gr[rp.p][cc] = rp.cs * xpr - rp.sn * yr;
-        Gi[rp.p][cc] = rp.cs * xpi - rp.sn * yi;
-        Gr[rp.q][cc] = rp.sn * xpr + rp.cs * yr;
-        Gi[rp.q][cc] = rp.sn * xpi + rp.cs * yi;
-      }
-      __syncthreads();
-      if (tid < 32 && rot[tid].on) {
-        const RotParam& rp = rot[tid];
-        Gr[rp.p][rp.q] = 0.0;
-        Gi[rp.p][rp.q] = 0.0;
-        Gr[rp.q][rp.p] = 0.0;
-        Gi[rp.q][rp.p] = 0.0;
-        Gr[rp.p][rp.p] = rp.ap;
-        Gi[rp.p][rp.p] = 0.0;
-        Gr[rp.q][rp.q] = rp.dq;
-        Gi[rp.q][rp.q] = 0.0;
+      if (on) {
+        // row update: [r_p; r_q] <- J^H [r_p; r_q],
+        // J^H = [[cs, -sn conj(e)], [sn, cs conj(e)]]
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int c = t + 8 * j;
+          const double2 xp = G[pp][c], xq = G[qq][c];
+          const double yr = er * xq.x + ei * xq.y;
+          const double yi = er * xq.y - ei * xq.x;
+          double2 np = make_double2(cs * xp.x - sn * yr, cs * xp.y - sn * yi);
+          double2 nq = make_double2(sn * xp.x + cs * yr, sn * xp.y + cs * yi);
+          // the 2x2 block becomes diag(ap, dq) exactly
+          if (c == pp) {
+            np = make_double2(ap, 0.0);
+            nq = make_double2(0.0, 0.0);
+          } else if (c == qq) {
+            np = make_double2(0.0, 0.0);
+            nq = make_double2(dq, 0.0);
+          }
+          G[pp][c] = np;
+          G[qq][c] = nq;
+        }
       }
       __syncthreads();
     }
-    const int nr = nrot_sweep;
-    if (tid == 0) nrot_total += nr;
+    // rotations of this sweep (one count per pair per round)
+    if (sweep_rot) atomicAdd(&nrot, sweep_rot);
     __syncthreads();
-    if (nr == 0) break;
+    const int total = nrot;
+    __syncthreads();
+    if (tid == 0) nrot = 0;
+    if (total == 0) break;
+    __syncthreads();
+    if (tid == 0) any_off += total;  // reuse as the running total
+    __syncthreads();
   }
-  const int total = nrot_total;
+  __syncthreads();
+  const int total = any_off - 1;  // any_off started at 1 after the check
   if (tid == 0) {
     flags[bz * p + pi] = total > 0 ? 1 : 0;
     if (total > 0) atomicAdd(&counters[bz], total);
@@ -291,7 +276,7 @@ __global__ __launch_bounds__(256) void jac_inner(
     double2* q = Qout + ((int64_t)bz * p + pi) * (int64_t)4096;
     for (int idx = tid; idx < 4096; idx += 256) {
       const int r = idx & 63, c = idx >> 6;
-      q[idx] = make_double2(Qr[r][c], Qi[r][c]);
+      q[idx] = Q[r][c];
     }
   }
 }
