@@ -45,17 +45,23 @@ __device__ __forceinline__ int rr_block(int pos, int round, int nb) {
   return pos == 0 ? 0 : 1 + (pos - 1 + round) % (nb - 1);
 }
 
-// Global column of local column c (0..63) of the block pair (I, J).
+// Global column of local column c (0..W-1) of the block pair (I, J) for
+// blocks of W/2 columns.
+template <int W>
 __device__ __forceinline__ int64_t pair_col(int c, int I, int J) {
-  return c < 32 ? (int64_t)I * 32 + c : (int64_t)J * 32 + (c - 32);
+  return c < W / 2 ? (int64_t)I * (W / 2) + c
+                   : (int64_t)J * (W / 2) + (c - W / 2);
 }
 
 // ---------------------------------------------------------------------------
-// 1. Partial Gram of a 64-row chunk of a block pair.
-// grid (rows_p/64, p, batch); Gp layout [batch][p][chunk][64*64] col-major.
+// 1. Partial Gram of a 64-row chunk of a block pair (W columns).
+// grid (rows_p/64, p, batch); Gp layout [batch][p][chunk][W*W] col-major.
+// Each of the 4 waves computes a (W/2) x (W/2) quadrant.
+template <int W>
 __global__ __launch_bounds__(256) void jac_gram(
     const double2* __restrict__ X, int64_t rows_p, int64_t sX, int p,
     int round, double2* __restrict__ Gp, const int* __restrict__ active) {
+  constexpr int T = W / 32;  // 16x16 MFMA tiles per wave per dimension
   const int bz = blockIdx.z;
   if (!active[bz]) return;
   const int nb = 2 * p;
@@ -67,24 +73,25 @@ __global__ __launch_bounds__(256) void jac_gram(
   const int I = min(b1, b2), J = max(b1, b2);
   X += (int64_t)bz * sX;
 
-  __shared__ double Xr[64][65];  // [local col][row]
-  __shared__ double Xi[64][65];
+  __shared__ double Xr[W][65];  // [local col][row]
+  __shared__ double Xi[W][65];
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < 64 * 64; idx += 256) {
+  for (int idx = tid; idx < 64 * W; idx += 256) {
     const int r = idx & 63, c = idx >> 6;
-    const double2 v = X[(int64_t)chunk * 64 + r + pair_col(c, I, J) * rows_p];
+    const double2 v =
+        X[(int64_t)chunk * 64 + r + pair_col<W>(c, I, J) * rows_p];
     Xr[c][r] = v.x;
     Xi[c][r] = v.y;
   }
   __syncthreads();
 
   const int lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-  d4_t gr[2][2], gi[2][2];
+  const int wm = (wave & 1) * (W / 2), wn = (wave >> 1) * (W / 2);
+  d4_t gr[T][T], gi[T][T];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < T; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < T; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         gr[i][j][e] = 0.0;
@@ -94,18 +101,18 @@ __global__ __launch_bounds__(256) void jac_gram(
 #pragma unroll 4
   for (int ks = 0; ks < 64; ks += 4) {
     const int kr = ks + (lane >> 4);
-    double ar[2], ai[2], br[2], bi[2];
+    double ar[T], ai[T], br[T], bi[T];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < T; ++i) {
       ar[i] = Xr[wm + i * 16 + (lane & 15)][kr];
       ai[i] = -Xi[wm + i * 16 + (lane & 15)][kr];
       br[i] = Xr[wn + i * 16 + (lane & 15)][kr];
       bi[i] = Xi[wn + i * 16 + (lane & 15)][kr];
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < T; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
+      for (int j = 0; j < T; ++j) {
         gr[i][j] = mfma64(ar[i], br[j], gr[i][j]);
         gr[i][j] = mfma64(-ai[i], bi[j], gr[i][j]);
         gi[i][j] = mfma64(ar[i], bi[j], gi[i][j]);
@@ -113,16 +120,16 @@ __global__ __launch_bounds__(256) void jac_gram(
       }
   }
   double2* out =
-      Gp + (((int64_t)bz * p + pi) * nchunk + chunk) * (int64_t)(64 * 64);
+      Gp + (((int64_t)bz * p + pi) * nchunk + chunk) * (int64_t)(W * W);
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < T; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < T; ++j) {
       const int col = wn + j * 16 + (lane & 15);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int row = wm + i * 16 + 4 * e + (lane >> 4);
-        out[row + 64 * col] = make_double2(gr[i][j][e], gi[i][j][e]);
+        out[row + W * col] = make_double2(gr[i][j][e], gi[i][j][e]);
       }
     }
 }
@@ -130,29 +137,35 @@ __global__ __launch_bounds__(256) void jac_gram(
 // ---------------------------------------------------------------------------
 // 2. Sum the partial Grams, run inner cyclic Jacobi sweep(s), emit Q.
 // grid (p, batch), 256 threads.  flags[bz*p + pi] = 1 if Q must be applied.
-// Each of the 32 rotation pairs of a round is owned by 8 threads, which all
-// derive the rotation from G_pp, G_qq, G_pq themselves; thread t of a pair
-// updates rows / columns t, t+8, ..., t+56.  Two barriers per round: after
-// the column update (G J, Q J) and after the row update (J^H G).
+// Each of the W/2 rotation pairs of a round is owned by 512/W threads, which
+// all derive the rotation from G_pp, G_qq, G_pq themselves; thread t of a
+// pair updates rows / columns t, t+TP, ...  Two barriers per round: after
+// the column update (G J, Q J) and after the row update (J^H G).  Operands
+// are loaded into registers before any store so LDS latencies overlap.
+template <int W>
 __global__ __launch_bounds__(256) void jac_inner(
     const double2* __restrict__ Gp, int nchunk, int p, double tol,
     const double* __restrict__ zthr, int n_inner, double2* __restrict__ Qout,
     int* __restrict__ flags, int* __restrict__ counters,
     const int* __restrict__ active) {
+  constexpr int NP = W / 2;       // rotation pairs per round
+  constexpr int TP = 256 / NP;    // threads per pair
+  constexpr int RPT = W / TP;     // rows (columns) per thread
   const int bz = blockIdx.y;
   const int pi = blockIdx.x;
   if (!active[bz]) return;
-  __shared__ double2 G[64][65];  // G[row][col]
-  __shared__ double2 Q[64][65];  // Q[row][col]
+  __shared__ double2 G[W][W + 1];  // G[row][col]
+  __shared__ double2 Q[W][W + 1];  // Q[row][col]
   __shared__ int any_off;
   __shared__ int nrot;
   const int tid = threadIdx.x;
-  const double2* src = Gp + ((int64_t)bz * p + pi) * nchunk * (int64_t)4096;
-  for (int idx = tid; idx < 4096; idx += 256) {
-    const int r = idx & 63, c = idx >> 6;
+  const double2* src =
+      Gp + ((int64_t)bz * p + pi) * nchunk * (int64_t)(W * W);
+  for (int idx = tid; idx < W * W; idx += 256) {
+    const int r = idx % W, c = idx / W;
     double sr = 0.0, si = 0.0;
     for (int ch = 0; ch < nchunk; ++ch) {
-      const double2 v = src[(int64_t)ch * 4096 + idx];
+      const double2 v = src[(int64_t)ch * (W * W) + idx];
       sr += v.x;
       si += v.y;
     }
@@ -168,8 +181,8 @@ __global__ __launch_bounds__(256) void jac_inner(
   __syncthreads();
   // Nothing to do when every pair already meets the tolerance (the common
   // case in late sweeps): skip the rounds.
-  for (int idx = tid; idx < 4096; idx += 256) {
-    const int r = idx & 63, c = idx >> 6;
+  for (int idx = tid; idx < W * W; idx += 256) {
+    const int r = idx % W, c = idx / W;
     if (r < c) {
       const double a = G[r][r].x, d = G[c][c].x;
       if (a > zt && d > zt) {
@@ -183,13 +196,13 @@ __global__ __launch_bounds__(256) void jac_inner(
     if (tid == 0) flags[bz * p + pi] = 0;
     return;
   }
-  const int pr = tid >> 3;  // rotation pair of this thread
-  const int t = tid & 7;
+  const int pr = tid / TP;  // rotation pair of this thread
+  const int t = tid % TP;
   for (int sweep = 0; sweep < n_inner; ++sweep) {
     int sweep_rot = 0;
-    for (int round = 0; round < 63; ++round) {
-      const int c1 = rr_block(pr, round, 64);
-      const int c2 = rr_block(63 - pr, round, 64);
+    for (int round = 0; round < W - 1; ++round) {
+      const int c1 = rr_block(pr, round, W);
+      const int c2 = rr_block(W - 1 - pr, round, W);
       const int pp = min(c1, c2), qq = max(c1, c2);
       const double a = G[pp][pp].x, d = G[qq][qq].x;
       const double2 g = G[pp][qq];
@@ -203,29 +216,40 @@ __global__ __launch_bounds__(256) void jac_inner(
                           (fabs(zeta) + sqrt(1.0 + zeta * zeta));
         cs = 1.0 / sqrt(1.0 + tt * tt);
         sn = tt * cs;
-        er = g.x / absc;   // e = e^{-i phi} = conj(g) / |g|
+        er = g.x / absc;  // e = e^{-i phi} = conj(g) / |g|
         ei = -g.y / absc;
         ap = a - tt * absc;
         dq = d + tt * absc;
         sweep_rot += (t == 0);
         // column update: [x_p x_q] <- [x_p x_q] J,
         // J = [[cs, sn], [-sn e, cs e]]
+        double2 gp[RPT], gq[RPT], qp[RPT], qv[RPT];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const int r = t + 8 * j;
+        for (int j = 0; j < RPT; ++j) {
+          const int r = t + TP * j;
+          gp[j] = G[r][pp];
+          gq[j] = G[r][qq];
+          qp[j] = Q[r][pp];
+          qv[j] = Q[r][qq];
+        }
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+          const int r = t + TP * j;
           {
-            const double2 xp = G[r][pp], xq = G[r][qq];
-            const double yr = er * xq.x - ei * xq.y;
-            const double yi = er * xq.y + ei * xq.x;
-            G[r][pp] = make_double2(cs * xp.x - sn * yr, cs * xp.y - sn * yi);
-            G[r][qq] = make_double2(sn * xp.x + cs * yr, sn * xp.y + cs * yi);
+            const double yr = er * gq[j].x - ei * gq[j].y;
+            const double yi = er * gq[j].y + ei * gq[j].x;
+            G[r][pp] = make_double2(cs * gp[j].x - sn * yr,
+                                    cs * gp[j].y - sn * yi);
+            G[r][qq] = make_double2(sn * gp[j].x + cs * yr,
+                                    sn * gp[j].y + cs * yi);
           }
           {
-            const double2 xp = Q[r][pp], xq = Q[r][qq];
-            const double yr = er * xq.x - ei * xq.y;
-            const double yi = er * xq.y + ei * xq.x;
-            Q[r][pp] = make_double2(cs * xp.x - sn * yr, cs * xp.y - sn * yi);
-            Q[r][qq] = make_double2(sn * xp.x + cs * yr, sn * xp.y + cs * yi);
+            const double yr = er * qv[j].x - ei * qv[j].y;
+            const double yi = er * qv[j].y + ei * qv[j].x;
+            Q[r][pp] = make_double2(cs * qp[j].x - sn * yr,
+                                    cs * qp[j].y - sn * yi);
+            Q[r][qq] = make_double2(sn * qp[j].x + cs * yr,
+                                    sn * qp[j].y + cs * yi);
           }
         }
       }
@@ -233,14 +257,21 @@ __global__ __launch_bounds__(256) void jac_inner(
       if (on) {
         // row update: [r_p; r_q] <- J^H [r_p; r_q],
         // J^H = [[cs, -sn conj(e)], [sn, cs conj(e)]]
+        double2 xp[RPT], xq[RPT];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const int c = t + 8 * j;
-          const double2 xp = G[pp][c], xq = G[qq][c];
-          const double yr = er * xq.x + ei * xq.y;
-          const double yi = er * xq.y - ei * xq.x;
-          double2 np = make_double2(cs * xp.x - sn * yr, cs * xp.y - sn * yi);
-          double2 nq = make_double2(sn * xp.x + cs * yr, sn * xp.y + cs * yi);
+        for (int j = 0; j < RPT; ++j) {
+          xp[j] = G[pp][t + TP * j];
+          xq[j] = G[qq][t + TP * j];
+        }
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+          const int c = t + TP * j;
+          const double yr = er * xq[j].x + ei * xq[j].y;
+          const double yi = er * xq[j].y - ei * xq[j].x;
+          double2 np = make_double2(cs * xp[j].x - sn * yr,
+                                    cs * xp[j].y - sn * yi);
+          double2 nq = make_double2(sn * xp[j].x + cs * yr,
+                                    sn * xp[j].y + cs * yi);
           // the 2x2 block becomes diag(ap, dq) exactly
           if (c == pp) {
             np = make_double2(ap, 0.0);
@@ -263,19 +294,19 @@ __global__ __launch_bounds__(256) void jac_inner(
     if (tid == 0) nrot = 0;
     if (total == 0) break;
     __syncthreads();
-    if (tid == 0) any_off += total;  // reuse as the running total
+    if (tid == 0) any_off += total;  // running total (started at 1)
     __syncthreads();
   }
   __syncthreads();
-  const int total = any_off - 1;  // any_off started at 1 after the check
+  const int total = any_off - 1;
   if (tid == 0) {
     flags[bz * p + pi] = total > 0 ? 1 : 0;
     if (total > 0) atomicAdd(&counters[bz], total);
   }
   if (total > 0) {
-    double2* q = Qout + ((int64_t)bz * p + pi) * (int64_t)4096;
-    for (int idx = tid; idx < 4096; idx += 256) {
-      const int r = idx & 63, c = idx >> 6;
+    double2* q = Qout + ((int64_t)bz * p + pi) * (int64_t)(W * W);
+    for (int idx = tid; idx < W * W; idx += 256) {
+      const int r = idx % W, c = idx / W;
       q[idx] = Q[r][c];
     }
   }
@@ -283,10 +314,12 @@ __global__ __launch_bounds__(256) void jac_inner(
 
 // ---------------------------------------------------------------------------
 // 3. X_IJ <- X_IJ Q for one 64-row chunk.  grid (rows/64, p, batch).
+template <int W>
 __global__ __launch_bounds__(256) void jac_update(
     double2* __restrict__ X, int64_t rows, int64_t sX, int p, int round,
     const double2* __restrict__ Q, const int* __restrict__ flags,
     const int* __restrict__ active, const int* __restrict__ want) {
+  constexpr int T = W / 32;
   const int bz = blockIdx.z;
   if (!active[bz]) return;
   if (want && !want[bz]) return;
@@ -298,30 +331,34 @@ __global__ __launch_bounds__(256) void jac_update(
   const int b2 = rr_block(nb - 1 - pi, round, nb);
   const int I = min(b1, b2), J = max(b1, b2);
   X += (int64_t)bz * sX;
-  const double2* Qp = Q + ((int64_t)bz * p + pi) * (int64_t)4096;
+  const double2* Qp = Q + ((int64_t)bz * p + pi) * (int64_t)(W * W);
 
-  __shared__ double Xr[64][65];  // [local col][row]
-  __shared__ double Xi[64][65];
-  __shared__ double Qr[64][65];  // [k][n]
-  __shared__ double Qi[64][65];
+  __shared__ double Xr[W][65];  // [local col][row]
+  __shared__ double Xi[W][65];
+  __shared__ double Qr[W][W + 1];  // [k][n]
+  __shared__ double Qi[W][W + 1];
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < 4096; idx += 256) {
+  for (int idx = tid; idx < 64 * W; idx += 256) {
     const int r = idx & 63, c = idx >> 6;
-    const double2 v = X[(int64_t)chunk * 64 + r + pair_col(c, I, J) * rows];
+    const double2 v = X[(int64_t)chunk * 64 + r + pair_col<W>(c, I, J) * rows];
     Xr[c][r] = v.x;
     Xi[c][r] = v.y;
+  }
+  for (int idx = tid; idx < W * W; idx += 256) {
+    const int r = idx % W, c = idx / W;
     const double2 qv = Qp[idx];  // Q[r][c] col-major
     Qr[r][c] = qv.x;
     Qi[r][c] = qv.y;
   }
   __syncthreads();
   const int lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-  d4_t yr[2][2], yi[2][2];
+  // output 64 rows x W cols; wave tile (32) x (W/2)
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * (W / 2);
+  d4_t yr[2][T], yi[2][T];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < T; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         yr[i][j][e] = 0.0;
@@ -329,20 +366,23 @@ __global__ __launch_bounds__(256) void jac_update(
       }
   // Y[row][n] = sum_k X[row][k] Q[k][n]
 #pragma unroll 4
-  for (int ks = 0; ks < 64; ks += 4) {
+  for (int ks = 0; ks < W; ks += 4) {
     const int kr = ks + (lane >> 4);
-    double ar[2], ai[2], br[2], bi[2];
+    double ar[2], ai[2], br[T], bi[T];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       ar[i] = Xr[kr][wm + i * 16 + (lane & 15)];
       ai[i] = Xi[kr][wm + i * 16 + (lane & 15)];
-      br[i] = Qr[kr][wn + i * 16 + (lane & 15)];
-      bi[i] = Qi[kr][wn + i * 16 + (lane & 15)];
+    }
+#pragma unroll
+    for (int j = 0; j < T; ++j) {
+      br[j] = Qr[kr][wn + j * 16 + (lane & 15)];
+      bi[j] = Qi[kr][wn + j * 16 + (lane & 15)];
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
+      for (int j = 0; j < T; ++j) {
         yr[i][j] = mfma64(ar[i], br[j], yr[i][j]);
         yr[i][j] = mfma64(-ai[i], bi[j], yr[i][j]);
         yi[i][j] = mfma64(ar[i], bi[j], yi[i][j]);
@@ -352,9 +392,9 @@ __global__ __launch_bounds__(256) void jac_update(
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < T; ++j) {
       const int c = wn + j * 16 + (lane & 15);
-      const int64_t gc = pair_col(c, I, J);
+      const int64_t gc = pair_col<W>(c, I, J);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int r = wm + i * 16 + 4 * e + (lane >> 4);
@@ -531,14 +571,20 @@ int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
   batch = batch_;
   rows_p = rows_p_;
   cols_p = cols_p_;
-  p = (int)(cols_p / 64);
+  // Block-pair width: one 64-column pair solved entirely in LDS for narrow
+  // problems; 32-column pairs otherwise (the inner sweep's LDS traffic per
+  // outer sweep grows linearly with W while the MFMA work does not change).
+  int wdef = 32;
+  if (const char* e = getenv("QTN_JACOBI_W")) wdef = atoi(e) == 64 ? 64 : 32;
+  W = (cols_p <= 64) ? 64 : wdef;
+  p = (int)(cols_p / W);
   const int64_t nchunk = std::max(rows_p, cols_p) / 64;
   void* ptr;
   QTN_TRY(ctx->buffer(kSlotGram,
-                      (size_t)batch * p * nchunk * 4096 * sizeof(double2),
+                      (size_t)batch * p * nchunk * W * W * sizeof(double2),
                       &ptr));
   Gp = (double2*)ptr;
-  QTN_TRY(ctx->buffer(kSlotQ, (size_t)batch * p * 4096 * sizeof(double2),
+  QTN_TRY(ctx->buffer(kSlotQ, (size_t)batch * p * W * W * sizeof(double2),
                       &ptr));
   Q = (double2*)ptr;
   // small per-problem arrays packed in one slot
@@ -581,7 +627,27 @@ int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
   return QTN_OK;
 }
 
-int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t sX, double2* W,
+template <int WW>
+int32_t JacobiEngine::launch_round(hipStream_t s, double2* X, int64_t sX,
+                                   double2* Wm, int64_t sW, int r, double tol,
+                                   int n_inner, int nchunk_x, int nchunk_w,
+                                   bool per_problem_w) {
+  hipLaunchKernelGGL(jac_gram<WW>, dim3(nchunk_x, p, batch), dim3(256), 0, s,
+                     (const double2*)X, rows_p, sX, p, r, Gp, active);
+  hipLaunchKernelGGL(jac_inner<WW>, dim3(p, batch), dim3(256), 0, s,
+                     (const double2*)Gp, nchunk_x, p, tol, zthr, n_inner, Q,
+                     flags, counters, active);
+  hipLaunchKernelGGL(jac_update<WW>, dim3(nchunk_x, p, batch), dim3(256), 0, s,
+                     X, rows_p, sX, p, r, (const double2*)Q, flags, active,
+                     (const int*)nullptr);
+  if (Wm)
+    hipLaunchKernelGGL(jac_update<WW>, dim3(nchunk_w, p, batch), dim3(256), 0,
+                       s, Wm, cols_p, sW, p, r, (const double2*)Q, flags,
+                       active, (const int*)(per_problem_w ? want_w : nullptr));
+  return QTN_OK;
+}
+
+int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t sX, double2* Wm,
                           int64_t sW, const int* want_w_host,
                           const int64_t* nsv_host, double cutoff,
                           int relative, int64_t chimax, int max_sweeps,
@@ -608,9 +674,9 @@ int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t sX, double2* W,
                      rows_p * cols_p, sX, (double)rows_p * DBL_EPSILON,
                      normF2, zthr, active, counters, misc + 1);
   QTN_CHECK_HIP(hipGetLastError());
-  if (W) {
+  if (Wm) {
     // W starts as the identity on the padded columns.
-    QTN_TRY(launch_identity(s, W, cols_p, sW, batch));
+    QTN_TRY(launch_identity(s, Wm, cols_p, sW, batch));
   }
   int* host_flags = nullptr;
   QTN_TRY(ctx->pinned_buffer(4 * sizeof(int), (void**)&host_flags));
@@ -618,19 +684,12 @@ int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t sX, double2* W,
   bool converged = false;
   for (; sweep < max_sweeps; ++sweep) {
     for (int r = 0; r < rounds; ++r) {
-      hipLaunchKernelGGL(jac_gram, dim3(nchunk_x, p, batch), dim3(256), 0, s,
-                         (const double2*)X, rows_p, sX, p, r, Gp, active);
-      hipLaunchKernelGGL(jac_inner, dim3(p, batch), dim3(256), 0, s,
-                         (const double2*)Gp, nchunk_x, p, tol, zthr, n_inner,
-                         Q, flags, counters, active);
-      hipLaunchKernelGGL(jac_update, dim3(nchunk_x, p, batch), dim3(256), 0, s,
-                         X, rows_p, sX, p, r, (const double2*)Q, flags,
-                         active, (const int*)nullptr);
-      if (W)
-        hipLaunchKernelGGL(jac_update, dim3(nchunk_w, p, batch), dim3(256), 0,
-                           s, W, cols_p, sW, p, r, (const double2*)Q, flags,
-                           active, (const int*)(want_w_host ? want_w
-                                                            : nullptr));
+      if (this->W == 64)
+        QTN_TRY(launch_round<64>(s, X, sX, Wm, sW, r, tol, n_inner, nchunk_x,
+                                 nchunk_w, want_w_host != nullptr));
+      else
+        QTN_TRY(launch_round<32>(s, X, sX, Wm, sW, r, tol, n_inner, nchunk_x,
+                                 nchunk_w, want_w_host != nullptr));
     }
     QTN_CHECK_HIP(hipGetLastError());
     hipLaunchKernelGGL(jac_sweep_end, dim3(1), dim3(256), 0, s, active,

@@ -18,7 +18,8 @@ struct JacobiEngine {
   int batch = 0;
   int64_t rows_p = 0;  // padded rows of X (multiple of 64)
   int64_t cols_p = 0;  // padded columns of X (multiple of 64)
-  int p = 0;           // block pairs per round (= cols_p / 64)
+  int W = 64;          // block-pair width (columns of one Gram block)
+  int p = 0;           // block pairs per round (= cols_p / W)
   // device arrays (valid after reserve)
   double2* Gp = nullptr;
   double2* Q = nullptr;
@@ -46,7 +47,11 @@ struct JacobiEngine {
   // Afterwards sig / perm hold the sorted column norms and kept / disc /
   // total / status the truncation decision for nsv_host[b] meaningful
   // values.
-  int32_t run(qtn_ctx* ctx, double2* X, int64_t sX, double2* W, int64_t sW,
+  template <int WW>
+  int32_t launch_round(hipStream_t s, double2* X, int64_t sX, double2* Wm,
+                       int64_t sW, int r, double tol, int n_inner,
+                       int nchunk_x, int nchunk_w, bool per_problem_w);
+  int32_t run(qtn_ctx* ctx, double2* X, int64_t sX, double2* Wm, int64_t sW,
               const int* want_w_host, const int64_t* nsv_host, double cutoff,
               int relative, int64_t chimax, int max_sweeps, int* sweeps_done);
 };
