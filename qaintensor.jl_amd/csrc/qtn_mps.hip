@@ -14,10 +14,12 @@
 // (a, s) with a fastest, columns (t, b) with t fastest, column-major.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
 #include "qtn_internal.hpp"
+#include "qtn_qr.hpp"
 #include "qtn_svd.hpp"
 
 struct qtn_mps {
@@ -36,7 +38,9 @@ namespace {
 
 constexpr int kSlotPhi = 30, kSlotXm = 31, kSlotVk = 32, kSlotDesc = 33,
               kSlotGates = 34, kSlotEnv = 35, kSlotT1 = 36, kSlotT2 = 37,
-              kSlotRed = 38, kSlotBits = 39, kSlotAmp = 40, kSlotW2 = 41;
+              kSlotRed = 38, kSlotBits = 39, kSlotAmp = 40, kSlotW2 = 41,
+              kSlotA1 = 42, kSlotA2 = 43, kSlotQd = 44, kSlotNrm = 45,
+              kSlotPerm2 = 46;
 
 unsigned grid_for(int64_t n) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256,
@@ -61,6 +65,8 @@ struct GateDesc {
   const double2* g;   // 4x4 gate
   int64_t chiL, chiM, chiR;
   int trans;          // 1: X = theta^H (M <= N), 0: X = theta (M > N)
+  int mode;           // V_k from: 0 X columns, 1 Q1 [X columns; 0], 2 W
+  int64_t ldx;        // leading dimension of X
 };
 
 // Phi <- g . Phi in place and X = (lambda_a Phi)^H (or lambda_a Phi when the
@@ -104,45 +110,62 @@ __global__ void gate_epilogue(const GateDesc* __restrict__ gd,
         const int64_t row = a + chiL * s, col = t + 2 * b;
         d.phi[row + M * col] = v;
         if (d.trans)
-          d.X[col + rows_p * row] = make_double2(la * v.x, -la * v.y);
+          d.X[col + d.ldx * row] = make_double2(la * v.x, -la * v.y);
         else
-          d.X[row + rows_p * col] = make_double2(la * v.x, la * v.y);
+          d.X[row + d.ldx * col] = make_double2(la * v.x, la * v.y);
       }
   }
 }
 
-// V_k[:, j] = X[:, perm[j]] / sig[j] (X = theta^H) or W[:, perm[j]]
-// (X = theta), N rows; B_{i+1}[j, c] = conj(V_k[c, j]) (k x N, ld k);
-// lambda_i = sig / |S_k|.
-__global__ void split_right(const GateDesc* __restrict__ gd,
-                            const double* __restrict__ sig,
-                            const int* __restrict__ perm,
-                            const int64_t* __restrict__ kept,
-                            const double* __restrict__ keptw, int64_t rows_p,
-                            int64_t cols_p) {
+// V_k (N x k, ld N): mode 0: X[:, perm[j]] / sig[j]; mode 1: the first M
+// rows from X[0:M, perm[j]] / sig[j] and zeros below (Q1 is applied
+// afterwards); mode 2: W[:, perm[j]].  grid (tiles, batch).
+__global__ void extract_vk(const GateDesc* __restrict__ gd,
+                           const double* __restrict__ sig,
+                           const int* __restrict__ perm,
+                           const int64_t* __restrict__ kept, int64_t cols_p) {
   const int bz = blockIdx.y;
   const GateDesc d = gd[bz];
   const int64_t k = kept[bz];
-  const int64_t N = 2 * d.chiR;
+  const int64_t N = 2 * d.chiR, M = 2 * d.chiL;
   const double* s = sig + (int64_t)bz * cols_p;
   const int* pm = perm + (int64_t)bz * cols_p;
   const int64_t total = N * k;
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
        idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
     const int64_t c = idx % N, j = idx / N;
-    double2 v;
-    if (d.trans) {
+    double2 v = make_double2(0.0, 0.0);
+    if (d.mode == 2) {
+      v = d.W[c + cols_p * (int64_t)pm[j]];
+    } else if (d.mode == 0 || c < M) {
       const double sj = s[j];
       const double inv = sj > 0.0 ? 1.0 / sj : 0.0;
-      const double2 x = d.X[c + rows_p * (int64_t)pm[j]];
+      const double2 x = d.X[c + d.ldx * (int64_t)pm[j]];
       v = make_double2(x.x * inv, x.y * inv);
-    } else {
-      v = d.W[c + cols_p * (int64_t)pm[j]];
     }
     d.Vk[c + N * j] = v;
+  }
+}
+
+// B_{i+1}[j, c] = conj(V_k[c, j]) (k x N, ld k); lambda_i = sig / |S_k|.
+__global__ void write_right(const GateDesc* __restrict__ gd,
+                            const double* __restrict__ sig,
+                            const int64_t* __restrict__ kept,
+                            const double* __restrict__ keptw,
+                            int64_t cols_p) {
+  const int bz = blockIdx.y;
+  const GateDesc d = gd[bz];
+  const int64_t k = kept[bz];
+  const int64_t N = 2 * d.chiR;
+  const int64_t total = N * k;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+       idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = idx % N, j = idx / N;
+    const double2 v = d.Vk[c + N * j];
     d.Br[j + k * c] = make_double2(v.x, -v.y);
   }
   if (blockIdx.x == 0) {
+    const double* s = sig + (int64_t)bz * cols_p;
     const double nrm = sqrt(keptw[bz]);
     for (int64_t j = threadIdx.x; j < k; j += blockDim.x)
       d.lamM[j] = s[j] / nrm;
@@ -257,6 +280,8 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
   }
   const int64_t rows_p = round_up64(maxXr), cols_p = round_up64(maxXc);
   const int64_t phi_sz = maxM * maxN, x_sz = rows_p * cols_p;
+  bool precond = true;
+  if (const char* e = getenv("QTN_JACOBI_PRECOND")) precond = atoi(e) != 0;
   const int64_t w_sz = cols_p * cols_p;
   void *dphi, *dX, *dVk, *ddesc, *dg, *dW = nullptr;
   if (any_w)
@@ -287,6 +312,9 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
     d.X = (double2*)dX + (size_t)g * x_sz;
     d.W = dW ? (double2*)dW + (size_t)g * w_sz : nullptr;
     d.trans = trans[g];
+    d.ldx = rows_p;
+    // QR-preconditioned Jacobi for theta^H wider than one block pair
+    d.mode = !trans[g] ? 2 : (precond && 2 * d.chiL > 64 ? 1 : 0);
     d.Vk = (double2*)dVk + (size_t)g * maxN * std::max(maxM, maxN);
     d.g = (const double2*)dg + 16 * g;
     const int64_t M = 2 * d.chiL, N = 2 * d.chiR;
@@ -321,13 +349,56 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
                      dim3(grid_for(maxM * maxN / 4), (unsigned)ng), dim3(256),
                      0, s, (const GateDesc*)d_gd, rows_p);
   QTN_CHECK_HIP(hipGetLastError());
-  // 3. SVD of every theta
+  // 3. SVD of every theta.  Mode-1 gates are preconditioned:
+  //    X P = Q1 R1 (columns sorted by norm), R1^H = Q2 R2, and the Jacobi
+  //    iteration runs on Y = R2^H (M x M, written over X).
+  std::vector<QrDims> qd(ng), qd2(ng);
+  bool any_pre = false, all_pre = true;
+  for (int g = 0; g < ng; ++g) {
+    const bool on = gd[g].mode == 1;
+    const int64_t M = 2 * gd[g].chiL, N = 2 * gd[g].chiR;
+    qd[g] = QrDims{N, M, on ? 1 : 0};
+    qd2[g] = QrDims{M, M, on ? 1 : 0};
+    any_pre = any_pre || on;
+    all_pre = all_pre && on;
+  }
+  BatchedQR qr1, qr2;
+  qr1.slot_base = 50;
+  qr2.slot_base = 60;
+  if (any_pre) {
+    void *dA1, *dA2, *pq;
+    QTN_TRY(ctx->buffer(kSlotA1, (size_t)ng * x_sz * sizeof(double2), &dA1));
+    QTN_TRY(ctx->buffer(kSlotA2, (size_t)ng * cols_p * cols_p *
+                                     sizeof(double2),
+                        &dA2));
+    QTN_TRY(ctx->buffer(kSlotQd, (size_t)ng * 2 * sizeof(QrDims), &pq));
+    QrDims* d_qd = (QrDims*)pq;
+    QTN_CHECK_HIP(hipMemcpyAsync(d_qd, qd.data(), ng * sizeof(QrDims),
+                                 hipMemcpyHostToDevice, s));
+    QTN_CHECK_HIP(hipMemcpyAsync(d_qd + ng, qd2.data(), ng * sizeof(QrDims),
+                                 hipMemcpyHostToDevice, s));
+    QTN_TRY(qr_sort_columns(ctx, (const double2*)dX, (double2*)dA1, rows_p,
+                            cols_p, x_sz, ng, d_qd, kSlotNrm, kSlotPerm2,
+                            nullptr));
+    QTN_TRY(qr1.factor(ctx, (double2*)dA1, rows_p, x_sz, cols_p, qd, true));
+    QTN_TRY(qr_r_to_adjoint(ctx, (const double2*)dA1, rows_p, x_sz,
+                            (double2*)dA2, cols_p, cols_p * cols_p, cols_p,
+                            cols_p, ng, d_qd));
+    QTN_TRY(qr2.factor(ctx, (double2*)dA2, cols_p, cols_p * cols_p, cols_p,
+                       qd2, false));
+    QTN_TRY(qr_r_to_adjoint(ctx, (const double2*)dA2, cols_p,
+                            cols_p * cols_p, (double2*)dX, rows_p, x_sz,
+                            rows_p, cols_p, ng, d_qd + ng));
+    QTN_CHECK_HIP(hipStreamSynchronize(s));  // qd vectors live on the host
+  }
   JacobiEngine eng;
-  QTN_TRY(eng.reserve(ctx, ng, rows_p, cols_p));
+  // every problem preconditioned: only the top cols_p rows are non-zero
+  const int64_t rows_eng = all_pre ? cols_p : rows_p;
+  QTN_TRY(eng.reserve(ctx, ng, rows_eng, cols_p));
   std::vector<int> wantw(ng);
   for (int g = 0; g < ng; ++g) wantw[g] = trans[g] ? 0 : 1;
   int sweeps = 0;
-  QTN_TRY(eng.run(ctx, (double2*)dX, x_sz, (double2*)dW, w_sz,
+  QTN_TRY(eng.run(ctx, (double2*)dX, rows_p, x_sz, (double2*)dW, w_sz,
                   any_w ? wantw.data() : nullptr, nsv.data(), cutoff, 1,
                   chimax, 60, &sweeps));
   std::vector<int64_t> kept(ng);
@@ -356,14 +427,28 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
                             hipMemcpyDeviceToHost));
     if (spec_len) *spec_len = nsv[0];
   }
-  // 4. B_{i+1} = V_k^H, lambda_i
+  // 4. V_k, then B_{i+1} = V_k^H and lambda_i
   int64_t maxk = 0;
   for (int g = 0; g < ng; ++g) maxk = std::max(maxk, kept[g]);
-  hipLaunchKernelGGL(split_right,
-                     dim3(grid_for(maxN * maxk), (unsigned)ng), dim3(256), 0,
-                     s, (const GateDesc*)d_gd, (const double*)eng.sig,
-                     (const int*)eng.perm, (const int64_t*)eng.kept,
-                     (const double*)eng.keptw, rows_p, cols_p);
+  hipLaunchKernelGGL(extract_vk, dim3(grid_for(maxN * maxk), (unsigned)ng),
+                     dim3(256), 0, s, (const GateDesc*)d_gd,
+                     (const double*)eng.sig, (const int*)eng.perm,
+                     (const int64_t*)eng.kept, cols_p);
+  QTN_CHECK_HIP(hipGetLastError());
+  if (any_pre) {
+    // V_k = Q1 [U_k; 0]
+    std::vector<int64_t> ldu(ng), ucols(ng);
+    for (int g = 0; g < ng; ++g) {
+      ldu[g] = 2 * gd[g].chiR;
+      ucols[g] = gd[g].mode == 1 ? kept[g] : 0;
+    }
+    const int64_t vk_stride = maxN * std::max(maxM, maxN);
+    QTN_TRY(qr1.apply_q(ctx, (double2*)dVk, ldu, vk_stride, ucols, qd));
+  }
+  hipLaunchKernelGGL(write_right, dim3(grid_for(maxN * maxk), (unsigned)ng),
+                     dim3(256), 0, s, (const GateDesc*)d_gd,
+                     (const double*)eng.sig, (const int64_t*)eng.kept,
+                     (const double*)eng.keptw, cols_p);
   QTN_CHECK_HIP(hipGetLastError());
   // 5. B_i = Phi V_k / |S_k|
   std::vector<GemmDesc<double>> g2(ng);

@@ -59,7 +59,7 @@ __device__ __forceinline__ int64_t pair_col(int c, int I, int J) {
 // Each of the 4 waves computes a (W/2) x (W/2) quadrant.
 template <int W>
 __global__ __launch_bounds__(256) void jac_gram(
-    const double2* __restrict__ X, int64_t rows_p, int64_t sX, int p,
+    const double2* __restrict__ X, int64_t ldx, int64_t sX, int p,
     int round, double2* __restrict__ Gp, const int* __restrict__ active) {
   constexpr int T = W / 32;  // 16x16 MFMA tiles per wave per dimension
   const int bz = blockIdx.z;
@@ -79,13 +79,16 @@ __global__ __launch_bounds__(256) void jac_gram(
   for (int idx = tid; idx < 64 * W; idx += 256) {
     const int r = idx & 63, c = idx >> 6;
     const double2 v =
-        X[(int64_t)chunk * 64 + r + pair_col<W>(c, I, J) * rows_p];
+        X[(int64_t)chunk * 64 + r + pair_col<W>(c, I, J) * ldx];
     Xr[c][r] = v.x;
     Xi[c][r] = v.y;
   }
   __syncthreads();
 
   const int lane = tid & 63, wave = tid >> 6;
+  // G is Hermitian: the lower-left quadrant (wave 1) is conj(upper-right)^T
+  // and is filled in by jac_inner.
+  if (wave == 1) return;
   const int wm = (wave & 1) * (W / 2), wn = (wave >> 1) * (W / 2);
   d4_t gr[T][T], gi[T][T];
 #pragma unroll
@@ -147,7 +150,7 @@ __global__ __launch_bounds__(256) void jac_inner(
     const double2* __restrict__ Gp, int nchunk, int p, double tol,
     const double* __restrict__ zthr, int n_inner, double2* __restrict__ Qout,
     int* __restrict__ flags, int* __restrict__ counters,
-    const int* __restrict__ active) {
+    const int* __restrict__ active, double* __restrict__ offmax) {
   constexpr int NP = W / 2;       // rotation pairs per round
   constexpr int TP = 256 / NP;    // threads per pair
   constexpr int RPT = W / TP;     // rows (columns) per thread
@@ -163,6 +166,8 @@ __global__ __launch_bounds__(256) void jac_inner(
       Gp + ((int64_t)bz * p + pi) * nchunk * (int64_t)(W * W);
   for (int idx = tid; idx < W * W; idx += 256) {
     const int r = idx % W, c = idx / W;
+    Q[r][c] = make_double2(r == c ? 1.0 : 0.0, 0.0);
+    if (r >= W / 2 && c < W / 2) continue;  // lower-left: by symmetry below
     double sr = 0.0, si = 0.0;
     for (int ch = 0; ch < nchunk; ++ch) {
       const double2 v = src[(int64_t)ch * (W * W) + idx];
@@ -170,7 +175,6 @@ __global__ __launch_bounds__(256) void jac_inner(
       si += v.y;
     }
     G[r][c] = make_double2(sr, si);
-    Q[r][c] = make_double2(r == c ? 1.0 : 0.0, 0.0);
   }
   if (tid == 0) {
     any_off = 0;
@@ -179,18 +183,31 @@ __global__ __launch_bounds__(256) void jac_inner(
   const double zt = zthr[bz];
   const double tol2 = tol * tol;
   __syncthreads();
-  // Nothing to do when every pair already meets the tolerance (the common
-  // case in late sweeps): skip the rounds.
+  for (int idx = tid; idx < (W / 2) * (W / 2); idx += 256) {
+    const int r = W / 2 + idx % (W / 2), c = idx / (W / 2);
+    const double2 u = G[c][r];
+    G[r][c] = make_double2(u.x, -u.y);
+  }
+  __syncthreads();
+  // Largest scaled off-diagonal |G_pq|^2 / (G_pp G_qq) of the block; nothing
+  // to do when it meets the tolerance (the common case in late sweeps).
+  double omax = 0.0;
   for (int idx = tid; idx < W * W; idx += 256) {
     const int r = idx % W, c = idx / W;
     if (r < c) {
       const double a = G[r][r].x, d = G[c][c].x;
       if (a > zt && d > zt) {
         const double2 g = G[r][c];
-        if (g.x * g.x + g.y * g.y > tol2 * a * d) atomicOr(&any_off, 1);
+        const double o2 = (g.x * g.x + g.y * g.y) / (a * d);
+        omax = fmax(omax, o2);
+        if (o2 > tol2) atomicOr(&any_off, 1);
       }
     }
   }
+  // positive doubles order like their bit patterns
+  if (omax > 0.0)
+    atomicMax((unsigned long long*)&offmax[bz],
+              (unsigned long long)__double_as_longlong(omax));
   __syncthreads();
   if (!any_off) {
     if (tid == 0) flags[bz * p + pi] = 0;
@@ -210,14 +227,15 @@ __global__ __launch_bounds__(256) void jac_inner(
       const bool on = (a > zt && d > zt && absc2 > tol2 * a * d);
       double cs = 1.0, sn = 0.0, er = 1.0, ei = 0.0, ap = a, dq = d;
       if (on) {
-        const double absc = sqrt(absc2);
-        const double zeta = (d - a) / (2.0 * absc);
+        const double inv = rsqrt(absc2);
+        const double absc = absc2 * inv;
+        const double zeta = 0.5 * (d - a) * inv;
         const double tt = (zeta >= 0.0 ? 1.0 : -1.0) /
                           (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-        cs = 1.0 / sqrt(1.0 + tt * tt);
+        cs = rsqrt(1.0 + tt * tt);
         sn = tt * cs;
-        er = g.x / absc;  // e = e^{-i phi} = conj(g) / |g|
-        ei = -g.y / absc;
+        er = g.x * inv;  // e = e^{-i phi} = conj(g) / |g|
+        ei = -g.y * inv;
         ap = a - tt * absc;
         dq = d + tt * absc;
         sweep_rot += (t == 0);
@@ -316,7 +334,7 @@ __global__ __launch_bounds__(256) void jac_inner(
 // 3. X_IJ <- X_IJ Q for one 64-row chunk.  grid (rows/64, p, batch).
 template <int W>
 __global__ __launch_bounds__(256) void jac_update(
-    double2* __restrict__ X, int64_t rows, int64_t sX, int p, int round,
+    double2* __restrict__ X, int64_t ldx, int64_t sX, int p, int round,
     const double2* __restrict__ Q, const int* __restrict__ flags,
     const int* __restrict__ active, const int* __restrict__ want) {
   constexpr int T = W / 32;
@@ -340,7 +358,7 @@ __global__ __launch_bounds__(256) void jac_update(
   const int tid = threadIdx.x;
   for (int idx = tid; idx < 64 * W; idx += 256) {
     const int r = idx & 63, c = idx >> 6;
-    const double2 v = X[(int64_t)chunk * 64 + r + pair_col<W>(c, I, J) * rows];
+    const double2 v = X[(int64_t)chunk * 64 + r + pair_col<W>(c, I, J) * ldx];
     Xr[c][r] = v.x;
     Xi[c][r] = v.y;
   }
@@ -398,23 +416,29 @@ __global__ __launch_bounds__(256) void jac_update(
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int r = wm + i * 16 + 4 * e + (lane >> 4);
-        X[(int64_t)chunk * 64 + r + gc * rows] =
+        X[(int64_t)chunk * 64 + r + gc * ldx] =
             make_double2(yr[i][j][e], yi[i][j][e]);
       }
     }
 }
 
-// After a sweep: a problem stays active only if it rotated something.
+// After a sweep: a problem stays active only if it rotated something and
+// the largest scaled off-diagonal |x_p^H x_q|^2 / (|x_p|^2 |x_q|^2) met in
+// the sweep was above conv2.  Below it the sweep's own rotations leave the
+// columns orthogonal to O(conv2) (cyclic Jacobi converges quadratically),
+// so the confirming sweep is skipped.
 __global__ void jac_sweep_end(int* __restrict__ active,
-                              int* __restrict__ counters, int batch,
-                              int* __restrict__ n_active) {
+                              int* __restrict__ counters,
+                              double* __restrict__ offmax, double conv2,
+                              int batch, int* __restrict__ n_active) {
   __shared__ int cnt;
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
   for (int b = threadIdx.x; b < batch; b += blockDim.x) {
-    const int a = active[b] && counters[b] > 0;
+    const int a = active[b] && counters[b] > 0 && offmax[b] > conv2;
     active[b] = a;
     counters[b] = 0;
+    offmax[b] = 0.0;
     if (a) atomicAdd(&cnt, 1);
   }
   __syncthreads();
@@ -424,7 +448,8 @@ __global__ void jac_sweep_end(int* __restrict__ active,
 // Frobenius norm^2 per problem, finiteness check, zero-column threshold.
 // grid (batch), 256 threads.
 __global__ __launch_bounds__(256) void jac_prepare(
-    const double2* __restrict__ X, int64_t n_elem, int64_t sX, double zrel,
+    const double2* __restrict__ X, int64_t rows, int64_t ldx, int64_t cols,
+    int64_t sX, double zrel,
     double* __restrict__ normF2, double* __restrict__ zthr,
     int* __restrict__ active, int* __restrict__ counters,
     int* __restrict__ bad) {
@@ -432,8 +457,8 @@ __global__ __launch_bounds__(256) void jac_prepare(
   X += (int64_t)bz * sX;
   double s = 0.0;
   int nonfinite = 0;
-  for (int64_t i = threadIdx.x; i < n_elem; i += 256) {
-    const double2 v = X[i];
+  for (int64_t i = threadIdx.x; i < rows * cols; i += 256) {
+    const double2 v = X[(i % rows) + (i / rows) * ldx];
     if (!isfinite(v.x) || !isfinite(v.y)) nonfinite = 1;
     s += v.x * v.x + v.y * v.y;
   }
@@ -461,13 +486,13 @@ __global__ __launch_bounds__(256) void jac_prepare(
 
 // Column norms: one wave per column.  grid (cols/4, batch), 256 threads.
 __global__ __launch_bounds__(256) void jac_colnorms(
-    const double2* __restrict__ X, int64_t rows, int64_t cols, int64_t sX,
-    double* __restrict__ sig) {
+    const double2* __restrict__ X, int64_t rows, int64_t ldx, int64_t cols,
+    int64_t sX, double* __restrict__ sig) {
   const int bz = blockIdx.y;
   const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (c >= cols) return;
-  const double2* x = X + (int64_t)bz * sX + c * rows;
+  const double2* x = X + (int64_t)bz * sX + c * ldx;
   double s = 0.0;
   for (int64_t r = lane; r < rows; r += 64) {
     const double2 v = x[r];
@@ -589,7 +614,7 @@ int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
   Q = (double2*)ptr;
   // small per-problem arrays packed in one slot
   const size_t small =
-      (size_t)batch * (p * sizeof(int) + 5 * sizeof(int) + 5 * sizeof(double) +
+      (size_t)batch * (p * sizeof(int) + 5 * sizeof(int) + 6 * sizeof(double) +
                        2 * sizeof(int64_t)) +
       1024;
   QTN_TRY(ctx->buffer(kSlotSmall, small, &ptr));
@@ -603,6 +628,8 @@ int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
   total = (double*)c;
   c += batch * sizeof(double);
   keptw = (double*)c;
+  c += batch * sizeof(double);
+  offmax = (double*)c;
   c += batch * sizeof(double);
   kept = (int64_t*)c;
   c += batch * sizeof(int64_t);
@@ -628,17 +655,17 @@ int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
 }
 
 template <int WW>
-int32_t JacobiEngine::launch_round(hipStream_t s, double2* X, int64_t sX,
-                                   double2* Wm, int64_t sW, int r, double tol,
-                                   int n_inner, int nchunk_x, int nchunk_w,
-                                   bool per_problem_w) {
+int32_t JacobiEngine::launch_round(hipStream_t s, double2* X, int64_t ldx,
+                                   int64_t sX, double2* Wm, int64_t sW, int r,
+                                   double tol, int n_inner, int nchunk_x,
+                                   int nchunk_w, bool per_problem_w) {
   hipLaunchKernelGGL(jac_gram<WW>, dim3(nchunk_x, p, batch), dim3(256), 0, s,
-                     (const double2*)X, rows_p, sX, p, r, Gp, active);
+                     (const double2*)X, ldx, sX, p, r, Gp, active);
   hipLaunchKernelGGL(jac_inner<WW>, dim3(p, batch), dim3(256), 0, s,
                      (const double2*)Gp, nchunk_x, p, tol, zthr, n_inner, Q,
-                     flags, counters, active);
+                     flags, counters, active, offmax);
   hipLaunchKernelGGL(jac_update<WW>, dim3(nchunk_x, p, batch), dim3(256), 0, s,
-                     X, rows_p, sX, p, r, (const double2*)Q, flags, active,
+                     X, ldx, sX, p, r, (const double2*)Q, flags, active,
                      (const int*)nullptr);
   if (Wm)
     hipLaunchKernelGGL(jac_update<WW>, dim3(nchunk_w, p, batch), dim3(256), 0,
@@ -647,8 +674,8 @@ int32_t JacobiEngine::launch_round(hipStream_t s, double2* X, int64_t sX,
   return QTN_OK;
 }
 
-int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t sX, double2* Wm,
-                          int64_t sW, const int* want_w_host,
+int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t ldx, int64_t sX,
+                          double2* Wm, int64_t sW, const int* want_w_host,
                           const int64_t* nsv_host, double cutoff,
                           int relative, int64_t chimax, int max_sweeps,
                           int* sweeps_done) {
@@ -661,6 +688,8 @@ int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t sX, double2* Wm,
   int n_inner_big = 1;
   if (const char* e = getenv("QTN_JACOBI_INNER")) n_inner_big = std::max(1, atoi(e));
   const int n_inner = (p == 1) ? 40 : n_inner_big;
+  // stop once a sweep's largest scaled off-diagonal is below sqrt(tol)
+  const double conv2 = tol;
   const int nchunk_x = (int)(rows_p / 64);
   const int nchunk_w = (int)(cols_p / 64);
 
@@ -670,9 +699,10 @@ int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t sX, double2* Wm,
     QTN_CHECK_HIP(hipMemcpyAsync(want_w, want_w_host, batch * sizeof(int),
                                  hipMemcpyHostToDevice, s));
   QTN_CHECK_HIP(hipMemsetAsync(misc, 0, 2 * sizeof(int), s));
-  hipLaunchKernelGGL(jac_prepare, dim3(batch), dim3(256), 0, s, X,
-                     rows_p * cols_p, sX, (double)rows_p * DBL_EPSILON,
-                     normF2, zthr, active, counters, misc + 1);
+  QTN_CHECK_HIP(hipMemsetAsync(offmax, 0, batch * sizeof(double), s));
+  hipLaunchKernelGGL(jac_prepare, dim3(batch), dim3(256), 0, s, X, rows_p,
+                     ldx, cols_p, sX, (double)rows_p * DBL_EPSILON, normF2,
+                     zthr, active, counters, misc + 1);
   QTN_CHECK_HIP(hipGetLastError());
   if (Wm) {
     // W starts as the identity on the padded columns.
@@ -685,15 +715,15 @@ int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t sX, double2* Wm,
   for (; sweep < max_sweeps; ++sweep) {
     for (int r = 0; r < rounds; ++r) {
       if (this->W == 64)
-        QTN_TRY(launch_round<64>(s, X, sX, Wm, sW, r, tol, n_inner, nchunk_x,
-                                 nchunk_w, want_w_host != nullptr));
+        QTN_TRY(launch_round<64>(s, X, ldx, sX, Wm, sW, r, tol, n_inner,
+                                 nchunk_x, nchunk_w, want_w_host != nullptr));
       else
-        QTN_TRY(launch_round<32>(s, X, sX, Wm, sW, r, tol, n_inner, nchunk_x,
-                                 nchunk_w, want_w_host != nullptr));
+        QTN_TRY(launch_round<32>(s, X, ldx, sX, Wm, sW, r, tol, n_inner,
+                                 nchunk_x, nchunk_w, want_w_host != nullptr));
     }
     QTN_CHECK_HIP(hipGetLastError());
     hipLaunchKernelGGL(jac_sweep_end, dim3(1), dim3(256), 0, s, active,
-                       counters, batch, misc);
+                       counters, offmax, conv2, batch, misc);
     QTN_CHECK_HIP(hipMemcpyAsync(host_flags, misc, 2 * sizeof(int),
                                  hipMemcpyDeviceToHost, s));
     QTN_CHECK_HIP(hipStreamSynchronize(s));
@@ -718,8 +748,8 @@ int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t sX, double2* Wm,
     return QTN_ERR_NOCONV;
   }
   hipLaunchKernelGGL(jac_colnorms, dim3((unsigned)((cols_p + 3) / 4), batch),
-                     dim3(256), 0, s, (const double2*)X, rows_p, cols_p, sX,
-                     sig);
+                     dim3(256), 0, s, (const double2*)X, rows_p, ldx, cols_p,
+                     sX, sig);
   if (cols_p > kSortMax) {
     set_error("svd: more than 4096 columns are not supported");
     return QTN_ERR_ARG;

@@ -28,6 +28,7 @@ struct JacobiEngine {
   double* disc = nullptr;   // discarded weight per problem
   double* total = nullptr;  // sum of all sigma^2 per problem
   double* keptw = nullptr;  // sum of the kept sigma^2 per problem
+  double* offmax = nullptr; // largest scaled off-diagonal of the sweep
   int64_t* kept = nullptr;
   int64_t* nsv = nullptr;
   int* flags = nullptr;
@@ -48,10 +49,13 @@ struct JacobiEngine {
   // total / status the truncation decision for nsv_host[b] meaningful
   // values.
   template <int WW>
-  int32_t launch_round(hipStream_t s, double2* X, int64_t sX, double2* Wm,
-                       int64_t sW, int r, double tol, int n_inner,
-                       int nchunk_x, int nchunk_w, bool per_problem_w);
-  int32_t run(qtn_ctx* ctx, double2* X, int64_t sX, double2* Wm, int64_t sW,
+  int32_t launch_round(hipStream_t s, double2* X, int64_t ldx, int64_t sX,
+                       double2* Wm, int64_t sW, int r, double tol,
+                       int n_inner, int nchunk_x, int nchunk_w,
+                       bool per_problem_w);
+  // X: rows_p rows are processed, leading dimension ldx >= rows_p.
+  int32_t run(qtn_ctx* ctx, double2* X, int64_t ldx, int64_t sX, double2* Wm,
+              int64_t sW,
               const int* want_w_host, const int64_t* nsv_host, double cutoff,
               int relative, int64_t chimax, int max_sweeps, int* sweeps_done);
 };

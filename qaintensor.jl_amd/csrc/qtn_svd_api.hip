@@ -120,7 +120,7 @@ int32_t svd_common(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
   QTN_TRY(eng.reserve(ctx, 1, rows_p, cols_p));
   const int64_t nsv = std::min(m, n);
   int sweeps = 0;
-  QTN_TRY(eng.run(ctx, (double2*)dX, rows_p * cols_p, (double2*)dW,
+  QTN_TRY(eng.run(ctx, (double2*)dX, rows_p, rows_p * cols_p, (double2*)dW,
                   cols_p * cols_p, nullptr, &nsv, cutoff, relative, chimax, 60,
                   &sweeps));
   // read back the decision
