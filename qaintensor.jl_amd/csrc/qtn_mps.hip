@@ -280,7 +280,9 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
   }
   const int64_t rows_p = round_up64(maxXr), cols_p = round_up64(maxXc);
   const int64_t phi_sz = maxM * maxN, x_sz = rows_p * cols_p;
-  bool precond = true;
+  // QR preconditioning halves the Jacobi sweeps but its panel chain costs
+  // about what it saves at chi = 256; off unless QTN_JACOBI_PRECOND=1.
+  bool precond = false;
   if (const char* e = getenv("QTN_JACOBI_PRECOND")) precond = atoi(e) != 0;
   const int64_t w_sz = cols_p * cols_p;
   void *dphi, *dX, *dVk, *ddesc, *dg, *dW = nullptr;

@@ -288,6 +288,229 @@ __global__ __launch_bounds__(256) void qr_panel(
   }
 }
 
+// Wave-shuffle then LDS sum of N complex partials per thread; every thread
+// gets the totals in out[].  scratch: 4 waves x N double2 in LDS.
+template <int N>
+__device__ __forceinline__ void block_sum_fast(double2 (&part)[N],
+                                               double2 (&out)[N],
+                                               double2* scratch) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // one butterfly step for all N values at a time, so the 2N shuffles of a
+  // step are independent and their latencies overlap
+  double x[N], y[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    x[i] = part[i].x;
+    y[i] = part[i].y;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double sx[N], sy[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      sx[i] = __shfl_xor(x[i], off, 64);
+      sy[i] = __shfl_xor(y[i], off, 64);
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      x[i] += sx[i];
+      y[i] += sy[i];
+    }
+  }
+  if (lane < N) {
+    double vx = 0.0, vy = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+      if (i == lane) {
+        vx = x[i];
+        vy = y[i];
+      }
+    scratch[wave * N + lane] = make_double2(vx, vy);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double2 t = scratch[i];
+    for (int w = 1; w < 4; ++w) {
+      t.x += scratch[w * N + i].x;
+      t.y += scratch[w * N + i].y;
+    }
+    out[i] = t;
+  }
+  __syncthreads();
+}
+
+// qr_panel with the panel (rows - r0 <= kLdsRows rows x NB) held in LDS.
+// Per column one norm reduction and one 16-wide reduction that yields both
+// w_c = v_j^H x_c (c > j, for the reflector application) and
+// Gv[i][j] = v_i^H v_j (i < j, for T).
+constexpr int kLdsRows = 512;
+__global__ __launch_bounds__(256) void qr_panel_lds(
+    double2* __restrict__ A, int64_t lda, int64_t sA, int k,
+    const QrDims* __restrict__ dims, double2* __restrict__ Vb,
+    double2* __restrict__ VTh, double2* __restrict__ VTn, int64_t sV,
+    int64_t sVh) {
+  const int bz = blockIdx.x;
+  const QrDims d = dims[bz];
+  const int64_t r0 = (int64_t)k * NB;
+  if (!d.active || r0 >= d.cols) return;
+  const int rows = (int)(d.rows - r0);  // rows of the panel, <= kLdsRows
+  const int ncp = (int)min((int64_t)NB, d.cols - r0);
+  A += (int64_t)bz * sA + r0 + r0 * lda;  // panel origin
+  Vb += (int64_t)bz * sV;
+  VTh += (int64_t)bz * sVh;
+  VTn += (int64_t)bz * sV;
+  const int tid = threadIdx.x;
+  __shared__ double2 P[NB][kLdsRows + 1];  // P[col][row], panel-local
+  __shared__ double2 scratch[4 * NB];
+  __shared__ double2 tau_s[NB];
+  __shared__ double2 T[NB][NB + 1];
+  __shared__ double2 Gv[NB][NB + 1];
+  for (int idx = tid; idx < NB * rows; idx += 256) {
+    const int r = idx % rows, c = idx / rows;
+    P[c][r] = c < ncp ? A[r + (int64_t)c * lda] : make_double2(0.0, 0.0);
+  }
+  if (tid < NB)
+    for (int c = 0; c < NB; ++c) {
+      Gv[tid][c] = make_double2(0.0, 0.0);
+      T[tid][c] = make_double2(0.0, 0.0);
+    }
+  __syncthreads();
+  for (int j = 0; j < ncp; ++j) {
+    double2 part[1] = {make_double2(0.0, 0.0)};
+    for (int r = j + 1 + tid; r < rows; r += 256) {
+      const double2 v = P[j][r];
+      part[0].x += v.x * v.x + v.y * v.y;
+    }
+    double2 tot[1];
+    block_sum_fast<1>(part, tot, scratch);
+    const double xn2 = tot[0].x;
+    const double2 alpha = P[j][j];
+    double2 tau, scl;
+    double beta;
+    if (xn2 == 0.0 && alpha.y == 0.0) {
+      tau = make_double2(0.0, 0.0);
+      scl = make_double2(0.0, 0.0);
+      beta = alpha.x;
+    } else {
+      const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+      beta = alpha.x >= 0.0 ? -nrm : nrm;
+      tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+      const double2 den = make_double2(alpha.x - beta, alpha.y);
+      const double dd = den.x * den.x + den.y * den.y;
+      scl = make_double2(den.x / dd, -den.y / dd);
+    }
+    if (tid == 0) tau_s[j] = tau;
+    for (int r = j + 1 + tid; r < rows; r += 256) P[j][r] = cmul(P[j][r], scl);
+    __syncthreads();
+    if (tid == 0) P[j][j] = make_double2(beta, 0.0);
+    // c > j: w_c = v_j^H x_c ; c < j: g_c = v_c^H v_j (v_c lives in P[c]
+    // below its diagonal, and v_j is zero above row j)
+    double2 wp[NB], w[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) wp[c] = make_double2(0.0, 0.0);
+    for (int r = j + tid; r < rows; r += 256) {
+      const double2 v = (r == j) ? make_double2(1.0, 0.0) : P[j][r];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        if (c > j && c < ncp) {
+          const double2 pr = cmulc(v, P[c][r]);
+          wp[c].x += pr.x;
+          wp[c].y += pr.y;
+        } else if (c < j) {
+          const double2 pr = cmulc(P[c][r], v);
+          wp[c].x += pr.x;
+          wp[c].y += pr.y;
+        }
+      }
+    }
+    block_sum_fast<NB>(wp, w, scratch);
+    if (tid < j) {
+      double2 g = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (c == tid) g = w[c];
+      Gv[tid][j] = g;
+    }
+    if (tau.x != 0.0 || tau.y != 0.0) {
+      for (int r = j + tid; r < rows; r += 256) {
+        const double2 v = (r == j) ? make_double2(1.0, 0.0) : P[j][r];
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          if (c > j && c < ncp) {
+            const double2 u = cmul(v, cmulc(tau, w[c]));
+            P[c][r].x -= u.x;
+            P[c][r].y -= u.y;
+          }
+      }
+    }
+    __syncthreads();
+  }
+  // write R / reflectors back
+  for (int idx = tid; idx < NB * rows; idx += 256) {
+    const int r = idx % rows, c = idx / rows;
+    if (c < ncp) A[r + (int64_t)c * lda] = P[c][r];
+  }
+  // T (zlarft): column j = -tau_j T[0:j,0:j] Gv[0:j, j], T[j][j] = tau_j.
+  // Row tid of column j only needs T[tid][l] for tid <= l < j.  Barriers
+  // are unconditional: every thread takes both of them each step.
+  for (int j = 0; j < ncp; ++j) {
+    double2 q = make_double2(0.0, 0.0);
+    const double2 tj = tau_s[j];
+    if (tid < j) {
+      double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int l = 0; l < NB; ++l)
+        if (l >= tid && l < j) {
+          const double2 pr = cmul(T[tid][l], Gv[l][j]);
+          acc.x += pr.x;
+          acc.y += pr.y;
+        }
+      q = cmul(tj, acc);
+    }
+    __syncthreads();
+    if (tid < j) T[tid][j] = make_double2(-q.x, -q.y);
+    if (tid == j) T[j][j] = tj;
+    __syncthreads();
+  }
+  // explicit V, V T^H and V T for every row of the problem (zero above r0).
+  // Thread (j = tid % NB, part = tid / NB) owns output column j and keeps
+  // column j of T and row j of T in registers.
+  {
+    const int j = tid % NB, part = tid / NB;
+    double2 tcol[NB], trow[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      tcol[i] = T[i][j];
+      const double2 t = T[j][i];
+      trow[i] = make_double2(t.x, -t.y);  // conj(T[j][i])
+    }
+    for (int64_t r = part; r < d.rows; r += 256 / NB) {
+      const int64_t rl = r - r0;
+      double2 ah = make_double2(0.0, 0.0), an = make_double2(0.0, 0.0);
+      double2 vj = make_double2(0.0, 0.0);
+      if (rl >= 0) {
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+          double2 v = make_double2(0.0, 0.0);
+          if (i < ncp && rl >= i)
+            v = (rl == i) ? make_double2(1.0, 0.0) : P[i][rl];
+          if (i == j) vj = v;
+          const double2 h = cmul(v, trow[i]);
+          const double2 n = cmul(v, tcol[i]);
+          ah.x += h.x;
+          ah.y += h.y;
+          an.x += n.x;
+          an.y += n.y;
+        }
+      }
+      Vb[r + j * lda] = vj;
+      VTh[r + j * lda] = ah;
+      VTn[r + j * lda] = an;
+    }
+  }
+}
+
 // Y = R^H (lower triangular n x n, ld ldy) from the upper triangle of the
 // factored A (ld lda); n = cols per problem; rest zero.
 __global__ void qr_r_adjoint(const double2* __restrict__ A, int64_t lda,
@@ -405,8 +628,13 @@ int32_t BatchedQR::factor(qtn_ctx* ctx, double2* A, int64_t lda, int64_t sA,
   for (int k = 0; k < npanel; ++k) {
     double2* Vk = V + (keep_q ? (int64_t)k * vs : 0);
     double2* VTnk = VTn + (keep_q ? (int64_t)k * vs : 0);
-    hipLaunchKernelGGL(qr_panel, dim3(batch), dim3(256), 0, s, A, lda, sA, k,
-                       (const QrDims*)ddims, Vk, VTh, VTnk, vstride, vs);
+    if (maxr - (int64_t)k * NB <= kLdsRows)
+      hipLaunchKernelGGL(qr_panel_lds, dim3(batch), dim3(256), 0, s, A, lda,
+                         sA, k, (const QrDims*)ddims, Vk, VTh, VTnk, vstride,
+                         vs);
+    else
+      hipLaunchKernelGGL(qr_panel, dim3(batch), dim3(256), 0, s, A, lda, sA,
+                         k, (const QrDims*)ddims, Vk, VTh, VTnk, vstride, vs);
     QTN_CHECK_HIP(hipGetLastError());
     if (maxn[k] == 0) continue;
     GemmDesc<double>* dk = dall + (size_t)k * 2 * batch;
