@@ -37,6 +37,30 @@ def gate_flops(chiL, chiM, chiR):
             4.0 * (14.0 * m * n * n + 8.0 * n ** 3))
 
 
+HBM_PEAK_GBS = 8000.0
+
+
+def roofline(prof):
+    """Roofline of the SVD kernel with the largest summed time in the timed
+    region (HIP events on the library stream): jac_inner is bounded by its
+    HBM traffic (partial Grams in, rotation block out); jac_gram and
+    jac_update are FP64 MFMA GEMMs."""
+    name = max(prof, key=lambda k: prof[k]["ms"])
+    p = prof[name]
+    avg_s = p["ms"] / max(1, p["count"]) / 1e3
+    if name == "jac_inner":
+        ach = p["bytes"] / max(1, p["count"]) / avg_s / 1e9
+        return {"kernel": name, "bound": "hbm", "achieved": ach,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                "avg_launch_us": avg_s * 1e6, "launches": p["count"]}
+    ach = p["flops"] / max(1, p["count"]) / avg_s / 1e12
+    return {"kernel": name, "bound": "mfma", "achieved": ach,
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
+            "avg_launch_us": avg_s * 1e6, "launches": p["count"]}
+
+
 def infidelity(a, b):
     """1 - |<a|b>|^2 / (<a|a><b|b>) (the truncated states are not exactly
     normalised)."""
@@ -73,6 +97,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="time the batched form only (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU "
@@ -92,24 +118,30 @@ def main():
 
     # plain form: one two-site update at a time (also the reference state)
     rec_plain = []
-    run_circuit(ref, pristine, layers, batched=False, record=rec_plain)
+    run_circuit(ref, pristine, layers, batched=args.only_batched,
+                record=rec_plain)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.plain_steps):
-        run_circuit(dev, pristine, layers, batched=False)
-    torch.cuda.synchronize()
-    plain_s = (time.perf_counter() - t0) / args.plain_steps
+    plain_s = float("nan")
+    if not args.only_batched:
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            run_circuit(dev, pristine, layers, batched=False)
+        torch.cuda.synchronize()
+        plain_s = (time.perf_counter() - t0) / args.plain_steps
 
     # batched form
     for _ in range(args.warmup):
         run_circuit(dev, pristine, layers, batched=True)
     torch.cuda.synchronize()
     ctx.counters(reset=True)
+    ctx.profile(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         run_circuit(dev, pristine, layers, batched=True)
     torch.cuda.synchronize()
     batched_s = (time.perf_counter() - t0) / args.steps
+    prof = ctx.profile_read()
+    ctx.profile(False)
     cnt = ctx.counters()
     inf_batched = infidelity(ref, dev)
 
@@ -124,7 +156,7 @@ def main():
             dims_flops += gate_flops(st_dims[w - 1], st_dims[w],
                                      st_dims[w + 1])
             st_dims[w] = k
-    plain_rate = n2q / plain_s
+    plain_rate = None if args.only_batched else n2q / plain_s
     batched_rate = n2q / batched_s
     achieved = dims_flops / batched_s / 1e12
 
@@ -147,19 +179,25 @@ def main():
                    "form": "batched-fp64", "n_qubits": N_QUBITS,
                    "depth": DEPTH, "chi_max": CHIMAX, "cutoff": CUTOFF},
         "forms": {
-            "plain": {"value": plain_rate, "ms_per_circuit": plain_s * 1e3,
-                      "infidelity_vs_fp64": 0.0},
+            "plain": (None if plain_rate is None else
+                      {"value": plain_rate, "ms_per_circuit": plain_s * 1e3,
+                       "infidelity_vs_fp64": 0.0}),
             "batched_fp64": {"value": batched_rate,
                              "ms_per_circuit": batched_s * 1e3,
                              "infidelity_vs_fp64": inf_batched,
                              "jacobi_sweeps_per_svd":
                                  cnt["sweeps"] / max(1, cnt["svd"])},
         },
-        "roofline": {"bound": "mfma", "achieved": achieved,
-                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "scope": "whole two-site update pipeline, algorithmic "
-                              "flops per circuit %.3g" % dims_flops},
+        "roofline": roofline(prof),
+        "pipeline_roofline": {
+            "bound": "mfma", "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "note": "whole two-site update, Golub-Reinsch flop convention "
+                    "%.3g flop per circuit" % dims_flops},
+        "kernels": {k: {"launches": v["count"],
+                        "avg_us": 1e3 * v["ms"] / max(1, v["count"])}
+                    for k, v in prof.items()},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(layers)

@@ -67,6 +67,15 @@ int32_t qtn_ctx_stream(qtn_ctx* ctx, void** stream);
  * n = number of slots to write (<= 4).  reset != 0 zeroes them after.     */
 int32_t qtn_ctx_counters(qtn_ctx* ctx, int64_t* out, int32_t n,
                          int32_t reset);
+/* Per-kernel timing with HIP events on the context stream (off by default).
+ * Kernel ids: 0 jac_inner, 1 jac_gram, 2 jac_update (the block Jacobi SVD
+ * kernels).  qtn_ctx_profile(ctx, 1) enables and clears the accumulators.
+ * qtn_ctx_profile_read returns, for one kernel id, the launch count, the
+ * summed launch durations (ms), and the algorithmic HBM bytes and flops of
+ * those launches (counted for the problems active in each launch).        */
+int32_t qtn_ctx_profile(qtn_ctx* ctx, int32_t enable);
+int32_t qtn_ctx_profile_read(qtn_ctx* ctx, int32_t kernel, int64_t* count,
+                             double* ms, double* bytes, double* flops);
 /* Message of the last failing call on this thread ("" if none).            */
 const char* qtn_last_error(void);
 /* Library version string. */

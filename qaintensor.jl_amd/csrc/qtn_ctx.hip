@@ -99,6 +99,7 @@ int32_t qtn_ctx_destroy(qtn_ctx* ctx) {
   for (auto& b : ctx->bufs)
     if (b.ptr) QTN_CHECK_HIP(hipFree(b.ptr));
   if (ctx->pinned) QTN_CHECK_HIP(hipHostFree(ctx->pinned));
+  for (auto e : ctx->ev_pool) QTN_CHECK_HIP(hipEventDestroy(e));
   QTN_CHECK_HIP(hipStreamDestroy(ctx->stream));
   delete ctx;
   return QTN_OK;
@@ -121,6 +122,31 @@ int32_t qtn_ctx_counters(qtn_ctx* ctx, int64_t* out, int32_t n,
                         ctx->n_batches};
   for (int i = 0; i < n; ++i) out[i] = v[i];
   if (reset) ctx->n_svd = ctx->n_sweeps = ctx->max_sweeps = ctx->n_batches = 0;
+  return QTN_OK;
+}
+
+int32_t qtn_ctx_profile(qtn_ctx* ctx, int32_t enable) {
+  qtn::clear_error();
+  QTN_ARG(ctx != nullptr, "qtn_ctx_profile: ctx is NULL");
+  QTN_CHECK_HIP(hipSetDevice(ctx->device));
+  ctx->prof = enable != 0;
+  for (int k = 0; k < qtn_ctx::kProfKernels; ++k) {
+    ctx->prof_count[k] = 0;
+    ctx->prof_ms[k] = ctx->prof_bytes[k] = ctx->prof_flops[k] = 0.0;
+  }
+  return QTN_OK;
+}
+
+int32_t qtn_ctx_profile_read(qtn_ctx* ctx, int32_t kernel, int64_t* count,
+                             double* ms, double* bytes, double* flops) {
+  qtn::clear_error();
+  QTN_ARG(ctx != nullptr, "qtn_ctx_profile_read: ctx is NULL");
+  QTN_ARG(kernel >= 0 && kernel < qtn_ctx::kProfKernels,
+          "qtn_ctx_profile_read: unknown kernel id");
+  if (count) *count = ctx->prof_count[kernel];
+  if (ms) *ms = ctx->prof_ms[kernel];
+  if (bytes) *bytes = ctx->prof_bytes[kernel];
+  if (flops) *flops = ctx->prof_flops[kernel];
   return QTN_OK;
 }
 

@@ -150,7 +150,8 @@ __global__ __launch_bounds__(256) void jac_inner(
     const double2* __restrict__ Gp, int nchunk, int p, double tol,
     const double* __restrict__ zthr, int n_inner, double2* __restrict__ Qout,
     int* __restrict__ flags, int* __restrict__ counters,
-    const int* __restrict__ active, double* __restrict__ offmax) {
+    const int* __restrict__ active, double* __restrict__ offmax,
+    int* __restrict__ qpairs) {
   constexpr int NP = W / 2;       // rotation pairs per round
   constexpr int TP = 256 / NP;    // threads per pair
   constexpr int RPT = W / TP;     // rows (columns) per thread
@@ -319,7 +320,10 @@ __global__ __launch_bounds__(256) void jac_inner(
   const int total = any_off - 1;
   if (tid == 0) {
     flags[bz * p + pi] = total > 0 ? 1 : 0;
-    if (total > 0) atomicAdd(&counters[bz], total);
+    if (total > 0) {
+      atomicAdd(&counters[bz], total);
+      atomicAdd(qpairs, 1);
+    }
   }
   if (total > 0) {
     double2* q = Qout + ((int64_t)bz * p + pi) * (int64_t)(W * W);
@@ -659,11 +663,27 @@ int32_t JacobiEngine::launch_round(hipStream_t s, double2* X, int64_t ldx,
                                    int64_t sX, double2* Wm, int64_t sW, int r,
                                    double tol, int n_inner, int nchunk_x,
                                    int nchunk_w, bool per_problem_w) {
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  if (ctx_ && ctx_->prof) {
+    for (int i = 0; i < 4; ++i) {
+      if (ctx_->ev_pool.empty()) {
+        hipEvent_t e;
+        QTN_CHECK_HIP(hipEventCreate(&e));
+        ctx_->ev_pool.push_back(e);
+      }
+      ev[i] = ctx_->ev_pool.back();
+      ctx_->ev_pool.pop_back();
+      prof_events.push_back(ev[i]);
+    }
+  }
+  if (ev[0]) QTN_CHECK_HIP(hipEventRecord(ev[0], s));
   hipLaunchKernelGGL(jac_gram<WW>, dim3(nchunk_x, p, batch), dim3(256), 0, s,
                      (const double2*)X, ldx, sX, p, r, Gp, active);
+  if (ev[1]) QTN_CHECK_HIP(hipEventRecord(ev[1], s));
   hipLaunchKernelGGL(jac_inner<WW>, dim3(p, batch), dim3(256), 0, s,
                      (const double2*)Gp, nchunk_x, p, tol, zthr, n_inner, Q,
-                     flags, counters, active, offmax);
+                     flags, counters, active, offmax, misc + 2);
+  if (ev[2]) QTN_CHECK_HIP(hipEventRecord(ev[2], s));
   hipLaunchKernelGGL(jac_update<WW>, dim3(nchunk_x, p, batch), dim3(256), 0, s,
                      X, ldx, sX, p, r, (const double2*)Q, flags, active,
                      (const int*)nullptr);
@@ -671,6 +691,48 @@ int32_t JacobiEngine::launch_round(hipStream_t s, double2* X, int64_t ldx,
     hipLaunchKernelGGL(jac_update<WW>, dim3(nchunk_w, p, batch), dim3(256), 0,
                        s, Wm, cols_p, sW, p, r, (const double2*)Q, flags,
                        active, (const int*)(per_problem_w ? want_w : nullptr));
+  if (ev[3]) QTN_CHECK_HIP(hipEventRecord(ev[3], s));
+  return QTN_OK;
+}
+
+// Adds the event times of the sweep just finished to the context's
+// per-kernel accumulators with the algorithmic work of its launches:
+//   jac_gram   flops 3/4 * 8*64*W^2 per (problem, pair, chunk); bytes: the
+//              64 x W X tile read + the 3/4 W^2 partial Gram written
+//   jac_inner  bytes: the partial Grams read + Q written per (problem, pair)
+//   jac_update flops 8*64*W^2 per (pair with a rotation, chunk); bytes: the
+//              tile read and written + Q read
+int32_t JacobiEngine::account_sweep(int n_active, int qpairs, int nchunk) {
+  qtn_ctx* ctx = ctx_;
+  const double Wd = W, c16 = 16.0;
+  const double rounds = 2.0 * p - 1.0;
+  const double gram_f = 0.75 * 8.0 * 64.0 * Wd * Wd;
+  const double gram_b = (64.0 * Wd + 0.75 * Wd * Wd) * c16;
+  const double inner_b = (nchunk * 0.75 * Wd * Wd + Wd * Wd) * c16;
+  const double upd_f = 8.0 * 64.0 * Wd * Wd;
+  const double upd_b = (2.0 * 64.0 * Wd + Wd * Wd) * c16;
+  const double pairs_per_launch = (double)n_active * p;
+  for (size_t i = 0; i + 3 < prof_events.size(); i += 4) {
+    float t01, t12, t23;
+    QTN_CHECK_HIP(hipEventElapsedTime(&t01, prof_events[i], prof_events[i + 1]));
+    QTN_CHECK_HIP(hipEventElapsedTime(&t12, prof_events[i + 1], prof_events[i + 2]));
+    QTN_CHECK_HIP(hipEventElapsedTime(&t23, prof_events[i + 2], prof_events[i + 3]));
+    ctx->prof_count[1] += 1;
+    ctx->prof_ms[1] += t01;
+    ctx->prof_flops[1] += pairs_per_launch * nchunk * gram_f;
+    ctx->prof_bytes[1] += pairs_per_launch * nchunk * gram_b;
+    ctx->prof_count[0] += 1;
+    ctx->prof_ms[0] += t12;
+    ctx->prof_bytes[0] += pairs_per_launch * inner_b;
+    ctx->prof_count[2] += 1;
+    ctx->prof_ms[2] += t23;
+  }
+  // rotated pairs are only known per sweep: spread over its launches
+  ctx->prof_flops[2] += (double)qpairs * nchunk * upd_f;
+  ctx->prof_bytes[2] += (double)qpairs * nchunk * upd_b;
+  (void)rounds;
+  for (auto e : prof_events) ctx->ev_pool.push_back(e);
+  prof_events.clear();
   return QTN_OK;
 }
 
@@ -698,7 +760,10 @@ int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t ldx, int64_t sX,
   if (want_w_host)
     QTN_CHECK_HIP(hipMemcpyAsync(want_w, want_w_host, batch * sizeof(int),
                                  hipMemcpyHostToDevice, s));
-  QTN_CHECK_HIP(hipMemsetAsync(misc, 0, 2 * sizeof(int), s));
+  QTN_CHECK_HIP(hipMemsetAsync(misc, 0, 3 * sizeof(int), s));
+  ctx_ = ctx;
+  prof_events.clear();
+  int n_act = batch;
   QTN_CHECK_HIP(hipMemsetAsync(offmax, 0, batch * sizeof(double), s));
   hipLaunchKernelGGL(jac_prepare, dim3(batch), dim3(256), 0, s, X, rows_p,
                      ldx, cols_p, sX, (double)rows_p * DBL_EPSILON, normF2,
@@ -724,9 +789,12 @@ int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t ldx, int64_t sX,
     QTN_CHECK_HIP(hipGetLastError());
     hipLaunchKernelGGL(jac_sweep_end, dim3(1), dim3(256), 0, s, active,
                        counters, offmax, conv2, batch, misc);
-    QTN_CHECK_HIP(hipMemcpyAsync(host_flags, misc, 2 * sizeof(int),
+    QTN_CHECK_HIP(hipMemcpyAsync(host_flags, misc, 3 * sizeof(int),
                                  hipMemcpyDeviceToHost, s));
+    QTN_CHECK_HIP(hipMemsetAsync(misc + 2, 0, sizeof(int), s));
     QTN_CHECK_HIP(hipStreamSynchronize(s));
+    if (ctx->prof) QTN_TRY(account_sweep(n_act, host_flags[2], nchunk_x));
+    n_act = host_flags[0];
     if (host_flags[1]) {
       set_error("svd: input contains NaN or Inf");
       return QTN_ERR_NOTFINITE;

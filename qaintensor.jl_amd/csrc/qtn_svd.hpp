@@ -3,6 +3,8 @@
 // Gram partials, rotation blocks and the per-problem result arrays, all in
 // context scratch slots.
 #pragma once
+#include <vector>
+
 #include "qtn_internal.hpp"
 
 namespace qtn {
@@ -38,6 +40,9 @@ struct JacobiEngine {
   int* want_w = nullptr;
   int* misc = nullptr;
   double* sig = nullptr;  // [batch][cols_p], sorted descending after run
+  qtn_ctx* ctx_ = nullptr;
+  std::vector<hipEvent_t> prof_events;  // 4 per round while profiling
+  int32_t account_sweep(int n_active, int qpairs, int nchunk);
   int* perm = nullptr;    // [batch][cols_p], column of X for each sigma
 
   int32_t reserve(qtn_ctx* ctx, int batch, int64_t rows_p, int64_t cols_p);

@@ -27,7 +27,7 @@ QTN_FORM_CHOSEN = 1
 # Every symbol include/qtn.h declares (checked by the CPU test suite).
 EXPORTS = [
     "qtn_ctx_create", "qtn_ctx_destroy", "qtn_ctx_sync", "qtn_ctx_stream",
-    "qtn_ctx_counters",
+    "qtn_ctx_counters", "qtn_ctx_profile", "qtn_ctx_profile_read",
     "qtn_last_error", "qtn_version",
     "qtn_contract_pair", "qtn_contract_pair_dev", "qtn_permute",
     "qtn_permute_dev", "qtn_zgemm_dev", "qtn_cgemm_dev",
@@ -73,6 +73,8 @@ def _declare(lib):
         "qtn_ctx_sync": (_i32, [_vp]),
         "qtn_ctx_stream": (_i32, [_vp, ctypes.POINTER(_vp)]),
         "qtn_ctx_counters": (_i32, [_vp, _pi64, _i32, _i32]),
+        "qtn_ctx_profile": (_i32, [_vp, _i32]),
+        "qtn_ctx_profile_read": (_i32, [_vp, _i32, _pi64, _pd, _pd, _pd]),
         "qtn_last_error": (ctypes.c_char_p, []),
         "qtn_version": (ctypes.c_char_p, []),
         "qtn_contract_pair": (_i32, [_vp, _pd, _i32, _pi64, _pd, _i32, _pi64,
@@ -196,6 +198,26 @@ class Context:
                                       int(bool(reset))))
         return dict(svd=int(out[0]), sweeps=int(out[1]),
                     max_sweeps=int(out[2]), batches=int(out[3]))
+
+    PROFILED_KERNELS = ("jac_inner", "jac_gram", "jac_update")
+
+    def profile(self, enable=True):
+        """Enable (and clear) or disable per-kernel HIP-event timing."""
+        check(load().qtn_ctx_profile(self.handle, int(bool(enable))))
+
+    def profile_read(self):
+        """{kernel: dict(count, ms, bytes, flops)} of the profiled kernels."""
+        out = {}
+        for kid, name in enumerate(self.PROFILED_KERNELS):
+            c = ctypes.c_int64(0)
+            ms, by, fl = ctypes.c_double(0), ctypes.c_double(0), \
+                ctypes.c_double(0)
+            check(load().qtn_ctx_profile_read(
+                self.handle, kid, ctypes.byref(c), ctypes.byref(ms),
+                ctypes.byref(by), ctypes.byref(fl)))
+            out[name] = dict(count=c.value, ms=ms.value, bytes=by.value,
+                             flops=fl.value)
+        return out
 
     def close(self):
         if self._h is not None:

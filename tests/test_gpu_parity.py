@@ -255,3 +255,24 @@ def test_config2_full_circuit(gpu_ctx):
     e.run_layers(layers, cutoff, chimax, batched=False)
     f = abs(d.overlap(e)) ** 2 / (d.norm2() * e.norm2())
     assert abs(f - 1.0) <= 1e-12
+
+
+def test_qr_preconditioned_jacobi_against_oracle(gpu_ctx, monkeypatch):
+    # the optional QR-preconditioned Jacobi (sorted columns, X = Q1 R1,
+    # R1^H = Q2 R2, Jacobi on R2^H) must reproduce the oracle like the
+    # default path
+    monkeypatch.setenv("QTN_JACOBI_PRECOND", "1")
+    n, chimax, cutoff = 40, 256, 1e-8
+    layers = C.brickwall_layers(n, 20, 2)[:9]
+    st = O.TebdState(n)
+    rec_o = []
+    O.run_layers(st, layers, cutoff, chimax, rec_o)
+    for batched in (True, False):
+        d = DeviceMPS(n, chimax)
+        rec = []
+        d.run_layers(layers, cutoff, chimax, batched=batched, record=rec)
+        assert [r[1] for r in rec] == [r[1] for r in rec_o]
+        bits = C.random_bitstrings(n, 64, 12)
+        a, ref = d.amplitudes(bits), st.amplitudes(bits)
+        assert np.linalg.norm(a - ref) <= TOL * np.linalg.norm(ref)
+        assert np.abs(d.expect_z() - st.expect_z()).max() <= TOL
