@@ -21,6 +21,9 @@
  *                            contract(net), src/contract.jl:242-264
  *   qtn_svd_trunc         <- LinearAlgebra.svd + truncation rule of
  *                            contract_svd, src/svd.jl:26-33
+ *   qtn_svd               <- LinearAlgebra.svd of MPS(psi), MPO(m),
+ *                            decompose!, switch! (src/mps.jl:147,157, ...)
+ *   qtn_contract_svd      <- contract_svd, src/svd.jl:7-38
  *   qtn_mps_*             <- MPS container / switch! two-site split,
  *                            src/mps.jl:7-89, src/switch.jl:18-56
  *   qtn_mpo_* / zip-up    <- apply_MPO wiring + contract(),
@@ -143,9 +146,31 @@ int32_t qtn_svd_trunc(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
                       int64_t lda, double cutoff, int32_t relative,
                       int64_t chimax, int64_t* k, double* U, double* S,
                       double* Vh, double* disc_weight);
+/* Thin SVD without truncation: LinearAlgebra.svd as called at
+ * src/mps.jl:147,157, src/mpo.jl:53,62, src/decompose.jl:26,38 and
+ * src/switch.jl:77.  All k = min(m,n) values are returned, descending:
+ * U (m x k, ld m), S (k), Vh (k x n, ld k); absorb = 1 returns diag(S)*V^H
+ * in Vh (the product every one of those call sites forms next).  Any
+ * output may be NULL.  An exactly zero singular value comes with a zero
+ * singular vector (no orthonormal completion).                            */
+int32_t qtn_svd(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
+                int64_t lda, int32_t absorb, double* U, double* S,
+                double* Vh);
 /* Full singular value list (min(m,n) values, descending) of A; no vectors. */
 int32_t qtn_svd_values(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
                        int64_t lda, double* S);
+
+/* contract_svd (src/svd.jl:7-38): contract leg legA (1-based) of A with
+ * leg legB of B after replacing each operand, matricised as
+ * (other legs) x (leg) and (leg) x (other legs), by its SVD truncated at
+ * the absolute tail norm er (src/svd.jl:29-33).  C has the free legs of A
+ * then of B, in order, column-major.  Errors: "Error must be positive"
+ * (er < 0), "Dimensions of contraction legs do not match", QTN_ERR_TRUNC
+ * when an operand has every singular value within er.                    */
+int32_t qtn_contract_svd(qtn_ctx* ctx, const double* A, int32_t ndA,
+                         const int64_t* dimsA, const double* B, int32_t ndB,
+                         const int64_t* dimsB, int32_t legA, int32_t legB,
+                         double er, double* C);
 
 /* ---- matrix-product states --------------------------------------------- */
 /* An MPS of `nsites` qubits lives on the device in right-canonical form

@@ -70,4 +70,11 @@ int32_t launch_identity(hipStream_t s, double2* W, int64_t n, int64_t sW,
 
 inline int64_t round_up64(int64_t x) { return ((x + 63) / 64) * 64; }
 
+// SVD of a device matrix (defined in qtn_svd_api.hip; contract documented
+// there).  Uses context slots 12 and 13.
+int32_t svd_device(qtn_ctx* ctx, const double2* dA, int64_t m, int64_t n,
+                   int64_t lda, bool truncate, double cutoff, int relative,
+                   int64_t chimax, int absorb, double2* dU, double* dS,
+                   double2* dVh, int64_t* k_out, double* disc_out);
+
 }  // namespace qtn

@@ -54,7 +54,8 @@ __global__ void gather_cols(const double2* __restrict__ src, int64_t ld_src,
   }
 }
 
-// dst (k x cols, ld k) row j = conj(src column perm[j])^T, optionally / sig.
+// dst (k x cols, ld k) row j = conj(src column perm[j])^T; scale 1 divides
+// by sig[j], scale 2 multiplies by it.
 __global__ void gather_rows_conj(const double2* __restrict__ src,
                                  int64_t ld_src, int64_t cols,
                                  const int* __restrict__ perm,
@@ -68,9 +69,9 @@ __global__ void gather_rows_conj(const double2* __restrict__ src,
     v.y = -v.y;
     if (scale) {
       const double s = sig[j];
-      const double inv = s > 0.0 ? 1.0 / s : 0.0;
-      v.x *= inv;
-      v.y *= inv;
+      const double f = scale == 2 ? s : (s > 0.0 ? 1.0 / s : 0.0);
+      v.x *= f;
+      v.y *= f;
     }
     dst[idx] = v;
   }
@@ -83,16 +84,22 @@ unsigned grid_for(int64_t n) {
 
 constexpr int kSlotA = 11, kSlotX = 12, kSlotW = 13, kSlotU = 14, kSlotV = 15;
 
-struct HostSvdOut {
-  int64_t k = 0;
-  double disc = 0.0;
-  std::vector<double> sig;
-};
+}  // namespace
 
-int32_t svd_common(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
-                   int64_t lda, double cutoff, int relative, int64_t chimax,
-                   bool want_vectors, int64_t* k_out, double* U, double* S,
-                   double* Vh, double* disc_out, bool values_only) {
+// SVD of a device matrix A (m x n, ld lda) by the batched Jacobi engine.
+//   truncate: tail rule of src/svd.jl:29-33 (er = cutoff, or cutoff*||s||
+//             when relative), then k <= chimax (chimax > 0); otherwise
+//             k = min(m, n) (exact zeros kept, as LinearAlgebra.svd does).
+//   absorb:   Vh rows are scaled by S (the S*V^H of src/switch.jl:82-89,
+//             src/mps.jl:152 and src/mpo.jl:56).
+// Device outputs (any may be null): dU (m x k, ld m), dS (k), dVh (k x n,
+// ld k); their capacity must be min(m, n) columns / values / rows.  For an
+// exactly zero singular value the matching vector is zero (the Jacobi
+// engine produces no orthonormal completion).  Uses slots 12 and 13.
+int32_t svd_device(qtn_ctx* ctx, const double2* dA, int64_t m, int64_t n,
+                   int64_t lda, bool truncate, double cutoff, int relative,
+                   int64_t chimax, int absorb, double2* dU, double* dS,
+                   double2* dVh, int64_t* k_out, double* disc_out) {
   hipStream_t s = ctx->stream;
   const int trans = (m < n) ? 1 : 0;  // columns of X = min(m, n)
   const int64_t xr = trans ? n : m;
@@ -102,84 +109,104 @@ int32_t svd_common(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
     set_error("svd: min(m, n) above 4096 is not supported");
     return QTN_ERR_ARG;
   }
-  void *dA, *dX, *dW = nullptr;
-  QTN_TRY(ctx->buffer(kSlotA, (size_t)m * n * sizeof(double2), &dA));
+  const bool want_vectors = dU != nullptr || dVh != nullptr;
+  void *dX, *dW = nullptr;
   QTN_TRY(ctx->buffer(kSlotX, (size_t)rows_p * cols_p * sizeof(double2),
                       &dX));
   if (want_vectors)
     QTN_TRY(ctx->buffer(kSlotW, (size_t)cols_p * cols_p * sizeof(double2),
                         &dW));
-  QTN_CHECK_HIP(hipMemcpy2DAsync(dA, m * sizeof(double2), A,
-                                 lda * sizeof(double2), m * sizeof(double2), n,
-                                 hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(build_x, dim3(grid_for(rows_p * cols_p)), dim3(256), 0,
-                     s, (const double2*)dA, m, n, m, trans, (double2*)dX,
-                     rows_p, cols_p);
+                     s, dA, m, n, lda, trans, (double2*)dX, rows_p, cols_p);
   QTN_CHECK_HIP(hipGetLastError());
   JacobiEngine eng;
   QTN_TRY(eng.reserve(ctx, 1, rows_p, cols_p));
   const int64_t nsv = std::min(m, n);
   int sweeps = 0;
   QTN_TRY(eng.run(ctx, (double2*)dX, rows_p, rows_p * cols_p, (double2*)dW,
-                  cols_p * cols_p, nullptr, &nsv, cutoff, relative, chimax, 60,
+                  cols_p * cols_p, nullptr, &nsv, truncate ? cutoff : 0.0,
+                  truncate ? relative : 0, truncate ? chimax : 0, 60,
                   &sweeps));
-  // read back the decision
-  int64_t k = 0;
-  int st = 0;
+  int64_t k = nsv;
   double disc = 0.0;
-  QTN_CHECK_HIP(hipMemcpyAsync(&k, eng.kept, sizeof(int64_t),
-                               hipMemcpyDeviceToHost, s));
-  QTN_CHECK_HIP(hipMemcpyAsync(&st, eng.status, sizeof(int),
-                               hipMemcpyDeviceToHost, s));
-  QTN_CHECK_HIP(hipMemcpyAsync(&disc, eng.disc, sizeof(double),
-                               hipMemcpyDeviceToHost, s));
-  QTN_CHECK_HIP(hipStreamSynchronize(s));
-  if (values_only) {
-    QTN_CHECK_HIP(hipMemcpy(S, eng.sig, nsv * sizeof(double),
-                            hipMemcpyDeviceToHost));
-    return QTN_OK;
-  }
-  if (st != 0) {
-    set_error("svd: every singular value is within the truncation error");
-    return QTN_ERR_TRUNC;
+  if (truncate) {
+    int st = 0;
+    QTN_CHECK_HIP(hipMemcpyAsync(&k, eng.kept, sizeof(int64_t),
+                                 hipMemcpyDeviceToHost, s));
+    QTN_CHECK_HIP(hipMemcpyAsync(&st, eng.status, sizeof(int),
+                                 hipMemcpyDeviceToHost, s));
+    QTN_CHECK_HIP(hipMemcpyAsync(&disc, eng.disc, sizeof(double),
+                                 hipMemcpyDeviceToHost, s));
+    QTN_CHECK_HIP(hipStreamSynchronize(s));
+    if (st != 0) {
+      set_error("svd: every singular value is within the truncation error");
+      return QTN_ERR_TRUNC;
+    }
   }
   if (k_out) *k_out = k;
   if (disc_out) *disc_out = disc;
-  if (S && k > 0)
-    QTN_CHECK_HIP(hipMemcpy(S, eng.sig, k * sizeof(double),
-                            hipMemcpyDeviceToHost));
-  if (k == 0) return QTN_OK;
-  void *dU = nullptr, *dV = nullptr;
-  if (U) QTN_TRY(ctx->buffer(kSlotU, (size_t)m * k * sizeof(double2), &dU));
-  if (Vh) QTN_TRY(ctx->buffer(kSlotV, (size_t)k * n * sizeof(double2), &dV));
+  if (dS && k > 0)
+    QTN_CHECK_HIP(hipMemcpyAsync(dS, eng.sig, k * sizeof(double),
+                                 hipMemcpyDeviceToDevice, s));
+  if (k == 0 || !want_vectors) return QTN_OK;
   if (!trans) {
     // X = A: A W = U diag(s), V = W
-    if (U)
+    if (dU)
       hipLaunchKernelGGL(gather_cols, dim3(grid_for(m * k)), dim3(256), 0, s,
                          (const double2*)dX, rows_p, m, eng.perm, eng.sig, 1,
-                         k, (double2*)dU);
-    if (Vh)
+                         k, dU);
+    if (dVh)
       hipLaunchKernelGGL(gather_rows_conj, dim3(grid_for(n * k)), dim3(256),
                          0, s, (const double2*)dW, cols_p, n, eng.perm,
-                         eng.sig, 0, k, (double2*)dV);
+                         eng.sig, absorb ? 2 : 0, k, dVh);
   } else {
     // X = A^H: A^H W = V diag(s), U = W
-    if (U)
+    if (dU)
       hipLaunchKernelGGL(gather_cols, dim3(grid_for(m * k)), dim3(256), 0, s,
                          (const double2*)dW, cols_p, m, eng.perm, eng.sig, 0,
-                         k, (double2*)dU);
-    if (Vh)
+                         k, dU);
+    if (dVh)
       hipLaunchKernelGGL(gather_rows_conj, dim3(grid_for(n * k)), dim3(256),
                          0, s, (const double2*)dX, rows_p, n, eng.perm,
-                         eng.sig, 1, k, (double2*)dV);
+                         eng.sig, absorb ? 0 : 1, k, dVh);
   }
   QTN_CHECK_HIP(hipGetLastError());
-  if (U)
-    QTN_CHECK_HIP(hipMemcpyAsync(U, dU, (size_t)m * k * sizeof(double2),
-                                 hipMemcpyDeviceToHost, s));
-  if (Vh)
-    QTN_CHECK_HIP(hipMemcpyAsync(Vh, dV, (size_t)k * n * sizeof(double2),
-                                 hipMemcpyDeviceToHost, s));
+  return QTN_OK;
+}
+
+namespace {
+
+// Host matrix in, host factors out (the qtn_svd* entry points).
+int32_t svd_host(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
+                 int64_t lda, bool truncate, double cutoff, int relative,
+                 int64_t chimax, int absorb, int64_t* k_out, double* U,
+                 double* S, double* Vh, double* disc_out) {
+  hipStream_t s = ctx->stream;
+  const int64_t kk = std::min(m, n);
+  void *dA, *dU = nullptr, *dV = nullptr, *dS = nullptr;
+  QTN_TRY(ctx->buffer(kSlotA, (size_t)m * n * sizeof(double2), &dA));
+  if (U) QTN_TRY(ctx->buffer(kSlotU, (size_t)m * kk * sizeof(double2), &dU));
+  if (Vh) QTN_TRY(ctx->buffer(kSlotV, (size_t)kk * n * sizeof(double2), &dV));
+  if (S) QTN_TRY(ctx->buffer(16, (size_t)kk * sizeof(double), &dS));
+  QTN_CHECK_HIP(hipMemcpy2DAsync(dA, m * sizeof(double2), A,
+                                 lda * sizeof(double2), m * sizeof(double2), n,
+                                 hipMemcpyHostToDevice, s));
+  int64_t k = 0;
+  QTN_TRY(svd_device(ctx, (const double2*)dA, m, n, m, truncate, cutoff,
+                     relative, chimax, absorb, (double2*)dU, (double*)dS,
+                     (double2*)dV, &k, disc_out));
+  if (k_out) *k_out = k;
+  if (k > 0) {
+    if (S)
+      QTN_CHECK_HIP(hipMemcpyAsync(S, dS, k * sizeof(double),
+                                   hipMemcpyDeviceToHost, s));
+    if (U)
+      QTN_CHECK_HIP(hipMemcpyAsync(U, dU, (size_t)m * k * sizeof(double2),
+                                   hipMemcpyDeviceToHost, s));
+    if (Vh)
+      QTN_CHECK_HIP(hipMemcpyAsync(Vh, dV, (size_t)k * n * sizeof(double2),
+                                   hipMemcpyDeviceToHost, s));
+  }
   QTN_CHECK_HIP(hipStreamSynchronize(s));
   return QTN_OK;
 }
@@ -199,9 +226,21 @@ int32_t qtn_svd_trunc(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
   QTN_ARG(lda >= m, "qtn_svd_trunc: lda < m");
   QTN_ARG(cutoff >= 0.0, "Error must be positive");
   QTN_CHECK_HIP(hipSetDevice(ctx->device));
-  return qtn::svd_common(ctx, m, n, A, lda, cutoff, relative, chimax,
-                         U != nullptr || Vh != nullptr, k, U, S, Vh,
-                         disc_weight, false);
+  return qtn::svd_host(ctx, m, n, A, lda, true, cutoff, relative, chimax, 0,
+                       k, U, S, Vh, disc_weight);
+}
+
+int32_t qtn_svd(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
+                int64_t lda, int32_t absorb, double* U, double* S,
+                double* Vh) {
+  qtn::clear_error();
+  QTN_ARG(ctx != nullptr && A != nullptr, "qtn_svd: NULL argument");
+  QTN_ARG(m >= 1 && n >= 1, "qtn_svd: empty matrix");
+  QTN_ARG(lda >= m, "qtn_svd: lda < m");
+  QTN_ARG(absorb == 0 || absorb == 1, "qtn_svd: absorb must be 0 or 1");
+  QTN_CHECK_HIP(hipSetDevice(ctx->device));
+  return qtn::svd_host(ctx, m, n, A, lda, false, 0.0, 0, 0, absorb, nullptr,
+                       U, S, Vh, nullptr);
 }
 
 int32_t qtn_svd_values(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
@@ -212,8 +251,8 @@ int32_t qtn_svd_values(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
   QTN_ARG(m >= 1 && n >= 1, "qtn_svd_values: empty matrix");
   QTN_ARG(lda >= m, "qtn_svd_values: lda < m");
   QTN_CHECK_HIP(hipSetDevice(ctx->device));
-  return qtn::svd_common(ctx, m, n, A, lda, 0.0, 0, 0, false, nullptr,
-                         nullptr, S, nullptr, nullptr, true);
+  return qtn::svd_host(ctx, m, n, A, lda, false, 0.0, 0, 0, 0, nullptr,
+                       nullptr, S, nullptr, nullptr);
 }
 
 }  // extern "C"

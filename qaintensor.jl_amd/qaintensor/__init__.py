@@ -4,5 +4,12 @@ from ._lib import (Context, QtnError, QtnLibraryError, default_context,
                    load as load_library, lib_path)
 from . import circuits
 from .device import DeviceMPS, FORM_PLAIN, FORM_CHOSEN
+from .network import (ErrorException, GeneralTensorNetwork, Summation,
+                      Tensor, TensorNetwork, contract, contract_rep,
+                      is_power_two, shift_pair, shift_summation)
+from .mps import (MPS, ClosedMPS, OpenMPS, PeriodicMPS, check_mps,
+                  contract_svd, contract_svd_mps, permute_mps, switch)
+from .mpo import (MPO, CircuitGate, apply_MPO, decompose, extend_MPO,
+                  tensor_circuit)
 
 __version__ = "0.1.0"

@@ -31,7 +31,7 @@ EXPORTS = [
     "qtn_last_error", "qtn_version",
     "qtn_contract_pair", "qtn_contract_pair_dev", "qtn_permute",
     "qtn_permute_dev", "qtn_zgemm_dev", "qtn_cgemm_dev",
-    "qtn_svd_trunc", "qtn_svd_values",
+    "qtn_svd_trunc", "qtn_svd", "qtn_svd_values", "qtn_contract_svd",
     "qtn_mps_create", "qtn_mps_destroy", "qtn_mps_copy", "qtn_mps_set_sites",
     "qtn_mps_bond_dims", "qtn_mps_bond_spectrum", "qtn_mps_get_site",
     "qtn_mps_apply_1q", "qtn_mps_apply_2q", "qtn_mps_apply_layer",
@@ -94,7 +94,10 @@ def _declare(lib):
                                  _i64, _i64]),
         "qtn_svd_trunc": (_i32, [_vp, _i64, _i64, _pd, _i64, _dbl, _i32, _i64,
                                  _pi64, _pd, _pd, _pd, _pd]),
+        "qtn_svd": (_i32, [_vp, _i64, _i64, _pd, _i64, _i32, _pd, _pd, _pd]),
         "qtn_svd_values": (_i32, [_vp, _i64, _i64, _pd, _i64, _pd]),
+        "qtn_contract_svd": (_i32, [_vp, _pd, _i32, _pi64, _pd, _i32, _pi64,
+                                    _i32, _i32, _dbl, _pd]),
         "qtn_mps_create": (_i32, [_vp, _i32, _i64, ctypes.POINTER(_vp)]),
         "qtn_mps_destroy": (_i32, [_vp]),
         "qtn_mps_copy": (_i32, [_vp, _vp]),

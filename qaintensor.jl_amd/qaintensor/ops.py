@@ -101,3 +101,41 @@ def svd_values(A, ctx=None):
     L.check(L.load().qtn_svd_values(ctx.handle, m, n, L.dptr(A), m,
                                     L.dptr(S)))
     return S
+
+
+def svd(A, absorb=False, ctx=None):
+    """Thin SVD without truncation on the GPU (qtn_svd): (U, S, Vh) with all
+    min(m, n) singular values, descending; absorb=True returns diag(S) @ Vh
+    in place of Vh.  A zero singular value comes with a zero vector."""
+    ctx = _ctx(ctx)
+    A = L.as_c128(A, "F")
+    if A.ndim != 2:
+        raise ValueError("svd: matrix expected")
+    m, n = A.shape
+    k = min(m, n)
+    U = np.zeros((m, k), dtype=np.complex128, order="F")
+    S = np.zeros(k, dtype=np.float64)
+    Vh = np.zeros((k, n), dtype=np.complex128, order="F")
+    L.check(L.load().qtn_svd(ctx.handle, m, n, L.dptr(A), m,
+                             int(bool(absorb)), L.dptr(U), L.dptr(S),
+                             L.dptr(Vh)))
+    return U, S, Vh
+
+
+def contract_svd(A, B, legA, legB, er=0.0, ctx=None):
+    """contract_svd on the GPU (qtn_contract_svd): leg legA of A with leg
+    legB of B (1-based), each operand first replaced by its SVD truncated at
+    the absolute tail norm er."""
+    ctx = _ctx(ctx)
+    A = L.as_c128(A, "F")
+    B = L.as_c128(B, "F")
+    if A.ndim < 1 or B.ndim < 1:
+        raise ValueError("contract_svd: tensors of rank >= 1 expected")
+    dA, dB = _dims(A), _dims(B)
+    shape = [A.shape[i] for i in range(A.ndim) if i != legA - 1] + \
+            [B.shape[i] for i in range(B.ndim) if i != legB - 1]
+    C = np.zeros(shape, dtype=np.complex128, order="F")
+    L.check(L.load().qtn_contract_svd(
+        ctx.handle, L.dptr(A), A.ndim, L.i64ptr(dA), L.dptr(B), B.ndim,
+        L.i64ptr(dB), int(legA), int(legB), float(er), L.dptr(C)))
+    return C
