@@ -6,9 +6,13 @@ A step is one full circuit from |0...0> (state reset by a device copy, then
 all 20 layers).  Forms:
   plain    -- complex FP64, one two-site update at a time
   batched  -- complex FP64, every gate of a layer in one batched pass
-The headline `value` is the fastest form whose state matches the plain form
-(infidelity printed per form).  Gate matrices are uploaded per layer
-(16 complex numbers per gate); the MPS stays resident in HBM.
+  chosen   -- batched, Jacobi sweeps in complex FP32 on MFMA, then V_k
+              re-orthogonalised (Newton-Schulz) and B_i, lambda_i recomputed
+              in FP64
+The headline `value` is the fastest form whose state is within 1e-6
+infidelity of the plain FP64 state (infidelity printed per form).  Gate
+matrices are uploaded per layer (16 complex numbers per gate); the MPS
+stays resident in HBM.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -40,25 +44,41 @@ def gate_flops(chiL, chiM, chiR):
 HBM_PEAK_GBS = 8000.0
 
 
-def roofline(prof):
+FP32_PEAK_TFLOPS = 157.3
+
+
+def roofline(prof, fp32=False):
     """Roofline of the SVD kernel with the largest summed time in the timed
-    region (HIP events on the library stream): jac_inner is bounded by its
-    HBM traffic (partial Grams in, rotation block out); jac_gram and
-    jac_update are FP64 MFMA GEMMs."""
+    region (HIP events on the library stream), from its algorithmic bytes
+    and flops (counted by the library per launch):
+      jac_round  -- the fused round (Gram + rotations + update of one block
+                    pair per workgroup): both fractions are computed and the
+                    larger one names the bound
+      jac_inner  -- HBM (partial Grams in, rotation block out)
+      jac_gram / jac_update -- MFMA GEMMs
+    MFMA peak: FP64 78.6 TF, or FP32 157.3 TF in the chosen form."""
     name = max(prof, key=lambda k: prof[k]["ms"])
     p = prof[name]
-    avg_s = p["ms"] / max(1, p["count"]) / 1e3
+    n = max(1, p["count"])
+    avg_s = p["ms"] / n / 1e3
+    peak = FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS
+    gbs = p["bytes"] / n / avg_s / 1e9
+    tfs = p["flops"] / n / avg_s / 1e12
+    hbm = {"kernel": name, "bound": "hbm", "achieved": gbs,
+           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+           "traffic": None, "avg_launch_us": avg_s * 1e6, "launches": n}
+    mfma = {"kernel": name, "bound": "mfma", "achieved": tfs, "peak": peak,
+            "unit": "TFLOP/s", "frac": tfs / peak, "traffic": None,
+            "avg_launch_us": avg_s * 1e6, "launches": n}
     if name == "jac_inner":
-        ach = p["bytes"] / max(1, p["count"]) / avg_s / 1e9
-        return {"kernel": name, "bound": "hbm", "achieved": ach,
-                "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS, "traffic": None,
-                "avg_launch_us": avg_s * 1e6, "launches": p["count"]}
-    ach = p["flops"] / max(1, p["count"]) / avg_s / 1e12
-    return {"kernel": name, "bound": "mfma", "achieved": ach,
-            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
-            "avg_launch_us": avg_s * 1e6, "launches": p["count"]}
+        return hbm
+    if name == "jac_round":
+        out = hbm if hbm["frac"] >= mfma["frac"] else mfma
+        out["other"] = {"bound": "mfma" if out is hbm else "hbm",
+                        "achieved": (mfma if out is hbm else hbm)["achieved"],
+                        "frac": (mfma if out is hbm else hbm)["frac"]}
+        return out
+    return mfma
 
 
 def infidelity(a, b):
@@ -67,9 +87,10 @@ def infidelity(a, b):
     return 1.0 - abs(a.overlap(b)) ** 2 / (a.norm2() * b.norm2())
 
 
-def run_circuit(dev, pristine, layers, batched, record=None):
+def run_circuit(dev, pristine, layers, batched, record=None, form=0):
     dev.copy_from(pristine)
-    dev.run_layers(layers, CUTOFF, CHIMAX, batched=batched, record=record)
+    dev.run_layers(layers, CUTOFF, CHIMAX, form=form, batched=batched,
+                   record=record)
 
 
 def cpu_baseline(layers):
@@ -98,7 +119,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-batched", action="store_true",
-                    help="time the batched form only (for profiler runs)")
+                    help="time the headline form only (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU "
@@ -107,7 +128,8 @@ def main():
         sys.exit(2)
 
     import torch
-    from qaintensor import DeviceMPS, circuits as C, default_context
+    from qaintensor import (DeviceMPS, FORM_CHOSEN, FORM_PLAIN,
+                            circuits as C, default_context)
 
     ctx = default_context()
     layers = C.brickwall_layers(N_QUBITS, DEPTH, SEED)
@@ -129,21 +151,31 @@ def main():
         torch.cuda.synchronize()
         plain_s = (time.perf_counter() - t0) / args.plain_steps
 
-    # batched form
-    for _ in range(args.warmup):
-        run_circuit(dev, pristine, layers, batched=True)
-    torch.cuda.synchronize()
-    ctx.counters(reset=True)
-    ctx.profile(True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        run_circuit(dev, pristine, layers, batched=True)
-    torch.cuda.synchronize()
-    batched_s = (time.perf_counter() - t0) / args.steps
-    prof = ctx.profile_read()
-    ctx.profile(False)
-    cnt = ctx.counters()
-    inf_batched = infidelity(ref, dev)
+    def timed(form, profile):
+        for _ in range(args.warmup):
+            run_circuit(dev, pristine, layers, batched=True, form=form)
+        torch.cuda.synchronize()
+        ctx.counters(reset=True)
+        if profile:
+            ctx.profile(True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            run_circuit(dev, pristine, layers, batched=True, form=form)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        prof = ctx.profile_read() if profile else None
+        ctx.profile(False)
+        return dt, prof, ctx.counters(), infidelity(ref, dev)
+
+    # batched FP64 form
+    batched_s, prof64, cnt, inf_batched = timed(FORM_PLAIN,
+                                                not args.only_batched)
+    # chosen form (FP32 sweeps, FP64 re-orthogonalisation): the headline
+    # when it stays within 1e-6 infidelity of the FP64 state
+    chosen_s, prof, cnt32, inf_chosen = timed(FORM_CHOSEN, True)
+    use_chosen = inf_chosen <= 1e-6
+    if not use_chosen:
+        prof = prof64
 
     # algorithmic flops of the circuit from the plain run's bond dims
     # replay the bond dimensions: kept ranks of every gate in order
@@ -158,25 +190,29 @@ def main():
             st_dims[w] = k
     plain_rate = None if args.only_batched else n2q / plain_s
     batched_rate = n2q / batched_s
-    achieved = dims_flops / batched_s / 1e12
+    chosen_rate = n2q / chosen_s
+    head_s = chosen_s if use_chosen else batched_s
+    head_rate = n2q / head_s
+    achieved = dims_flops / head_s / 1e12
 
     line = {
         "metric": "2q-gate applies/s, 40q MPS chi=256: plain FP64 vs chosen "
                   "form at <=1e-6 infidelity",
-        "value": batched_rate,
+        "value": head_rate,
         "unit": "2q-gate applies/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": batched_s * 1e3,
+        "ms_per_step": head_s * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128",
+        "dtype": "c64-sweeps/c128-core" if use_chosen else "c128",
         "data": "synthetic (seeded Haar brick-wall gates)",
         "config": {"workload": "40-qubit MPS, Haar brick-wall depth 20, "
                                "chi_max 256, cutoff 1e-8 (390 2q gates)",
-                   "form": "batched-fp64", "n_qubits": N_QUBITS,
+                   "form": "chosen" if use_chosen else "batched-fp64",
+                   "n_qubits": N_QUBITS,
                    "depth": DEPTH, "chi_max": CHIMAX, "cutoff": CUTOFF},
         "forms": {
             "plain": (None if plain_rate is None else
@@ -187,8 +223,15 @@ def main():
                              "infidelity_vs_fp64": inf_batched,
                              "jacobi_sweeps_per_svd":
                                  cnt["sweeps"] / max(1, cnt["svd"])},
+            "chosen": {"value": chosen_rate,
+                       "ms_per_circuit": chosen_s * 1e3,
+                       "infidelity_vs_fp64": inf_chosen,
+                       "jacobi_sweeps_per_svd":
+                           cnt32["sweeps"] / max(1, cnt32["svd"]),
+                       "vs_plain": (None if plain_rate is None else
+                                    chosen_rate / plain_rate)},
         },
-        "roofline": roofline(prof),
+        "roofline": roofline(prof, fp32=use_chosen),
         "pipeline_roofline": {
             "bound": "mfma", "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",

@@ -71,8 +71,9 @@ int32_t qtn_ctx_stream(qtn_ctx* ctx, void** stream);
 int32_t qtn_ctx_counters(qtn_ctx* ctx, int64_t* out, int32_t n,
                          int32_t reset);
 /* Per-kernel timing with HIP events on the context stream (off by default).
- * Kernel ids: 0 jac_inner, 1 jac_gram, 2 jac_update (the block Jacobi SVD
- * kernels).  qtn_ctx_profile(ctx, 1) enables and clears the accumulators.
+ * Kernel ids: 0 jac_inner, 1 jac_gram, 2 jac_update (the three-launch
+ * block Jacobi round), 3 jac_round (the fused round used for 32-column
+ * block pairs).  qtn_ctx_profile(ctx, 1) enables and clears the accumulators.
  * qtn_ctx_profile_read returns, for one kernel id, the launch count, the
  * summed launch durations (ms), and the algorithmic HBM bytes and flops of
  * those launches (counted for the problems active in each launch).        */

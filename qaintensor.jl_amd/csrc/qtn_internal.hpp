@@ -99,12 +99,12 @@ struct qtn_ctx {
   // work counters (see qtn_ctx_counters)
   int64_t n_svd = 0, n_sweeps = 0, max_sweeps = 0, n_batches = 0;
   // per-kernel event timing (qtn_ctx_profile)
-  static constexpr int kProfKernels = 3;
+  static constexpr int kProfKernels = 4;
   bool prof = false;
-  int64_t prof_count[kProfKernels] = {0, 0, 0};
-  double prof_ms[kProfKernels] = {0, 0, 0};
-  double prof_bytes[kProfKernels] = {0, 0, 0};
-  double prof_flops[kProfKernels] = {0, 0, 0};
+  int64_t prof_count[kProfKernels] = {0, 0, 0, 0};
+  double prof_ms[kProfKernels] = {0, 0, 0, 0};
+  double prof_bytes[kProfKernels] = {0, 0, 0, 0};
+  double prof_flops[kProfKernels] = {0, 0, 0, 0};
   std::vector<hipEvent_t> ev_pool;  // recycled events
   // Small pinned host staging buffer for scalar read-backs.
   void* pinned = nullptr;

@@ -40,7 +40,8 @@ constexpr int kSlotPhi = 30, kSlotXm = 31, kSlotVk = 32, kSlotDesc = 33,
               kSlotGates = 34, kSlotEnv = 35, kSlotT1 = 36, kSlotT2 = 37,
               kSlotRed = 38, kSlotBits = 39, kSlotAmp = 40, kSlotW2 = 41,
               kSlotA1 = 42, kSlotA2 = 43, kSlotQd = 44, kSlotNrm = 45,
-              kSlotPerm2 = 46;
+              kSlotPerm2 = 46, kSlotX32 = 70, kSlotW32 = 71, kSlotNsG = 72,
+              kSlotNsT = 73, kSlotS64 = 74, kSlotMix = 75, kSlotEng32 = 80;
 
 unsigned grid_for(int64_t n) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256,
@@ -152,7 +153,7 @@ __global__ void write_right(const GateDesc* __restrict__ gd,
                             const double* __restrict__ sig,
                             const int64_t* __restrict__ kept,
                             const double* __restrict__ keptw,
-                            int64_t cols_p) {
+                            int64_t cols_p, int write_lambda) {
   const int bz = blockIdx.y;
   const GateDesc d = gd[bz];
   const int64_t k = kept[bz];
@@ -164,7 +165,7 @@ __global__ void write_right(const GateDesc* __restrict__ gd,
     const double2 v = d.Vk[c + N * j];
     d.Br[j + k * c] = make_double2(v.x, -v.y);
   }
-  if (blockIdx.x == 0) {
+  if (write_lambda && blockIdx.x == 0) {
     const double* s = sig + (int64_t)bz * cols_p;
     const double nrm = sqrt(keptw[bz]);
     for (int64_t j = threadIdx.x; j < k; j += blockDim.x)
@@ -246,6 +247,138 @@ __global__ __launch_bounds__(256) void env_reduce(
   if (threadIdx.x < 4) out[threadIdx.x] = red[threadIdx.x][0];
 }
 
+// ---- chosen form: FP32 sweeps, FP64 re-orthogonalisation ----------------
+// Y = (complex float) X over n elements.
+__global__ void to_single(const double2* __restrict__ X,
+                          float2* __restrict__ Y, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 v = X[i];
+    Y[i] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
+// V_k (N x k, ld N, FP64) per gate from the engine that solved it:
+// src 1 (FP32 sweeps): mode 0 X32[:, perm32[j]] / sig32[j], mode 2
+// W32[:, perm32[j]]; src 0 (FP64 engine): mode 0 X[:, perm[j]] / sig[j],
+// mode 2 W[:, perm[j]].  X32 / W32 of gate bz start at bz * x_sz / w_sz;
+// X and W have the batch layout (ld d.ldx / cols_p).  grid (tiles, batch).
+__global__ void extract_vk_mixed(const GateDesc* __restrict__ gd,
+                                 const int* __restrict__ src,
+                                 const float2* __restrict__ X32,
+                                 int64_t x_sz,
+                                 const float2* __restrict__ W32,
+                                 int64_t w_sz,
+                                 const double* __restrict__ sig32,
+                                 const int* __restrict__ perm32,
+                                 const double* __restrict__ sig64,
+                                 const int* __restrict__ perm64,
+                                 const int64_t* __restrict__ kept,
+                                 int64_t cols_p, int64_t cols32,
+                                 int64_t cols64) {
+  const int bz = blockIdx.y;
+  const GateDesc d = gd[bz];
+  const int64_t k = kept[bz];
+  const int64_t N = 2 * d.chiR;
+  const bool single = src[bz] != 0;
+  // sig / perm rows are as wide as the engine that solved the gate
+  const int64_t sc = single ? cols32 : cols64;
+  const double* s = (single ? sig32 : sig64) + (int64_t)bz * sc;
+  const int* pm = (single ? perm32 : perm64) + (int64_t)bz * sc;
+  const int64_t total = N * k;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+       idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = idx % N, j = idx / N;
+    const int64_t col = pm[j];
+    double2 v;
+    if (d.mode == 2) {
+      if (single) {
+        const float2 w = W32[bz * w_sz + c + cols_p * col];
+        v = make_double2(w.x, w.y);
+      } else {
+        v = d.W[c + cols_p * col];
+      }
+    } else {
+      const double sj = s[j];
+      const double inv = sj > 0.0 ? 1.0 / sj : 0.0;
+      double2 x;
+      if (single) {
+        const float2 xf = X32[bz * x_sz + c + d.ldx * col];
+        x = make_double2(xf.x, xf.y);
+      } else {
+        x = d.X[c + d.ldx * col];
+      }
+      v = make_double2(x.x * inv, x.y * inv);
+    }
+    d.Vk[c + N * j] = v;
+  }
+}
+
+// Newton-Schulz step V <- 1.5 V - 0.5 T with T = V (V^H V) (N x k, ld N).
+__global__ void ns_combine(const GateDesc* __restrict__ gd,
+                           const double2* __restrict__ T, int64_t t_sz,
+                           const int64_t* __restrict__ kept) {
+  const int bz = blockIdx.y;
+  const GateDesc d = gd[bz];
+  const int64_t total = 2 * d.chiR * kept[bz];
+  const double2* t = T + bz * t_sz;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+       idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+    const double2 v = d.Vk[idx], w = t[idx];
+    d.Vk[idx] = make_double2(1.5 * v.x - 0.5 * w.x, 1.5 * v.y - 0.5 * w.y);
+  }
+}
+
+// sig64[bz][j] = |diag(lambda_{i-1} (x) 1) C[:, j]| for C = Phi V_k, held in
+// B_i (2 chiL x k, ld 2 chiL).  One wave per column; grid (ceil(k/4), batch).
+__global__ __launch_bounds__(256) void left_colnorms(
+    const GateDesc* __restrict__ gd, const int64_t* __restrict__ kept,
+    double* __restrict__ sig64, int64_t cols_p) {
+  const int bz = blockIdx.y;
+  const GateDesc d = gd[bz];
+  const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (j >= kept[bz]) return;
+  const int64_t M = 2 * d.chiL;
+  const double2* c = d.Bl + M * j;
+  double acc = 0.0;
+  for (int64_t r = lane; r < M; r += 64) {
+    const double2 v = c[r];
+    const double la = d.lamL[r % d.chiL];
+    acc += la * la * (v.x * v.x + v.y * v.y);
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (lane == 0) sig64[bz * cols_p + j] = sqrt(acc);
+}
+
+// B_i <- C / |S_k| and lambda_i = S_k / |S_k| with S_k from left_colnorms;
+// the kept weight is summed smallest-first.  grid (tiles, batch).
+__global__ __launch_bounds__(256) void left_finish(
+    const GateDesc* __restrict__ gd, const int64_t* __restrict__ kept,
+    const double* __restrict__ sig64, int64_t cols_p) {
+  const int bz = blockIdx.y;
+  const GateDesc d = gd[bz];
+  const int64_t k = kept[bz];
+  const double* s = sig64 + bz * cols_p;
+  __shared__ double nrm;
+  if (threadIdx.x == 0) {
+    double w = 0.0;
+    for (int64_t j = k - 1; j >= 0; --j) w += s[j] * s[j];
+    nrm = sqrt(w);
+  }
+  __syncthreads();
+  const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+  const int64_t total = 2 * d.chiL * k;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+       idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+    const double2 v = d.Bl[idx];
+    d.Bl[idx] = make_double2(v.x * inv, v.y * inv);
+  }
+  if (blockIdx.x == 0)
+    for (int64_t j = threadIdx.x; j < k; j += blockDim.x)
+      d.lamM[j] = s[j] * inv;
+}
+
 __global__ void fill_one(double2* p) { *p = make_double2(1.0, 0.0); }
 __global__ void fill_lambda_one(double* p) { *p = 1.0; }
 
@@ -255,8 +388,9 @@ __global__ void fill_lambda_one(double* p) { *p = 1.0; }
 // Batched two-site update of disjoint gates (host orchestration).
 int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
                         const double* gates, double cutoff, int64_t chimax,
-                        int64_t* kept_out, double* disc_out, double* spec_out,
-                        int64_t* spec_len) {
+                        int form, int64_t* kept_out, double* disc_out,
+                        double* spec_out, int64_t* spec_len) {
+  const bool chosen = form == QTN_FORM_CHOSEN;
   qtn_ctx* ctx = m->ctx;
   hipStream_t s = ctx->stream;
   std::vector<char> used(m->n + 1, 0);
@@ -284,6 +418,7 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
   // about what it saves at chi = 256; off unless QTN_JACOBI_PRECOND=1.
   bool precond = false;
   if (const char* e = getenv("QTN_JACOBI_PRECOND")) precond = atoi(e) != 0;
+  if (chosen) precond = false;
   const int64_t w_sz = cols_p * cols_p;
   void *dphi, *dX, *dVk, *ddesc, *dg, *dW = nullptr;
   if (any_w)
@@ -294,7 +429,7 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
                                    sizeof(double2),
                       &dVk));
   const size_t desc_bytes = (size_t)ng * (sizeof(GateDesc) +
-                                          2 * sizeof(GemmDesc<double>));
+                                          4 * sizeof(GemmDesc<double>));
   QTN_TRY(ctx->buffer(kSlotDesc, desc_bytes, &ddesc));
   QTN_TRY(ctx->buffer(kSlotGates, (size_t)ng * 16 * sizeof(double2), &dg));
   std::vector<GateDesc> gd(ng);
@@ -393,30 +528,110 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
                             rows_p, cols_p, ng, d_qd + ng));
     QTN_CHECK_HIP(hipStreamSynchronize(s));  // qd vectors live on the host
   }
-  JacobiEngine eng;
-  // every problem preconditioned: only the top cols_p rows are non-zero
-  const int64_t rows_eng = all_pre ? cols_p : rows_p;
-  QTN_TRY(eng.reserve(ctx, ng, rows_eng, cols_p));
+  // The plain form solves every gate with the FP64 engine.  The chosen
+  // form first tries FP32 sweeps on the gates whose theta exceeds the cap
+  // (min(M, N) > chimax) and keeps that result when the cap binds inside
+  // the spectrum FP32 resolves (kept == chimax with every kept value above
+  // the FP32 noise floor); every other gate goes to the FP64 engine.
+  JacobiEngine eng, eng32;
+  eng32.slot_base = kSlotEng32;
   std::vector<int> wantw(ng);
   for (int g = 0; g < ng; ++g) wantw[g] = trans[g] ? 0 : 1;
   int sweeps = 0;
-  QTN_TRY(eng.run(ctx, (double2*)dX, rows_p, x_sz, (double2*)dW, w_sz,
-                  any_w ? wantw.data() : nullptr, nsv.data(), cutoff, 1,
-                  chimax, 60, &sweeps));
-  std::vector<int64_t> kept(ng);
-  std::vector<int> st(ng);
-  std::vector<double> disc(ng), total(ng), keptw(ng);
-  QTN_CHECK_HIP(hipMemcpyAsync(kept.data(), eng.kept, ng * sizeof(int64_t),
-                               hipMemcpyDeviceToHost, s));
-  QTN_CHECK_HIP(hipMemcpyAsync(st.data(), eng.status, ng * sizeof(int),
-                               hipMemcpyDeviceToHost, s));
-  QTN_CHECK_HIP(hipMemcpyAsync(disc.data(), eng.disc, ng * sizeof(double),
-                               hipMemcpyDeviceToHost, s));
-  QTN_CHECK_HIP(hipMemcpyAsync(total.data(), eng.total, ng * sizeof(double),
-                               hipMemcpyDeviceToHost, s));
-  QTN_CHECK_HIP(hipMemcpyAsync(keptw.data(), eng.keptw, ng * sizeof(double),
-                               hipMemcpyDeviceToHost, s));
-  QTN_CHECK_HIP(hipStreamSynchronize(s));
+  std::vector<int64_t> kept(ng, 0);
+  std::vector<int> st(ng, 0), src(ng, 0), need64(ng, 1);
+  std::vector<double> disc(ng, 0.0), total(ng, 0.0), keptw(ng, 0.0);
+  auto read_results = [&](JacobiEngine& e, std::vector<int64_t>& k,
+                          std::vector<int>& t, std::vector<double>& d,
+                          std::vector<double>& tt,
+                          std::vector<double>& kw) -> int32_t {
+    QTN_CHECK_HIP(hipMemcpyAsync(k.data(), e.kept, ng * sizeof(int64_t),
+                                 hipMemcpyDeviceToHost, s));
+    QTN_CHECK_HIP(hipMemcpyAsync(t.data(), e.status, ng * sizeof(int),
+                                 hipMemcpyDeviceToHost, s));
+    QTN_CHECK_HIP(hipMemcpyAsync(d.data(), e.disc, ng * sizeof(double),
+                                 hipMemcpyDeviceToHost, s));
+    QTN_CHECK_HIP(hipMemcpyAsync(tt.data(), e.total, ng * sizeof(double),
+                                 hipMemcpyDeviceToHost, s));
+    QTN_CHECK_HIP(hipMemcpyAsync(kw.data(), e.keptw, ng * sizeof(double),
+                                 hipMemcpyDeviceToHost, s));
+    QTN_CHECK_HIP(hipStreamSynchronize(s));
+    return QTN_OK;
+  };
+  void *dX32 = nullptr, *dW32 = nullptr;
+  if (chosen) {
+    std::vector<int> cand(ng, 0);
+    bool any32 = false;
+    for (int g = 0; g < ng; ++g) {
+      cand[g] = nsv[g] > chimax ? 1 : 0;
+      any32 = any32 || cand[g];
+    }
+    if (any32) {
+      // the sweeps run on a complex FP32 copy of X (and W)
+      QTN_TRY(ctx->buffer(kSlotX32, (size_t)ng * x_sz * sizeof(float2),
+                          &dX32));
+      if (any_w)
+        QTN_TRY(ctx->buffer(kSlotW32, (size_t)ng * w_sz * sizeof(float2),
+                            &dW32));
+      hipLaunchKernelGGL(to_single, dim3(grid_for((int64_t)ng * x_sz)),
+                         dim3(256), 0, s, (const double2*)dX, (float2*)dX32,
+                         (int64_t)ng * x_sz);
+      QTN_CHECK_HIP(hipGetLastError());
+      QTN_TRY(eng32.reserve(ctx, ng, rows_p, cols_p, true));
+      QTN_TRY(eng32.run32(ctx, (float2*)dX32, rows_p, x_sz, (float2*)dW32,
+                          w_sz, any_w ? wantw.data() : nullptr, nsv.data(),
+                          cutoff, 1, chimax, 60, &sweeps, cand.data()));
+      std::vector<int64_t> k32(ng);
+      std::vector<int> st32(ng);
+      std::vector<double> d32(ng), t32(ng), kw32(ng);
+      QTN_TRY(read_results(eng32, k32, st32, d32, t32, kw32));
+      for (int g = 0; g < ng; ++g) {
+        if (cand[g] && st32[g] == 0 && k32[g] == chimax) {
+          src[g] = 1;
+          need64[g] = 0;
+          kept[g] = k32[g];
+          disc[g] = d32[g];
+          total[g] = t32[g];
+          keptw[g] = kw32[g];
+        }
+      }
+    }
+  }
+  bool any64 = false;
+  for (int g = 0; g < ng; ++g) any64 = any64 || need64[g];
+  if (any64) {
+    // every problem preconditioned: only the top cols_p rows are non-zero
+    int64_t rows_eng = all_pre ? cols_p : rows_p, cols_eng = cols_p;
+    if (chosen) {
+      // the FP64 engine is padded to the gates it solves, not to the batch
+      int64_t r64 = 1, c64 = 1;
+      for (int g = 0; g < ng; ++g) {
+        if (!need64[g]) continue;
+        const int64_t M = 2 * gd[g].chiL, N = 2 * gd[g].chiR;
+        r64 = std::max(r64, trans[g] ? N : M);
+        c64 = std::max(c64, trans[g] ? M : N);
+      }
+      rows_eng = round_up64(r64);
+      cols_eng = round_up64(c64);
+    }
+    eng.ldw = cols_p;  // W keeps the batch layout
+    QTN_TRY(eng.reserve(ctx, ng, rows_eng, cols_eng));
+    QTN_TRY(eng.run(ctx, (double2*)dX, rows_p, x_sz, (double2*)dW, w_sz,
+                    any_w ? wantw.data() : nullptr, nsv.data(), cutoff, 1,
+                    chimax, 60, &sweeps, chosen ? need64.data() : nullptr));
+    std::vector<int64_t> k64(ng);
+    std::vector<int> st64(ng);
+    std::vector<double> d64(ng), t64(ng), kw64(ng);
+    QTN_TRY(read_results(eng, k64, st64, d64, t64, kw64));
+    for (int g = 0; g < ng; ++g) {
+      if (!need64[g]) continue;
+      kept[g] = k64[g];
+      st[g] = st64[g];
+      disc[g] = d64[g];
+      total[g] = t64[g];
+      keptw[g] = kw64[g];
+    }
+  }
   for (int g = 0; g < ng; ++g) {
     if (st[g] != 0) {
       set_error("apply_2q: every singular value is within the truncation "
@@ -425,18 +640,79 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
     }
   }
   if (spec_out && ng == 1) {
-    QTN_CHECK_HIP(hipMemcpy(spec_out, eng.sig, nsv[0] * sizeof(double),
-                            hipMemcpyDeviceToHost));
+    QTN_CHECK_HIP(hipMemcpy(spec_out, src[0] ? eng32.sig : eng.sig,
+                            nsv[0] * sizeof(double), hipMemcpyDeviceToHost));
     if (spec_len) *spec_len = nsv[0];
+  }
+  // per-gate kept rank and source engine on the device (chosen form)
+  int64_t* d_kept = eng.kept;
+  int* d_src = nullptr;
+  if (chosen) {
+    void* pm;
+    QTN_TRY(ctx->buffer(kSlotMix, (size_t)ng * (sizeof(int64_t) + sizeof(int)),
+                        &pm));
+    d_kept = (int64_t*)pm;
+    d_src = (int*)(d_kept + ng);
+    QTN_CHECK_HIP(hipMemcpyAsync(d_kept, kept.data(), ng * sizeof(int64_t),
+                                 hipMemcpyHostToDevice, s));
+    QTN_CHECK_HIP(hipMemcpyAsync(d_src, src.data(), ng * sizeof(int),
+                                 hipMemcpyHostToDevice, s));
   }
   // 4. V_k, then B_{i+1} = V_k^H and lambda_i
   int64_t maxk = 0;
   for (int g = 0; g < ng; ++g) maxk = std::max(maxk, kept[g]);
-  hipLaunchKernelGGL(extract_vk, dim3(grid_for(maxN * maxk), (unsigned)ng),
-                     dim3(256), 0, s, (const GateDesc*)d_gd,
-                     (const double*)eng.sig, (const int*)eng.perm,
-                     (const int64_t*)eng.kept, cols_p);
-  QTN_CHECK_HIP(hipGetLastError());
+  if (chosen) {
+    hipLaunchKernelGGL(extract_vk_mixed,
+                       dim3(grid_for(maxN * maxk), (unsigned)ng), dim3(256),
+                       0, s, (const GateDesc*)d_gd, (const int*)d_src,
+                       (const float2*)dX32, x_sz, (const float2*)dW32, w_sz,
+                       (const double*)eng32.sig, (const int*)eng32.perm,
+                       (const double*)eng.sig, (const int*)eng.perm,
+                       (const int64_t*)d_kept, cols_p, eng32.cols_p,
+                       eng.cols_p);
+    QTN_CHECK_HIP(hipGetLastError());
+    // Newton-Schulz in FP64: V <- V (3 I - V^H V) / 2, three steps take the
+    // FP32 orthogonality error (<~1e-3 on the smallest kept columns) below
+    // FP64 rounding.
+    void *dG, *dT;
+    const int64_t t_sz = maxN * maxk;
+    QTN_TRY(ctx->buffer(kSlotNsG, (size_t)ng * maxk * maxk * sizeof(double2),
+                        &dG));
+    QTN_TRY(ctx->buffer(kSlotNsT, (size_t)ng * t_sz * sizeof(double2), &dT));
+    std::vector<GemmDesc<double>> ga(ng), gb(ng);
+    for (int g = 0; g < ng; ++g) {
+      const int64_t N = 2 * gd[g].chiR, k = kept[g];
+      double2* G = (double2*)dG + (size_t)g * maxk * maxk;
+      ga[g] = GemmDesc<double>{k, k, N, make_double2(1.0, 0.0),
+                               make_double2(0.0, 0.0), gd[g].Vk, N,
+                               gd[g].Vk, N, G, k};
+      gb[g] = GemmDesc<double>{N, k, k, make_double2(1.0, 0.0),
+                               make_double2(0.0, 0.0), gd[g].Vk, N, G, k,
+                               (double2*)dT + (size_t)g * t_sz, N};
+    }
+    GemmDesc<double>* d_ga = d_g2 + ng;
+    GemmDesc<double>* d_gb = d_ga + ng;
+    QTN_CHECK_HIP(hipMemcpyAsync(d_ga, ga.data(),
+                                 ng * sizeof(GemmDesc<double>),
+                                 hipMemcpyHostToDevice, s));
+    QTN_CHECK_HIP(hipMemcpyAsync(d_gb, gb.data(),
+                                 ng * sizeof(GemmDesc<double>),
+                                 hipMemcpyHostToDevice, s));
+    for (int it = 0; it < 3; ++it) {
+      QTN_TRY(launch_zgemm_grouped(s, 2, 0, d_ga, ng, maxk, maxk));
+      QTN_TRY(launch_zgemm_grouped(s, 0, 0, d_gb, ng, maxN, maxk));
+      hipLaunchKernelGGL(ns_combine, dim3(grid_for(maxN * maxk), (unsigned)ng),
+                         dim3(256), 0, s, (const GateDesc*)d_gd,
+                         (const double2*)dT, t_sz, (const int64_t*)d_kept);
+      QTN_CHECK_HIP(hipGetLastError());
+    }
+  } else {
+    hipLaunchKernelGGL(extract_vk, dim3(grid_for(maxN * maxk), (unsigned)ng),
+                       dim3(256), 0, s, (const GateDesc*)d_gd,
+                       (const double*)eng.sig, (const int*)eng.perm,
+                       (const int64_t*)eng.kept, cols_p);
+    QTN_CHECK_HIP(hipGetLastError());
+  }
   if (any_pre) {
     // V_k = Q1 [U_k; 0]
     std::vector<int64_t> ldu(ng), ucols(ng);
@@ -449,8 +725,8 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
   }
   hipLaunchKernelGGL(write_right, dim3(grid_for(maxN * maxk), (unsigned)ng),
                      dim3(256), 0, s, (const GateDesc*)d_gd,
-                     (const double*)eng.sig, (const int64_t*)eng.kept,
-                     (const double*)eng.keptw, cols_p);
+                     (const double*)eng.sig, (const int64_t*)d_kept,
+                     (const double*)eng.keptw, cols_p, chosen ? 0 : 1);
   QTN_CHECK_HIP(hipGetLastError());
   // 5. B_i = Phi V_k / |S_k|
   std::vector<GemmDesc<double>> g2(ng);
@@ -461,7 +737,8 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
     e.M = 2 * d.chiL;
     e.N = kept[g];
     e.K = 2 * d.chiR;
-    e.alpha = make_double2(nrm > 0 ? 1.0 / nrm : 0.0, 0.0);
+    // the chosen form normalises after recomputing |S_k| in FP64
+    e.alpha = make_double2(chosen ? 1.0 : (nrm > 0 ? 1.0 / nrm : 0.0), 0.0);
     e.beta = make_double2(0.0, 0.0);
     e.A = d.phi;
     e.lda = 2 * d.chiL;
@@ -473,6 +750,22 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
   QTN_CHECK_HIP(hipMemcpyAsync(d_g2, g2.data(), ng * sizeof(GemmDesc<double>),
                                hipMemcpyHostToDevice, s));
   QTN_TRY(launch_zgemm_grouped(s, 0, 0, d_g2, ng, maxM, maxk));
+  if (chosen) {
+    // S_k = column norms of diag(lambda_{i-1}) Phi V_k in FP64; B_i and
+    // lambda_i are normalised by |S_k| (overwriting what write_right put in
+    // lambda_i from the FP32 values)
+    void* dS64;
+    QTN_TRY(ctx->buffer(kSlotS64, (size_t)ng * cols_p * sizeof(double),
+                        &dS64));
+    hipLaunchKernelGGL(left_colnorms,
+                       dim3((unsigned)((maxk + 3) / 4), (unsigned)ng),
+                       dim3(256), 0, s, (const GateDesc*)d_gd,
+                       (const int64_t*)d_kept, (double*)dS64, cols_p);
+    hipLaunchKernelGGL(left_finish, dim3(grid_for(maxM * maxk), (unsigned)ng),
+                       dim3(256), 0, s, (const GateDesc*)d_gd,
+                       (const int64_t*)d_kept, (const double*)dS64, cols_p);
+    QTN_CHECK_HIP(hipGetLastError());
+  }
   for (int g = 0; g < ng; ++g) {
     m->dims[sites[g]] = kept[g];
     if (kept_out) kept_out[g] = kept[g];
@@ -673,12 +966,13 @@ int32_t qtn_mps_apply_2q(qtn_mps* m, int32_t site, const double* g,
   clear_error();
   QTN_ARG(m && g, "qtn_mps_apply_2q: NULL argument");
   QTN_ARG(cutoff >= 0.0, "Error must be positive");
-  QTN_ARG(form == QTN_FORM_PLAIN, "qtn_mps_apply_2q: unknown form");
+  QTN_ARG(form == QTN_FORM_PLAIN || form == QTN_FORM_CHOSEN,
+          "qtn_mps_apply_2q: unknown form");
   QTN_ARG(chimax <= m->cap, "qtn_mps_apply_2q: chimax above capacity");
   QTN_CHECK_HIP(hipSetDevice(m->ctx->device));
   const int64_t cm = (chimax > 0) ? chimax : m->cap;
-  return mps_apply_gates(m, 1, &site, g, cutoff, cm, kept, disc_weight,
-                         spectrum, spectrum_len);
+  return mps_apply_gates(m, 1, &site, g, cutoff, cm, form, kept,
+                         disc_weight, spectrum, spectrum_len);
 }
 
 int32_t qtn_mps_apply_layer(qtn_mps* m, int32_t ngates, const int32_t* sites,
@@ -689,12 +983,13 @@ int32_t qtn_mps_apply_layer(qtn_mps* m, int32_t ngates, const int32_t* sites,
   QTN_ARG(m && (ngates == 0 || (sites && gates)),
           "qtn_mps_apply_layer: NULL argument");
   QTN_ARG(cutoff >= 0.0, "Error must be positive");
-  QTN_ARG(form == QTN_FORM_PLAIN, "qtn_mps_apply_layer: unknown form");
+  QTN_ARG(form == QTN_FORM_PLAIN || form == QTN_FORM_CHOSEN,
+          "qtn_mps_apply_layer: unknown form");
   QTN_ARG(chimax <= m->cap, "qtn_mps_apply_layer: chimax above capacity");
   if (ngates == 0) return QTN_OK;
   QTN_CHECK_HIP(hipSetDevice(m->ctx->device));
   const int64_t cm = (chimax > 0) ? chimax : m->cap;
-  return mps_apply_gates(m, ngates, sites, gates, cutoff, cm, kept,
+  return mps_apply_gates(m, ngates, sites, gates, cutoff, cm, form, kept,
                          disc_weight, nullptr, nullptr);
 }
 

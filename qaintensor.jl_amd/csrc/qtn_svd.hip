@@ -33,12 +33,47 @@ namespace qtn {
 namespace {
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
 
-// v_mfma_f64_16x16x4f64: A[i][k] from lane i + 16k, B[k][j] from lane
-// j + 16k, result element e of lane l is D[4*e + l/16][l%16].
-__device__ __forceinline__ d4_t mfma64(double a, double b, d4_t c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
+// Precision traits of the sweep kernels.  Both MFMA forms take A[i][k] from
+// lane i + 16k and B[k][j] from lane j + 16k; the f64 result element e of
+// lane l is D[4*e + l/16][l%16], the f32 one D[4*(l/16) + e][l%16].
+template <typename R>
+struct JT;
+template <>
+struct JT<double> {
+  typedef double2 c_t;
+  typedef d4_t acc_t;
+  static __device__ __forceinline__ acc_t mfma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int e) {
+    return 4 * e + (lane >> 4);
+  }
+  static __device__ __forceinline__ c_t mk(double re, double im) {
+    return make_double2(re, im);
+  }
+  static __device__ __forceinline__ double rsq(double x) { return rsqrt(x); }
+  static __device__ __forceinline__ double sq(double x) { return sqrt(x); }
+  static __device__ __forceinline__ double ab(double x) { return fabs(x); }
+};
+template <>
+struct JT<float> {
+  typedef float2 c_t;
+  typedef f4_t acc_t;
+  static __device__ __forceinline__ acc_t mfma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int e) {
+    return 4 * (lane >> 4) + e;
+  }
+  static __device__ __forceinline__ c_t mk(float re, float im) {
+    return make_float2(re, im);
+  }
+  static __device__ __forceinline__ float rsq(float x) { return rsqrtf(x); }
+  static __device__ __forceinline__ float sq(float x) { return sqrtf(x); }
+  static __device__ __forceinline__ float ab(float x) { return fabsf(x); }
+};
 
 // Round-robin (circle method) position -> block for 2p blocks.
 __device__ __forceinline__ int rr_block(int pos, int round, int nb) {
@@ -57,10 +92,13 @@ __device__ __forceinline__ int64_t pair_col(int c, int I, int J) {
 // 1. Partial Gram of a 64-row chunk of a block pair (W columns).
 // grid (rows_p/64, p, batch); Gp layout [batch][p][chunk][W*W] col-major.
 // Each of the 4 waves computes a (W/2) x (W/2) quadrant.
-template <int W>
+template <typename R, int W>
 __global__ __launch_bounds__(256) void jac_gram(
-    const double2* __restrict__ X, int64_t ldx, int64_t sX, int p,
-    int round, double2* __restrict__ Gp, const int* __restrict__ active) {
+    const typename JT<R>::c_t* __restrict__ X, int64_t ldx, int64_t sX,
+    int p, int round, typename JT<R>::c_t* __restrict__ Gp,
+    const int* __restrict__ active) {
+  using c_t = typename JT<R>::c_t;
+  using acc_t = typename JT<R>::acc_t;
   constexpr int T = W / 32;  // 16x16 MFMA tiles per wave per dimension
   const int bz = blockIdx.z;
   if (!active[bz]) return;
@@ -73,12 +111,12 @@ __global__ __launch_bounds__(256) void jac_gram(
   const int I = min(b1, b2), J = max(b1, b2);
   X += (int64_t)bz * sX;
 
-  __shared__ double Xr[W][65];  // [local col][row]
-  __shared__ double Xi[W][65];
+  __shared__ R Xr[W][65];  // [local col][row]
+  __shared__ R Xi[W][65];
   const int tid = threadIdx.x;
   for (int idx = tid; idx < 64 * W; idx += 256) {
     const int r = idx & 63, c = idx >> 6;
-    const double2 v =
+    const c_t v =
         X[(int64_t)chunk * 64 + r + pair_col<W>(c, I, J) * ldx];
     Xr[c][r] = v.x;
     Xi[c][r] = v.y;
@@ -90,21 +128,21 @@ __global__ __launch_bounds__(256) void jac_gram(
   // and is filled in by jac_inner.
   if (wave == 1) return;
   const int wm = (wave & 1) * (W / 2), wn = (wave >> 1) * (W / 2);
-  d4_t gr[T][T], gi[T][T];
+  acc_t gr[T][T], gi[T][T];
 #pragma unroll
   for (int i = 0; i < T; ++i)
 #pragma unroll
     for (int j = 0; j < T; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        gr[i][j][e] = 0.0;
-        gi[i][j][e] = 0.0;
+        gr[i][j][e] = (R)0;
+        gi[i][j][e] = (R)0;
       }
   // G[c1][c2] = sum_r conj(X[r][c1]) X[r][c2]
 #pragma unroll 4
   for (int ks = 0; ks < 64; ks += 4) {
     const int kr = ks + (lane >> 4);
-    double ar[T], ai[T], br[T], bi[T];
+    R ar[T], ai[T], br[T], bi[T];
 #pragma unroll
     for (int i = 0; i < T; ++i) {
       ar[i] = Xr[wm + i * 16 + (lane & 15)][kr];
@@ -116,13 +154,13 @@ __global__ __launch_bounds__(256) void jac_gram(
     for (int i = 0; i < T; ++i)
 #pragma unroll
       for (int j = 0; j < T; ++j) {
-        gr[i][j] = mfma64(ar[i], br[j], gr[i][j]);
-        gr[i][j] = mfma64(-ai[i], bi[j], gr[i][j]);
-        gi[i][j] = mfma64(ar[i], bi[j], gi[i][j]);
-        gi[i][j] = mfma64(ai[i], br[j], gi[i][j]);
+        gr[i][j] = JT<R>::mfma(ar[i], br[j], gr[i][j]);
+        gr[i][j] = JT<R>::mfma(-ai[i], bi[j], gr[i][j]);
+        gi[i][j] = JT<R>::mfma(ar[i], bi[j], gi[i][j]);
+        gi[i][j] = JT<R>::mfma(ai[i], br[j], gi[i][j]);
       }
   }
-  double2* out =
+  c_t* out =
       Gp + (((int64_t)bz * p + pi) * nchunk + chunk) * (int64_t)(W * W);
 #pragma unroll
   for (int i = 0; i < T; ++i)
@@ -131,89 +169,47 @@ __global__ __launch_bounds__(256) void jac_gram(
       const int col = wn + j * 16 + (lane & 15);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int row = wm + i * 16 + 4 * e + (lane >> 4);
-        out[row + W * col] = make_double2(gr[i][j][e], gi[i][j][e]);
+        const int row = wm + i * 16 + JT<R>::row(lane, e);
+        out[row + W * col] = JT<R>::mk(gr[i][j][e], gi[i][j][e]);
       }
     }
 }
 
-// ---------------------------------------------------------------------------
-// 2. Sum the partial Grams, run inner cyclic Jacobi sweep(s), emit Q.
-// grid (p, batch), 256 threads.  flags[bz*p + pi] = 1 if Q must be applied.
-// Each of the W/2 rotation pairs of a round is owned by 512/W threads, which
-// all derive the rotation from G_pp, G_qq, G_pq themselves; thread t of a
-// pair updates rows / columns t, t+TP, ...  Two barriers per round: after
-// the column update (G J, Q J) and after the row update (J^H G).  Operands
-// are loaded into registers before any store so LDS latencies overlap.
-template <int W>
-__global__ __launch_bounds__(256) void jac_inner(
-    const double2* __restrict__ Gp, int nchunk, int p, double tol,
-    const double* __restrict__ zthr, int n_inner, double2* __restrict__ Qout,
-    int* __restrict__ flags, int* __restrict__ counters,
-    const int* __restrict__ active, double* __restrict__ offmax,
-    int* __restrict__ qpairs) {
+// Check and n_inner cyclic sweeps on the Hermitian Gram block G (LDS),
+// accumulating the rotations in Q (the identity on entry).  Returns the
+// number of rotations (0 when the block already meets the tolerance).
+// Called by all 256 threads; any_off / nrot are zeroed __shared__ ints
+// (visible after a barrier) and offmax_bz the problem's sweep maximum.
+template <typename R, int W>
+__device__ __forceinline__ int inner_solve(
+    typename JT<R>::c_t (*G)[W + 1], typename JT<R>::c_t (*Q)[W + 1], R zt,
+    R tol2, int n_inner, double* offmax_bz, int& any_off, int& nrot) {
+  using c_t = typename JT<R>::c_t;
   constexpr int NP = W / 2;       // rotation pairs per round
   constexpr int TP = 256 / NP;    // threads per pair
   constexpr int RPT = W / TP;     // rows (columns) per thread
-  const int bz = blockIdx.y;
-  const int pi = blockIdx.x;
-  if (!active[bz]) return;
-  __shared__ double2 G[W][W + 1];  // G[row][col]
-  __shared__ double2 Q[W][W + 1];  // Q[row][col]
-  __shared__ int any_off;
-  __shared__ int nrot;
   const int tid = threadIdx.x;
-  const double2* src =
-      Gp + ((int64_t)bz * p + pi) * nchunk * (int64_t)(W * W);
-  for (int idx = tid; idx < W * W; idx += 256) {
-    const int r = idx % W, c = idx / W;
-    Q[r][c] = make_double2(r == c ? 1.0 : 0.0, 0.0);
-    if (r >= W / 2 && c < W / 2) continue;  // lower-left: by symmetry below
-    double sr = 0.0, si = 0.0;
-    for (int ch = 0; ch < nchunk; ++ch) {
-      const double2 v = src[(int64_t)ch * (W * W) + idx];
-      sr += v.x;
-      si += v.y;
-    }
-    G[r][c] = make_double2(sr, si);
-  }
-  if (tid == 0) {
-    any_off = 0;
-    nrot = 0;
-  }
-  const double zt = zthr[bz];
-  const double tol2 = tol * tol;
-  __syncthreads();
-  for (int idx = tid; idx < (W / 2) * (W / 2); idx += 256) {
-    const int r = W / 2 + idx % (W / 2), c = idx / (W / 2);
-    const double2 u = G[c][r];
-    G[r][c] = make_double2(u.x, -u.y);
-  }
-  __syncthreads();
   // Largest scaled off-diagonal |G_pq|^2 / (G_pp G_qq) of the block; nothing
   // to do when it meets the tolerance (the common case in late sweeps).
-  double omax = 0.0;
+  R omax = 0;
   for (int idx = tid; idx < W * W; idx += 256) {
     const int r = idx % W, c = idx / W;
     if (r < c) {
-      const double a = G[r][r].x, d = G[c][c].x;
+      const R a = G[r][r].x, d = G[c][c].x;
       if (a > zt && d > zt) {
-        const double2 g = G[r][c];
-        const double o2 = (g.x * g.x + g.y * g.y) / (a * d);
+        const c_t g = G[r][c];
+        const R o2 = (g.x * g.x + g.y * g.y) / (a * d);
         omax = fmax(omax, o2);
         if (o2 > tol2) atomicOr(&any_off, 1);
       }
     }
   }
   // positive doubles order like their bit patterns
-  if (omax > 0.0)
-    atomicMax((unsigned long long*)&offmax[bz],
-              (unsigned long long)__double_as_longlong(omax));
+  if (omax > (R)0)
+    atomicMax((unsigned long long*)offmax_bz,
+              (unsigned long long)__double_as_longlong((double)omax));
   __syncthreads();
-  if (!any_off) {
-    if (tid == 0) flags[bz * p + pi] = 0;
-    return;
-  }
+  if (!any_off) return 0;
   const int pr = tid / TP;  // rotation pair of this thread
   const int t = tid % TP;
   for (int sweep = 0; sweep < n_inner; ++sweep) {
@@ -222,18 +218,18 @@ __global__ __launch_bounds__(256) void jac_inner(
       const int c1 = rr_block(pr, round, W);
       const int c2 = rr_block(W - 1 - pr, round, W);
       const int pp = min(c1, c2), qq = max(c1, c2);
-      const double a = G[pp][pp].x, d = G[qq][qq].x;
-      const double2 g = G[pp][qq];
-      const double absc2 = g.x * g.x + g.y * g.y;
+      const R a = G[pp][pp].x, d = G[qq][qq].x;
+      const c_t g = G[pp][qq];
+      const R absc2 = g.x * g.x + g.y * g.y;
       const bool on = (a > zt && d > zt && absc2 > tol2 * a * d);
-      double cs = 1.0, sn = 0.0, er = 1.0, ei = 0.0, ap = a, dq = d;
+      R cs = 1, sn = 0, er = 1, ei = 0, ap = a, dq = d;
       if (on) {
-        const double inv = rsqrt(absc2);
-        const double absc = absc2 * inv;
-        const double zeta = 0.5 * (d - a) * inv;
-        const double tt = (zeta >= 0.0 ? 1.0 : -1.0) /
-                          (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-        cs = rsqrt(1.0 + tt * tt);
+        const R inv = JT<R>::rsq(absc2);
+        const R absc = absc2 * inv;
+        const R zeta = (R)0.5 * (d - a) * inv;
+        const R tt = (zeta >= (R)0 ? (R)1 : (R)-1) /
+                     (JT<R>::ab(zeta) + JT<R>::sq((R)1 + zeta * zeta));
+        cs = JT<R>::rsq((R)1 + tt * tt);
         sn = tt * cs;
         er = g.x * inv;  // e = e^{-i phi} = conj(g) / |g|
         ei = -g.y * inv;
@@ -242,7 +238,7 @@ __global__ __launch_bounds__(256) void jac_inner(
         sweep_rot += (t == 0);
         // column update: [x_p x_q] <- [x_p x_q] J,
         // J = [[cs, sn], [-sn e, cs e]]
-        double2 gp[RPT], gq[RPT], qp[RPT], qv[RPT];
+        c_t gp[RPT], gq[RPT], qp[RPT], qv[RPT];
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
           const int r = t + TP * j;
@@ -255,20 +251,20 @@ __global__ __launch_bounds__(256) void jac_inner(
         for (int j = 0; j < RPT; ++j) {
           const int r = t + TP * j;
           {
-            const double yr = er * gq[j].x - ei * gq[j].y;
-            const double yi = er * gq[j].y + ei * gq[j].x;
-            G[r][pp] = make_double2(cs * gp[j].x - sn * yr,
-                                    cs * gp[j].y - sn * yi);
-            G[r][qq] = make_double2(sn * gp[j].x + cs * yr,
-                                    sn * gp[j].y + cs * yi);
+            const R yr = er * gq[j].x - ei * gq[j].y;
+            const R yi = er * gq[j].y + ei * gq[j].x;
+            G[r][pp] = JT<R>::mk(cs * gp[j].x - sn * yr,
+                                 cs * gp[j].y - sn * yi);
+            G[r][qq] = JT<R>::mk(sn * gp[j].x + cs * yr,
+                                 sn * gp[j].y + cs * yi);
           }
           {
-            const double yr = er * qv[j].x - ei * qv[j].y;
-            const double yi = er * qv[j].y + ei * qv[j].x;
-            Q[r][pp] = make_double2(cs * qp[j].x - sn * yr,
-                                    cs * qp[j].y - sn * yi);
-            Q[r][qq] = make_double2(sn * qp[j].x + cs * yr,
-                                    sn * qp[j].y + cs * yi);
+            const R yr = er * qv[j].x - ei * qv[j].y;
+            const R yi = er * qv[j].y + ei * qv[j].x;
+            Q[r][pp] = JT<R>::mk(cs * qp[j].x - sn * yr,
+                                 cs * qp[j].y - sn * yi);
+            Q[r][qq] = JT<R>::mk(sn * qp[j].x + cs * yr,
+                                 sn * qp[j].y + cs * yi);
           }
         }
       }
@@ -276,7 +272,7 @@ __global__ __launch_bounds__(256) void jac_inner(
       if (on) {
         // row update: [r_p; r_q] <- J^H [r_p; r_q],
         // J^H = [[cs, -sn conj(e)], [sn, cs conj(e)]]
-        double2 xp[RPT], xq[RPT];
+        c_t xp[RPT], xq[RPT];
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
           xp[j] = G[pp][t + TP * j];
@@ -285,19 +281,17 @@ __global__ __launch_bounds__(256) void jac_inner(
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
           const int c = t + TP * j;
-          const double yr = er * xq[j].x + ei * xq[j].y;
-          const double yi = er * xq[j].y - ei * xq[j].x;
-          double2 np = make_double2(cs * xp[j].x - sn * yr,
-                                    cs * xp[j].y - sn * yi);
-          double2 nq = make_double2(sn * xp[j].x + cs * yr,
-                                    sn * xp[j].y + cs * yi);
+          const R yr = er * xq[j].x + ei * xq[j].y;
+          const R yi = er * xq[j].y - ei * xq[j].x;
+          c_t np = JT<R>::mk(cs * xp[j].x - sn * yr, cs * xp[j].y - sn * yi);
+          c_t nq = JT<R>::mk(sn * xp[j].x + cs * yr, sn * xp[j].y + cs * yi);
           // the 2x2 block becomes diag(ap, dq) exactly
           if (c == pp) {
-            np = make_double2(ap, 0.0);
-            nq = make_double2(0.0, 0.0);
+            np = JT<R>::mk(ap, (R)0);
+            nq = JT<R>::mk((R)0, (R)0);
           } else if (c == qq) {
-            np = make_double2(0.0, 0.0);
-            nq = make_double2(dq, 0.0);
+            np = JT<R>::mk((R)0, (R)0);
+            nq = JT<R>::mk(dq, (R)0);
           }
           G[pp][c] = np;
           G[qq][c] = nq;
@@ -317,7 +311,98 @@ __global__ __launch_bounds__(256) void jac_inner(
     __syncthreads();
   }
   __syncthreads();
-  const int total = any_off - 1;
+  return any_off - 1;
+}
+
+// ---------------------------------------------------------------------------
+// 2. Sum the partial Grams, run inner cyclic Jacobi sweep(s), emit Q.
+// grid (p, batch), 256 threads.  flags[bz*p + pi] = 1 if Q must be applied.
+// Each of the W/2 rotation pairs of a round is owned by 512/W threads, which
+// all derive the rotation from G_pp, G_qq, G_pq themselves; thread t of a
+// pair updates rows / columns t, t+TP, ...  Two barriers per round: after
+// the column update (G J, Q J) and after the row update (J^H G).  Operands
+// are loaded into registers before any store so LDS latencies overlap.
+template <typename R, int W>
+__global__ __launch_bounds__(256) void jac_inner(
+    const typename JT<R>::c_t* __restrict__ Gp, int nchunk, int p,
+    double tol, const double* __restrict__ zthr, int n_inner,
+    typename JT<R>::c_t* __restrict__ Qout, int* __restrict__ flags,
+    int* __restrict__ counters, const int* __restrict__ active,
+    double* __restrict__ offmax, int* __restrict__ qpairs) {
+  using c_t = typename JT<R>::c_t;
+  constexpr int NP = W / 2;       // rotation pairs per round
+  constexpr int TP = 256 / NP;    // threads per pair
+  constexpr int RPT = W / TP;     // rows (columns) per thread
+  const int bz = blockIdx.y;
+  const int pi = blockIdx.x;
+  if (!active[bz]) return;
+  __shared__ c_t G[W][W + 1];  // G[row][col]
+  __shared__ c_t Q[W][W + 1];  // Q[row][col]
+  __shared__ int any_off;
+  __shared__ int nrot;
+  const int tid = threadIdx.x;
+  const c_t* src =
+      Gp + ((int64_t)bz * p + pi) * nchunk * (int64_t)(W * W);
+  // Sum the partial Grams chunk by chunk (in chunk order), with CG chunks'
+  // loads in flight per thread; the lower-left quadrant is filled by
+  // symmetry below.
+  constexpr int IPT = W * W / 256;       // entries per thread
+  constexpr int CG = IPT >= 16 ? 1 : 16 / IPT;
+  R sr[IPT], si[IPT];
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    sr[i] = 0;
+    si[i] = 0;
+  }
+  for (int ch0 = 0; ch0 < nchunk; ch0 += CG) {
+    c_t v[CG][IPT];
+#pragma unroll
+    for (int u = 0; u < CG; ++u)
+#pragma unroll
+      for (int i = 0; i < IPT; ++i) {
+        const int idx = tid + 256 * i;
+        const int r = idx % W, c = idx / W;
+        const bool ll = r >= W / 2 && c < W / 2;
+        v[u][i] = (ch0 + u < nchunk && !ll)
+                      ? src[(int64_t)(ch0 + u) * (W * W) + idx]
+                      : JT<R>::mk((R)0, (R)0);
+      }
+#pragma unroll
+    for (int u = 0; u < CG; ++u)
+#pragma unroll
+      for (int i = 0; i < IPT; ++i) {
+        if (ch0 + u < nchunk) {
+          sr[i] += v[u][i].x;
+          si[i] += v[u][i].y;
+        }
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    const int idx = tid + 256 * i;
+    const int r = idx % W, c = idx / W;
+    Q[r][c] = JT<R>::mk(r == c ? (R)1 : (R)0, (R)0);
+    if (!(r >= W / 2 && c < W / 2)) G[r][c] = JT<R>::mk(sr[i], si[i]);
+  }
+  if (tid == 0) {
+    any_off = 0;
+    nrot = 0;
+  }
+  const R zt = (R)zthr[bz];
+  const R tol2 = (R)(tol * tol);
+  __syncthreads();
+  for (int idx = tid; idx < (W / 2) * (W / 2); idx += 256) {
+    const int r = W / 2 + idx % (W / 2), c = idx / (W / 2);
+    const c_t u = G[c][r];
+    G[r][c] = JT<R>::mk(u.x, -u.y);
+  }
+  __syncthreads();
+  const int total = inner_solve<R, W>(G, Q, zt, tol2, n_inner, &offmax[bz],
+                                      any_off, nrot);
+  if (total == 0) {
+    if (tid == 0) flags[bz * p + pi] = 0;
+    return;
+  }
   if (tid == 0) {
     flags[bz * p + pi] = total > 0 ? 1 : 0;
     if (total > 0) {
@@ -326,7 +411,7 @@ __global__ __launch_bounds__(256) void jac_inner(
     }
   }
   if (total > 0) {
-    double2* q = Qout + ((int64_t)bz * p + pi) * (int64_t)(W * W);
+    c_t* q = Qout + ((int64_t)bz * p + pi) * (int64_t)(W * W);
     for (int idx = tid; idx < W * W; idx += 256) {
       const int r = idx % W, c = idx / W;
       q[idx] = Q[r][c];
@@ -336,11 +421,14 @@ __global__ __launch_bounds__(256) void jac_inner(
 
 // ---------------------------------------------------------------------------
 // 3. X_IJ <- X_IJ Q for one 64-row chunk.  grid (rows/64, p, batch).
-template <int W>
+template <typename R, int W>
 __global__ __launch_bounds__(256) void jac_update(
-    double2* __restrict__ X, int64_t ldx, int64_t sX, int p, int round,
-    const double2* __restrict__ Q, const int* __restrict__ flags,
-    const int* __restrict__ active, const int* __restrict__ want) {
+    typename JT<R>::c_t* __restrict__ X, int64_t ldx, int64_t sX, int p,
+    int round, const typename JT<R>::c_t* __restrict__ Q,
+    const int* __restrict__ flags, const int* __restrict__ active,
+    const int* __restrict__ want) {
+  using c_t = typename JT<R>::c_t;
+  using acc_t = typename JT<R>::acc_t;
   constexpr int T = W / 32;
   const int bz = blockIdx.z;
   if (!active[bz]) return;
@@ -353,22 +441,22 @@ __global__ __launch_bounds__(256) void jac_update(
   const int b2 = rr_block(nb - 1 - pi, round, nb);
   const int I = min(b1, b2), J = max(b1, b2);
   X += (int64_t)bz * sX;
-  const double2* Qp = Q + ((int64_t)bz * p + pi) * (int64_t)(W * W);
+  const c_t* Qp = Q + ((int64_t)bz * p + pi) * (int64_t)(W * W);
 
-  __shared__ double Xr[W][65];  // [local col][row]
-  __shared__ double Xi[W][65];
-  __shared__ double Qr[W][W + 1];  // [k][n]
-  __shared__ double Qi[W][W + 1];
+  __shared__ R Xr[W][65];  // [local col][row]
+  __shared__ R Xi[W][65];
+  __shared__ R Qr[W][W + 1];  // [k][n]
+  __shared__ R Qi[W][W + 1];
   const int tid = threadIdx.x;
   for (int idx = tid; idx < 64 * W; idx += 256) {
     const int r = idx & 63, c = idx >> 6;
-    const double2 v = X[(int64_t)chunk * 64 + r + pair_col<W>(c, I, J) * ldx];
+    const c_t v = X[(int64_t)chunk * 64 + r + pair_col<W>(c, I, J) * ldx];
     Xr[c][r] = v.x;
     Xi[c][r] = v.y;
   }
   for (int idx = tid; idx < W * W; idx += 256) {
     const int r = idx % W, c = idx / W;
-    const double2 qv = Qp[idx];  // Q[r][c] col-major
+    const c_t qv = Qp[idx];  // Q[r][c] col-major
     Qr[r][c] = qv.x;
     Qi[r][c] = qv.y;
   }
@@ -376,21 +464,21 @@ __global__ __launch_bounds__(256) void jac_update(
   const int lane = tid & 63, wave = tid >> 6;
   // output 64 rows x W cols; wave tile (32) x (W/2)
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * (W / 2);
-  d4_t yr[2][T], yi[2][T];
+  acc_t yr[2][T], yi[2][T];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < T; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        yr[i][j][e] = 0.0;
-        yi[i][j][e] = 0.0;
+        yr[i][j][e] = (R)0;
+        yi[i][j][e] = (R)0;
       }
   // Y[row][n] = sum_k X[row][k] Q[k][n]
 #pragma unroll 4
   for (int ks = 0; ks < W; ks += 4) {
     const int kr = ks + (lane >> 4);
-    double ar[2], ai[2], br[T], bi[T];
+    R ar[2], ai[2], br[T], bi[T];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       ar[i] = Xr[kr][wm + i * 16 + (lane & 15)];
@@ -405,10 +493,10 @@ __global__ __launch_bounds__(256) void jac_update(
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int j = 0; j < T; ++j) {
-        yr[i][j] = mfma64(ar[i], br[j], yr[i][j]);
-        yr[i][j] = mfma64(-ai[i], bi[j], yr[i][j]);
-        yi[i][j] = mfma64(ar[i], bi[j], yi[i][j]);
-        yi[i][j] = mfma64(ai[i], br[j], yi[i][j]);
+        yr[i][j] = JT<R>::mfma(ar[i], br[j], yr[i][j]);
+        yr[i][j] = JT<R>::mfma(-ai[i], bi[j], yr[i][j]);
+        yi[i][j] = JT<R>::mfma(ar[i], bi[j], yi[i][j]);
+        yi[i][j] = JT<R>::mfma(ai[i], br[j], yi[i][j]);
       }
   }
 #pragma unroll
@@ -419,11 +507,245 @@ __global__ __launch_bounds__(256) void jac_update(
       const int64_t gc = pair_col<W>(c, I, J);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int r = wm + i * 16 + 4 * e + (lane >> 4);
+        const int r = wm + i * 16 + JT<R>::row(lane, e);
         X[(int64_t)chunk * 64 + r + gc * ldx] =
-            make_double2(yr[i][j][e], yi[i][j][e]);
+            JT<R>::mk(yr[i][j][e], yi[i][j][e]);
       }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Fused round: one workgroup per block pair does all three steps of the
+// pair -- G = X_IJ^H X_IJ on MFMA streaming X_IJ through LDS in 64-row
+// chunks (register-prefetched one chunk ahead), the inner sweeps on G in
+// LDS, then X_IJ <- X_IJ Q (and W_IJ <- W_IJ Q when wanted) on MFMA chunk
+// by chunk in place.  One launch per round, no partial Grams in memory.
+// grid (p, batch), 256 threads.  W = 32 (blocks of 16 columns).
+template <typename R, int W>
+__device__ __forceinline__ void load_tile(
+    const typename JT<R>::c_t* __restrict__ X, int64_t ldx, int chunk, int I,
+    int J, typename JT<R>::c_t (&v)[64 * W / 256]) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < 64 * W / 256; ++u) {
+    const int idx = tid + 256 * u;
+    const int r = idx & 63, c = idx >> 6;
+    v[u] = X[(int64_t)chunk * 64 + r + pair_col<W>(c, I, J) * ldx];
+  }
+}
+
+template <typename R, int W>
+__device__ __forceinline__ void store_tile_lds(
+    const typename JT<R>::c_t (&v)[64 * W / 256], R (*Xr)[65], R (*Xi)[65]) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int u = 0; u < 64 * W / 256; ++u) {
+    const int idx = tid + 256 * u;
+    const int r = idx & 63, c = idx >> 6;
+    Xr[c][r] = v[u].x;
+    Xi[c][r] = v[u].y;
+  }
+}
+
+// Y = X_IJ Q over every 64-row chunk of one matrix, in place.
+template <typename R, int W>
+__device__ __forceinline__ void apply_q_chunks(
+    typename JT<R>::c_t* __restrict__ X, int64_t ldx, int nchunk, int I,
+    int J, R (*Xr)[65], R (*Xi)[65], R (*Qr)[W + 1], R (*Qi)[W + 1]) {
+  using c_t = typename JT<R>::c_t;
+  using acc_t = typename JT<R>::acc_t;
+  constexpr int T = W / 32;
+  constexpr int EPT = 64 * W / 256;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * (W / 2);
+  c_t nxt[EPT];
+  load_tile<R, W>(X, ldx, 0, I, J, nxt);
+  for (int ch = 0; ch < nchunk; ++ch) {
+    __syncthreads();  // the previous chunk's MFMA reads of Xr / Xi are done
+    store_tile_lds<R, W>(nxt, Xr, Xi);
+    __syncthreads();
+    if (ch + 1 < nchunk) load_tile<R, W>(X, ldx, ch + 1, I, J, nxt);
+    acc_t yr[2][T], yi[2][T];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < T; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          yr[i][j][e] = (R)0;
+          yi[i][j][e] = (R)0;
+        }
+#pragma unroll 4
+    for (int ks = 0; ks < W; ks += 4) {
+      const int kr = ks + (lane >> 4);
+      R ar[2], ai[2], br[T], bi[T];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ar[i] = Xr[kr][wm + i * 16 + (lane & 15)];
+        ai[i] = Xi[kr][wm + i * 16 + (lane & 15)];
+      }
+#pragma unroll
+      for (int j = 0; j < T; ++j) {
+        br[j] = Qr[kr][wn + j * 16 + (lane & 15)];
+        bi[j] = Qi[kr][wn + j * 16 + (lane & 15)];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < T; ++j) {
+          yr[i][j] = JT<R>::mfma(ar[i], br[j], yr[i][j]);
+          yr[i][j] = JT<R>::mfma(-ai[i], bi[j], yr[i][j]);
+          yi[i][j] = JT<R>::mfma(ar[i], bi[j], yi[i][j]);
+          yi[i][j] = JT<R>::mfma(ai[i], br[j], yi[i][j]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < T; ++j) {
+        const int c = wn + j * 16 + (lane & 15);
+        const int64_t gc = pair_col<W>(c, I, J);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int r = wm + i * 16 + JT<R>::row(lane, e);
+          X[(int64_t)ch * 64 + r + gc * ldx] =
+              JT<R>::mk(yr[i][j][e], yi[i][j][e]);
+        }
+      }
+  }
+}
+
+template <typename R, int W>
+__global__ __launch_bounds__(256) void jac_round(
+    typename JT<R>::c_t* __restrict__ X, int64_t ldx, int64_t sX,
+    int nchunk_x, typename JT<R>::c_t* __restrict__ Wm, int64_t ldw,
+    int64_t sW, int nchunk_w, const int* __restrict__ want, int p, int round,
+    double tol, const double* __restrict__ zthr, int n_inner,
+    int* __restrict__ counters, const int* __restrict__ active,
+    double* __restrict__ offmax, int* __restrict__ qpairs) {
+  using c_t = typename JT<R>::c_t;
+  using acc_t = typename JT<R>::acc_t;
+  constexpr int T = W / 32;
+  constexpr int EPT = 64 * W / 256;
+  const int bz = blockIdx.y;
+  const int pi = blockIdx.x;
+  if (!active[bz]) return;
+  const int nb = 2 * p;
+  const int b1 = rr_block(pi, round, nb);
+  const int b2 = rr_block(nb - 1 - pi, round, nb);
+  const int I = min(b1, b2), J = max(b1, b2);
+  X += (int64_t)bz * sX;
+
+  __shared__ R Xr[W][65];  // [local col][row] of the current chunk
+  __shared__ R Xi[W][65];
+  __shared__ c_t G[W][W + 1];
+  __shared__ c_t Q[W][W + 1];
+  // [k][n] planes of Q for the MFMA update live in G's storage (G is dead
+  // once the inner sweeps are done), so two workgroups fit in one CU's LDS
+  R (*Qr)[W + 1] = reinterpret_cast<R (*)[W + 1]>(&G[0][0]);
+  R (*Qi)[W + 1] = Qr + W;
+  __shared__ int any_off;
+  __shared__ int nrot;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+
+  // 1. Gram: waves 0, 2, 3 accumulate the upper-left, lower-right and
+  // upper-right (W/2)^2 quadrants over all chunks.
+  const int wm = (wave & 1) * (W / 2), wn = (wave >> 1) * (W / 2);
+  acc_t gr[T][T], gi[T][T];
+#pragma unroll
+  for (int i = 0; i < T; ++i)
+#pragma unroll
+    for (int j = 0; j < T; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        gr[i][j][e] = (R)0;
+        gi[i][j][e] = (R)0;
+      }
+  {
+    c_t nxt[EPT];
+    load_tile<R, W>(X, ldx, 0, I, J, nxt);
+    for (int ch = 0; ch < nchunk_x; ++ch) {
+      __syncthreads();
+      store_tile_lds<R, W>(nxt, Xr, Xi);
+      __syncthreads();
+      if (ch + 1 < nchunk_x) load_tile<R, W>(X, ldx, ch + 1, I, J, nxt);
+      if (wave != 1) {
+#pragma unroll 4
+        for (int ks = 0; ks < 64; ks += 4) {
+          const int kr = ks + (lane >> 4);
+          R ar[T], ai[T], br[T], bi[T];
+#pragma unroll
+          for (int i = 0; i < T; ++i) {
+            ar[i] = Xr[wm + i * 16 + (lane & 15)][kr];
+            ai[i] = -Xi[wm + i * 16 + (lane & 15)][kr];
+            br[i] = Xr[wn + i * 16 + (lane & 15)][kr];
+            bi[i] = Xi[wn + i * 16 + (lane & 15)][kr];
+          }
+#pragma unroll
+          for (int i = 0; i < T; ++i)
+#pragma unroll
+            for (int j = 0; j < T; ++j) {
+              gr[i][j] = JT<R>::mfma(ar[i], br[j], gr[i][j]);
+              gr[i][j] = JT<R>::mfma(-ai[i], bi[j], gr[i][j]);
+              gi[i][j] = JT<R>::mfma(ar[i], bi[j], gi[i][j]);
+              gi[i][j] = JT<R>::mfma(ai[i], br[j], gi[i][j]);
+            }
+        }
+      }
+    }
+  }
+  if (wave != 1) {
+#pragma unroll
+    for (int i = 0; i < T; ++i)
+#pragma unroll
+      for (int j = 0; j < T; ++j) {
+        const int col = wn + j * 16 + (lane & 15);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int row = wm + i * 16 + JT<R>::row(lane, e);
+          G[row][col] = JT<R>::mk(gr[i][j][e], gi[i][j][e]);
+        }
+      }
+  }
+  for (int idx = tid; idx < W * W; idx += 256) {
+    const int r = idx % W, c = idx / W;
+    Q[r][c] = JT<R>::mk(r == c ? (R)1 : (R)0, (R)0);
+  }
+  if (tid == 0) {
+    any_off = 0;
+    nrot = 0;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < (W / 2) * (W / 2); idx += 256) {
+    const int r = W / 2 + idx % (W / 2), c = idx / (W / 2);
+    const c_t u = G[c][r];
+    G[r][c] = JT<R>::mk(u.x, -u.y);
+  }
+  __syncthreads();
+
+  // 2. inner sweeps
+  const R zt = (R)zthr[bz];
+  const R tol2 = (R)(tol * tol);
+  const int total = inner_solve<R, W>(G, Q, zt, tol2, n_inner, &offmax[bz],
+                                      any_off, nrot);
+  if (total == 0) return;
+  if (tid == 0) {
+    atomicAdd(&counters[bz], total);
+    atomicAdd(qpairs, 1);
+  }
+  // 3. X_IJ <- X_IJ Q (and W_IJ <- W_IJ Q)
+  for (int idx = tid; idx < W * W; idx += 256) {
+    const int r = idx % W, c = idx / W;
+    const c_t q = Q[r][c];
+    Qr[r][c] = q.x;
+    Qi[r][c] = q.y;
+  }
+  apply_q_chunks<R, W>(X, ldx, nchunk_x, I, J, Xr, Xi, Qr, Qi);
+  if (Wm && (!want || want[bz]))
+    apply_q_chunks<R, W>(Wm + (int64_t)bz * sW, ldw, nchunk_w, I, J, Xr, Xi,
+                         Qr, Qi);
 }
 
 // After a sweep: a problem stays active only if it rotated something and
@@ -451,20 +773,31 @@ __global__ void jac_sweep_end(int* __restrict__ active,
 
 // Frobenius norm^2 per problem, finiteness check, zero-column threshold.
 // grid (batch), 256 threads.
+template <typename R>
 __global__ __launch_bounds__(256) void jac_prepare(
-    const double2* __restrict__ X, int64_t rows, int64_t ldx, int64_t cols,
+    const typename JT<R>::c_t* __restrict__ X, int64_t rows, int64_t ldx,
+    int64_t cols,
     int64_t sX, double zrel,
     double* __restrict__ normF2, double* __restrict__ zthr,
     int* __restrict__ active, int* __restrict__ counters,
-    int* __restrict__ bad) {
+    int* __restrict__ bad, const int* __restrict__ mask) {
   const int bz = blockIdx.x;
+  if (mask && !mask[bz]) {
+    if (threadIdx.x == 0) {
+      normF2[bz] = 0.0;
+      zthr[bz] = 0.0;
+      active[bz] = 0;
+      counters[bz] = 0;
+    }
+    return;
+  }
   X += (int64_t)bz * sX;
   double s = 0.0;
   int nonfinite = 0;
   for (int64_t i = threadIdx.x; i < rows * cols; i += 256) {
-    const double2 v = X[(i % rows) + (i / rows) * ldx];
+    const typename JT<R>::c_t v = X[(i % rows) + (i / rows) * ldx];
     if (!isfinite(v.x) || !isfinite(v.y)) nonfinite = 1;
-    s += v.x * v.x + v.y * v.y;
+    s += (double)v.x * v.x + (double)v.y * v.y;
   }
   __shared__ double red[256];
   __shared__ int nf;
@@ -489,20 +822,24 @@ __global__ __launch_bounds__(256) void jac_prepare(
 }
 
 // Column norms: one wave per column.  grid (cols/4, batch), 256 threads.
+template <typename R>
 __global__ __launch_bounds__(256) void jac_colnorms(
-    const double2* __restrict__ X, int64_t rows, int64_t ldx, int64_t cols,
-    int64_t sX, double* __restrict__ sig) {
+    const typename JT<R>::c_t* __restrict__ X, int64_t rows, int64_t ldx,
+    int64_t cols, int64_t sX, double* __restrict__ sig,
+    const double* __restrict__ floor2) {
   const int bz = blockIdx.y;
   const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (c >= cols) return;
-  const double2* x = X + (int64_t)bz * sX + c * ldx;
+  const typename JT<R>::c_t* x = X + (int64_t)bz * sX + c * ldx;
   double s = 0.0;
   for (int64_t r = lane; r < rows; r += 64) {
-    const double2 v = x[r];
-    s += v.x * v.x + v.y * v.y;
+    const typename JT<R>::c_t v = x[r];
+    s += (double)v.x * v.x + (double)v.y * v.y;
   }
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  // columns the sweeps left alone (below the zero threshold) count as zero
+  if (floor2 && s <= floor2[bz]) s = 0.0;
   if (lane == 0) sig[(int64_t)bz * cols + c] = sqrt(s);
 }
 
@@ -596,8 +933,16 @@ __global__ __launch_bounds__(1024) void jac_sort_trunc(
 
 // ---------------------------------------------------------------------------
 int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
-                              int64_t cols_p_) {
+                              int64_t cols_p_, bool single_) {
   batch = batch_;
+  single = single_;
+  // The fused round (one workgroup per block pair) leaves each 64-row
+  // chunk's load latency exposed with only 1-2 workgroups per CU; the
+  // three-launch round overlaps them across all chunks and is faster at
+  // chi = 256, so the fused round is opt-in (QTN_JACOBI_FUSED=1).
+  fused = false;
+  if (const char* e = getenv("QTN_JACOBI_FUSED")) fused = atoi(e) != 0;
+  const size_t csz = single ? sizeof(float2) : sizeof(double2);
   rows_p = rows_p_;
   cols_p = cols_p_;
   // Block-pair width: one 64-column pair solved entirely in LDS for narrow
@@ -609,19 +954,17 @@ int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
   p = (int)(cols_p / W);
   const int64_t nchunk = std::max(rows_p, cols_p) / 64;
   void* ptr;
-  QTN_TRY(ctx->buffer(kSlotGram,
-                      (size_t)batch * p * nchunk * W * W * sizeof(double2),
-                      &ptr));
-  Gp = (double2*)ptr;
-  QTN_TRY(ctx->buffer(kSlotQ, (size_t)batch * p * W * W * sizeof(double2),
-                      &ptr));
-  Q = (double2*)ptr;
+  QTN_TRY(ctx->buffer(slot_base + 0,
+                      (size_t)batch * p * nchunk * W * W * csz, &ptr));
+  Gp = ptr;
+  QTN_TRY(ctx->buffer(slot_base + 1, (size_t)batch * p * W * W * csz, &ptr));
+  Q = ptr;
   // small per-problem arrays packed in one slot
   const size_t small =
-      (size_t)batch * (p * sizeof(int) + 5 * sizeof(int) + 6 * sizeof(double) +
+      (size_t)batch * (p * sizeof(int) + 6 * sizeof(int) + 6 * sizeof(double) +
                        2 * sizeof(int64_t)) +
       1024;
-  QTN_TRY(ctx->buffer(kSlotSmall, small, &ptr));
+  QTN_TRY(ctx->buffer(slot_base + 2, small, &ptr));
   char* c = (char*)ptr;
   normF2 = (double*)c;
   c += batch * sizeof(double);
@@ -649,20 +992,28 @@ int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
   c += batch * sizeof(int);
   want_w = (int*)c;
   c += batch * sizeof(int);
+  mask = (int*)c;
+  c += batch * sizeof(int);
   misc = (int*)c;  // [0] = n_active, [1] = nonfinite flag
-  QTN_TRY(ctx->buffer(kSlotSig, (size_t)batch * cols_p * sizeof(double),
+  QTN_TRY(ctx->buffer(slot_base + 3, (size_t)batch * cols_p * sizeof(double),
                       &ptr));
   sig = (double*)ptr;
-  QTN_TRY(ctx->buffer(kSlotPerm, (size_t)batch * cols_p * sizeof(int), &ptr));
+  QTN_TRY(ctx->buffer(slot_base + 4, (size_t)batch * cols_p * sizeof(int),
+                      &ptr));
   perm = (int*)ptr;
   return QTN_OK;
 }
 
-template <int WW>
-int32_t JacobiEngine::launch_round(hipStream_t s, double2* X, int64_t ldx,
-                                   int64_t sX, double2* Wm, int64_t sW, int r,
+template <typename R, int WW>
+int32_t JacobiEngine::launch_round(hipStream_t s, void* Xv, int64_t ldx,
+                                   int64_t sX, void* Wv, int64_t sW, int r,
                                    double tol, int n_inner, int nchunk_x,
                                    int nchunk_w, bool per_problem_w) {
+  using c_t = typename JT<R>::c_t;
+  c_t* X = (c_t*)Xv;
+  c_t* Wm = (c_t*)Wv;
+  c_t* G = (c_t*)Gp;
+  c_t* Qc = (c_t*)Q;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   if (ctx_ && ctx_->prof) {
     for (int i = 0; i < 4; ++i) {
@@ -677,20 +1028,33 @@ int32_t JacobiEngine::launch_round(hipStream_t s, double2* X, int64_t ldx,
     }
   }
   if (ev[0]) QTN_CHECK_HIP(hipEventRecord(ev[0], s));
-  hipLaunchKernelGGL(jac_gram<WW>, dim3(nchunk_x, p, batch), dim3(256), 0, s,
-                     (const double2*)X, ldx, sX, p, r, Gp, active);
+  if (fused && WW == 32) {
+    hipLaunchKernelGGL((jac_round<R, 32>), dim3(p, batch), dim3(256), 0, s, X,
+                       ldx, sX, nchunk_x, Wm, ldw ? ldw : cols_p, sW, nchunk_w,
+                       (const int*)(per_problem_w ? want_w : nullptr), p, r,
+                       tol, zthr, n_inner, counters, active, offmax,
+                       misc + 2);
+    if (ev[1]) QTN_CHECK_HIP(hipEventRecord(ev[1], s));
+    if (ev[2]) QTN_CHECK_HIP(hipEventRecord(ev[2], s));
+    if (ev[3]) QTN_CHECK_HIP(hipEventRecord(ev[3], s));
+    return QTN_OK;
+  }
+  hipLaunchKernelGGL((jac_gram<R, WW>), dim3(nchunk_x, p, batch), dim3(256),
+                     0, s, (const c_t*)X, ldx, sX, p, r, G, active);
   if (ev[1]) QTN_CHECK_HIP(hipEventRecord(ev[1], s));
-  hipLaunchKernelGGL(jac_inner<WW>, dim3(p, batch), dim3(256), 0, s,
-                     (const double2*)Gp, nchunk_x, p, tol, zthr, n_inner, Q,
+  hipLaunchKernelGGL((jac_inner<R, WW>), dim3(p, batch), dim3(256), 0, s,
+                     (const c_t*)G, nchunk_x, p, tol, zthr, n_inner, Qc,
                      flags, counters, active, offmax, misc + 2);
   if (ev[2]) QTN_CHECK_HIP(hipEventRecord(ev[2], s));
-  hipLaunchKernelGGL(jac_update<WW>, dim3(nchunk_x, p, batch), dim3(256), 0, s,
-                     X, ldx, sX, p, r, (const double2*)Q, flags, active,
-                     (const int*)nullptr);
+  hipLaunchKernelGGL((jac_update<R, WW>), dim3(nchunk_x, p, batch),
+                     dim3(256), 0, s, X, ldx, sX, p, r, (const c_t*)Qc, flags,
+                     active, (const int*)nullptr);
   if (Wm)
-    hipLaunchKernelGGL(jac_update<WW>, dim3(nchunk_w, p, batch), dim3(256), 0,
-                       s, Wm, cols_p, sW, p, r, (const double2*)Q, flags,
-                       active, (const int*)(per_problem_w ? want_w : nullptr));
+    hipLaunchKernelGGL((jac_update<R, WW>), dim3(nchunk_w, p, batch),
+                       dim3(256), 0, s, Wm, ldw ? ldw : cols_p, sW, p, r,
+                       (const c_t*)Qc,
+                       flags, active,
+                       (const int*)(per_problem_w ? want_w : nullptr));
   if (ev[3]) QTN_CHECK_HIP(hipEventRecord(ev[3], s));
   return QTN_OK;
 }
@@ -704,7 +1068,28 @@ int32_t JacobiEngine::launch_round(hipStream_t s, double2* X, int64_t ldx,
 //              tile read and written + Q read
 int32_t JacobiEngine::account_sweep(int n_active, int qpairs, int nchunk) {
   qtn_ctx* ctx = ctx_;
-  const double Wd = W, c16 = 16.0;
+  if (fused && W == 32) {
+    // jac_round (id 3): the Gram of every active pair plus X_IJ <- X_IJ Q
+    // for every rotated pair; bytes: X_IJ read once for the Gram, read and
+    // written once more for the update
+    const double Wd = W, c16 = single ? 8.0 : 16.0;
+    const double rows = (double)nchunk * 64.0;
+    for (size_t i = 0; i + 3 < prof_events.size(); i += 4) {
+      float t01;
+      QTN_CHECK_HIP(hipEventElapsedTime(&t01, prof_events[i],
+                                        prof_events[i + 1]));
+      ctx->prof_count[3] += 1;
+      ctx->prof_ms[3] += t01;
+      ctx->prof_flops[3] += (double)n_active * p * 0.75 * 8.0 * rows * Wd * Wd;
+      ctx->prof_bytes[3] += (double)n_active * p * rows * Wd * c16;
+    }
+    ctx->prof_flops[3] += (double)qpairs * 8.0 * rows * Wd * Wd;
+    ctx->prof_bytes[3] += (double)qpairs * 2.0 * rows * Wd * c16;
+    for (auto e : prof_events) ctx->ev_pool.push_back(e);
+    prof_events.clear();
+    return QTN_OK;
+  }
+  const double Wd = W, c16 = single ? 8.0 : 16.0;
   const double rounds = 2.0 * p - 1.0;
   const double gram_f = 0.75 * 8.0 * 64.0 * Wd * Wd;
   const double gram_b = (64.0 * Wd + 0.75 * Wd * Wd) * c16;
@@ -740,9 +1125,47 @@ int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t ldx, int64_t sX,
                           double2* Wm, int64_t sW, const int* want_w_host,
                           const int64_t* nsv_host, double cutoff,
                           int relative, int64_t chimax, int max_sweeps,
-                          int* sweeps_done) {
+                          int* sweeps_done, const int* mask_host) {
+  if (single) {
+    set_error("svd: engine reserved for FP32 sweeps");
+    return QTN_ERR_ARG;
+  }
+  return run_t<double>(ctx, X, ldx, sX, Wm, sW, want_w_host, nsv_host, cutoff,
+                       relative, chimax, max_sweeps, sweeps_done, mask_host);
+}
+
+int32_t JacobiEngine::run32(qtn_ctx* ctx, float2* X, int64_t ldx, int64_t sX,
+                            float2* Wm, int64_t sW, const int* want_w_host,
+                            const int64_t* nsv_host, double cutoff,
+                            int relative, int64_t chimax, int max_sweeps,
+                            int* sweeps_done, const int* mask_host) {
+  if (!single) {
+    set_error("svd: engine reserved for FP64 sweeps");
+    return QTN_ERR_ARG;
+  }
+  return run_t<float>(ctx, X, ldx, sX, Wm, sW, want_w_host, nsv_host, cutoff,
+                      relative, chimax, max_sweeps, sweeps_done, mask_host);
+}
+
+// FP64: tol = sqrt(rows) eps / 2 and zero columns below rows eps |X|_F.
+// FP32 (the chosen form's sweeps): tol = sqrt(rows) eps32 and columns below
+// 8 sqrt(rows) eps32 |X|_F (the rounding noise a column gathers over the
+// sweeps) are left alone.
+template <typename R>
+int32_t JacobiEngine::run_t(qtn_ctx* ctx, typename CType<R>::type* X,
+                            int64_t ldx, int64_t sX,
+                            typename CType<R>::type* Wm, int64_t sW,
+                            const int* want_w_host, const int64_t* nsv_host,
+                            double cutoff, int relative, int64_t chimax,
+                            int max_sweeps, int* sweeps_done,
+                            const int* mask_host) {
+  using c_t = typename JT<R>::c_t;
   hipStream_t s = ctx->stream;
-  const double tol = std::sqrt((double)rows_p) * (DBL_EPSILON * 0.5);
+  const bool dbl = sizeof(R) == sizeof(double);
+  const double tol = dbl ? std::sqrt((double)rows_p) * (DBL_EPSILON * 0.5)
+                         : std::sqrt((double)rows_p) * FLT_EPSILON;
+  const double zrel = dbl ? (double)rows_p * DBL_EPSILON
+                          : 8.0 * std::sqrt((double)rows_p) * FLT_EPSILON;
   const int nb = 2 * p;
   const int rounds = nb - 1;
   // inner sweeps per block pair: the whole problem when it is one block
@@ -760,18 +1183,28 @@ int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t ldx, int64_t sX,
   if (want_w_host)
     QTN_CHECK_HIP(hipMemcpyAsync(want_w, want_w_host, batch * sizeof(int),
                                  hipMemcpyHostToDevice, s));
+  if (mask_host)
+    QTN_CHECK_HIP(hipMemcpyAsync(mask, mask_host, batch * sizeof(int),
+                                 hipMemcpyHostToDevice, s));
   QTN_CHECK_HIP(hipMemsetAsync(misc, 0, 3 * sizeof(int), s));
   ctx_ = ctx;
   prof_events.clear();
   int n_act = batch;
+  if (mask_host) {
+    n_act = 0;
+    for (int b = 0; b < batch; ++b) n_act += mask_host[b] ? 1 : 0;
+  }
+  const int n_solved = n_act;
   QTN_CHECK_HIP(hipMemsetAsync(offmax, 0, batch * sizeof(double), s));
-  hipLaunchKernelGGL(jac_prepare, dim3(batch), dim3(256), 0, s, X, rows_p,
-                     ldx, cols_p, sX, (double)rows_p * DBL_EPSILON, normF2,
-                     zthr, active, counters, misc + 1);
+  hipLaunchKernelGGL(jac_prepare<R>, dim3(batch), dim3(256), 0, s,
+                     (const c_t*)X, rows_p, ldx, cols_p, sX, zrel, normF2,
+                     zthr, active, counters, misc + 1,
+                     (const int*)(mask_host ? mask : nullptr));
   QTN_CHECK_HIP(hipGetLastError());
   if (Wm) {
     // W starts as the identity on the padded columns.
-    QTN_TRY(launch_identity(s, Wm, cols_p, sW, batch));
+    QTN_TRY(launch_identity_t<R>(s, Wm, cols_p, ldw ? ldw : cols_p, sW,
+                                 batch));
   }
   int* host_flags = nullptr;
   QTN_TRY(ctx->pinned_buffer(4 * sizeof(int), (void**)&host_flags));
@@ -780,11 +1213,13 @@ int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t ldx, int64_t sX,
   for (; sweep < max_sweeps; ++sweep) {
     for (int r = 0; r < rounds; ++r) {
       if (this->W == 64)
-        QTN_TRY(launch_round<64>(s, X, ldx, sX, Wm, sW, r, tol, n_inner,
-                                 nchunk_x, nchunk_w, want_w_host != nullptr));
+        QTN_TRY((launch_round<R, 64>(s, X, ldx, sX, Wm, sW, r, tol, n_inner,
+                                     nchunk_x, nchunk_w,
+                                     want_w_host != nullptr)));
       else
-        QTN_TRY(launch_round<32>(s, X, ldx, sX, Wm, sW, r, tol, n_inner,
-                                 nchunk_x, nchunk_w, want_w_host != nullptr));
+        QTN_TRY((launch_round<R, 32>(s, X, ldx, sX, Wm, sW, r, tol, n_inner,
+                                     nchunk_x, nchunk_w,
+                                     want_w_host != nullptr)));
     }
     QTN_CHECK_HIP(hipGetLastError());
     hipLaunchKernelGGL(jac_sweep_end, dim3(1), dim3(256), 0, s, active,
@@ -806,8 +1241,8 @@ int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t ldx, int64_t sX,
     }
   }
   if (sweeps_done) *sweeps_done = sweep;
-  ctx->n_svd += batch;
-  ctx->n_sweeps += (int64_t)sweep * batch;
+  ctx->n_svd += n_solved;
+  ctx->n_sweeps += (int64_t)sweep * n_solved;
   ctx->max_sweeps = std::max<int64_t>(ctx->max_sweeps, sweep);
   ctx->n_batches += 1;
   if (!converged) {
@@ -815,9 +1250,10 @@ int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t ldx, int64_t sX,
               std::to_string(max_sweeps) + " sweeps");
     return QTN_ERR_NOCONV;
   }
-  hipLaunchKernelGGL(jac_colnorms, dim3((unsigned)((cols_p + 3) / 4), batch),
-                     dim3(256), 0, s, (const double2*)X, rows_p, ldx, cols_p,
-                     sX, sig);
+  hipLaunchKernelGGL(jac_colnorms<R>,
+                     dim3((unsigned)((cols_p + 3) / 4), batch), dim3(256), 0,
+                     s, (const c_t*)X, rows_p, ldx, cols_p, sX, sig,
+                     (const double*)(dbl ? nullptr : zthr));
   if (cols_p > kSortMax) {
     set_error("svd: more than 4096 columns are not supported");
     return QTN_ERR_ARG;
@@ -830,23 +1266,35 @@ int32_t JacobiEngine::run(qtn_ctx* ctx, double2* X, int64_t ldx, int64_t sX,
 }
 
 namespace {
-__global__ void identity_kernel(double2* W, int64_t n, int64_t sW) {
+template <typename R>
+__global__ void identity_kernel(typename JT<R>::c_t* W, int64_t n,
+                                int64_t ld, int64_t sW) {
   const int64_t bz = blockIdx.y;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t r = i % n, c = i / n;
-    W[bz * sW + i] = make_double2(r == c ? 1.0 : 0.0, 0.0);
+    W[bz * sW + r + ld * c] = JT<R>::mk(r == c ? (R)1 : (R)0, (R)0);
   }
 }
 }  // namespace
 
-int32_t launch_identity(hipStream_t s, double2* W, int64_t n, int64_t sW,
-                        int batch) {
+template <typename R>
+int32_t launch_identity_t(hipStream_t s, typename CType<R>::type* W,
+                          int64_t n, int64_t ld, int64_t sW, int batch) {
   const int64_t blocks = std::min<int64_t>((n * n + 255) / 256, 4096);
-  hipLaunchKernelGGL(identity_kernel, dim3((unsigned)blocks, batch),
-                     dim3(256), 0, s, W, n, sW);
+  hipLaunchKernelGGL(identity_kernel<R>, dim3((unsigned)blocks, batch),
+                     dim3(256), 0, s, W, n, ld, sW);
   QTN_CHECK_HIP(hipGetLastError());
   return QTN_OK;
+}
+template int32_t launch_identity_t<double>(hipStream_t, double2*, int64_t,
+                                           int64_t, int64_t, int);
+template int32_t launch_identity_t<float>(hipStream_t, float2*, int64_t,
+                                          int64_t, int64_t, int);
+
+int32_t launch_identity(hipStream_t s, double2* W, int64_t n, int64_t sW,
+                        int batch) {
+  return launch_identity_t<double>(s, W, n, n, sW, batch);
 }
 
 }  // namespace qtn

@@ -22,9 +22,14 @@ struct JacobiEngine {
   int64_t cols_p = 0;  // padded columns of X (multiple of 64)
   int W = 64;          // block-pair width (columns of one Gram block)
   int p = 0;           // block pairs per round (= cols_p / W)
-  // device arrays (valid after reserve)
-  double2* Gp = nullptr;
-  double2* Q = nullptr;
+  bool single = false; // FP32 sweeps (complex float X, Gram and rotations)
+  int slot_base = kSlotGram;  // first of the 5 context slots it uses
+  bool fused = false;  // W = 32 rounds as one jac_round launch
+  int64_t ldw = 0;     // leading dimension of W (0: cols_p)
+  // device arrays (valid after reserve); Gram partials and rotation blocks
+  // are complex double, or complex float when `single`
+  void* Gp = nullptr;
+  void* Q = nullptr;
   double* normF2 = nullptr;
   double* zthr = nullptr;
   double* disc = nullptr;   // discarded weight per problem
@@ -38,6 +43,7 @@ struct JacobiEngine {
   int* active = nullptr;
   int* status = nullptr;
   int* want_w = nullptr;
+  int* mask = nullptr;  // problems to solve (host mask of run*)
   int* misc = nullptr;
   double* sig = nullptr;  // [batch][cols_p], sorted descending after run
   qtn_ctx* ctx_ = nullptr;
@@ -45,7 +51,8 @@ struct JacobiEngine {
   int32_t account_sweep(int n_active, int qpairs, int nchunk);
   int* perm = nullptr;    // [batch][cols_p], column of X for each sigma
 
-  int32_t reserve(qtn_ctx* ctx, int batch, int64_t rows_p, int64_t cols_p);
+  int32_t reserve(qtn_ctx* ctx, int batch, int64_t rows_p, int64_t cols_p,
+                  bool single = false);
   // Orthogonalise the columns of every X (rows_p x cols_p, ld rows_p,
   // batch stride sX); if W is given (cols_p x cols_p, stride sW) it
   // accumulates the rotations starting from the identity, for the problems
@@ -53,20 +60,38 @@ struct JacobiEngine {
   // Afterwards sig / perm hold the sorted column norms and kept / disc /
   // total / status the truncation decision for nsv_host[b] meaningful
   // values.
-  template <int WW>
-  int32_t launch_round(hipStream_t s, double2* X, int64_t ldx, int64_t sX,
-                       double2* Wm, int64_t sW, int r, double tol,
-                       int n_inner, int nchunk_x, int nchunk_w,
-                       bool per_problem_w);
+  template <typename R, int WW>
+  int32_t launch_round(hipStream_t s, void* X, int64_t ldx, int64_t sX,
+                       void* Wm, int64_t sW, int r, double tol, int n_inner,
+                       int nchunk_x, int nchunk_w, bool per_problem_w);
+  template <typename R>
+  int32_t run_t(qtn_ctx* ctx, typename CType<R>::type* X, int64_t ldx,
+                int64_t sX, typename CType<R>::type* Wm, int64_t sW,
+                const int* want_w_host, const int64_t* nsv_host,
+                double cutoff, int relative, int64_t chimax, int max_sweeps,
+                int* sweeps_done, const int* mask_host);
   // X: rows_p rows are processed, leading dimension ldx >= rows_p.
+  // mask_host (may be NULL = all): problems with mask 0 are left alone and
+  // their results are meaningless.
   int32_t run(qtn_ctx* ctx, double2* X, int64_t ldx, int64_t sX, double2* Wm,
               int64_t sW,
               const int* want_w_host, const int64_t* nsv_host, double cutoff,
-              int relative, int64_t chimax, int max_sweeps, int* sweeps_done);
+              int relative, int64_t chimax, int max_sweeps, int* sweeps_done,
+              const int* mask_host = nullptr);
+  // The same sweeps in complex FP32 (engine reserved with single = true).
+  // Columns that end below the FP32 noise floor (8 sqrt(rows) eps32 |X|_F)
+  // are never rotated and count as exact zeros in sig and the truncation.
+  int32_t run32(qtn_ctx* ctx, float2* X, int64_t ldx, int64_t sX, float2* Wm,
+                int64_t sW, const int* want_w_host, const int64_t* nsv_host,
+                double cutoff, int relative, int64_t chimax, int max_sweeps,
+                int* sweeps_done, const int* mask_host = nullptr);
 };
 
 int32_t launch_identity(hipStream_t s, double2* W, int64_t n, int64_t sW,
                         int batch);
+template <typename R>
+int32_t launch_identity_t(hipStream_t s, typename CType<R>::type* W,
+                          int64_t n, int64_t ld, int64_t sW, int batch);
 
 inline int64_t round_up64(int64_t x) { return ((x + 63) / 64) * 64; }
 
