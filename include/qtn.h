@@ -54,6 +54,7 @@ extern "C" {
 
 typedef struct qtn_ctx qtn_ctx;
 typedef struct qtn_mps qtn_mps;
+typedef struct qtn_mpo qtn_mpo;
 
 /* ---- context ----------------------------------------------------------- */
 /* Opens device `device`, creates one HIP stream and a workspace pool.      */
@@ -183,9 +184,12 @@ int32_t qtn_mps_create(qtn_ctx* ctx, int32_t nsites, int64_t chimax_cap,
 int32_t qtn_mps_destroy(qtn_mps* mps);
 /* Copy state a into b (same nsites and capacity).                          */
 int32_t qtn_mps_copy(const qtn_mps* a, qtn_mps* b);
-/* Replace the state by host site tensors (dims[3*j..3*j+2] for site j,
- * left-canonical not required); the library brings it to right-canonical
- * form with bond spectra (exact, no truncation unless chimax_cap binds).   */
+/* Replace the state by host site tensors of any gauge (dims[3*j..3*j+2] =
+ * (chi_j, 2, chi_{j+1}) for 0-based site j, boundary bonds 1): an SVD sweep
+ * left to right, then one right to left bring it to the right-canonical
+ * form with bond spectra, normalised (only exact zeros are dropped, and
+ * bonds beyond chimax_cap are truncated to it).  Replaces MPS construction
+ * from tensors (src/mps.jl:99-113) for the device state.                  */
 int32_t qtn_mps_set_sites(qtn_mps* mps, const int64_t* dims,
                           const double* const* sites);
 /* Bond dimensions chi_0..chi_n (n+1 values).                               */
@@ -226,6 +230,27 @@ int32_t qtn_mps_expect_z(const qtn_mps* mps, double* out);
 int32_t qtn_mps_overlap(const qtn_mps* a, const qtn_mps* b, double* out);
 /* <psi|psi> (real).                                                        */
 int32_t qtn_mps_norm2(const qtn_mps* mps, double* out);
+
+/* ---- matrix-product operators (apply_MPO, src/mpo.jl:184-252) ----------- */
+/* MPO of nsites tensors W_j (D_j, 2 out, 2 in, D_{j+1}), column-major host
+ * arrays, D_0 = D_n = 1 (the middle-tensor layout of MPO(m),
+ * src/mpo.jl:66); (W psi)[s'] = sum_s W[s', s] psi[s], wire j <-> bit j-1. */
+int32_t qtn_mpo_create(qtn_ctx* ctx, int32_t nsites, const int64_t* bond_dims,
+                       const double* const* sites, qtn_mpo** mpo);
+int32_t qtn_mpo_destroy(qtn_mpo* mpo);
+/* <psi|W|psi> (complex, 2 doubles) by a left-to-right environment sweep on
+ * MFMA GEMMs; psi is normalised in the device gauge.                      */
+int32_t qtn_mps_expect_mpo(const qtn_mps* mps, const qtn_mpo* mpo,
+                           double* out);
+/* out <- W|in> compressed: zip-up left to right (truncation per bond with
+ * the relative cutoff and chimax), then a right-to-left SVD sweep that
+ * recompresses with the same rule into out's right-canonical gauge.
+ * *norm receives |W in| of the compressed state (out is normalised).
+ * Replaces contract(apply_MPO(psi, mpo, iwire)) for chains too long to
+ * contract densely.                                                        */
+int32_t qtn_mps_apply_mpo(const qtn_mps* in, const qtn_mpo* mpo,
+                          double cutoff, int64_t chimax, qtn_mps* out,
+                          double* norm);
 
 #ifdef __cplusplus
 }

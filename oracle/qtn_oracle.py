@@ -46,6 +46,14 @@ PINNING = {
     "TebdState": "untruncated runs equal the dense statevector simulator; "
                  "product-state closed forms",
     "StateVector": "closed forms: X/H/CNOT on basis states, unitarity",
+    "tfim_mpo": "contracted to a dense matrix it equals the kron-sum TFIM "
+                "Hamiltonian (tfim_dense) for n <= 10",
+    "expect_mpo": "equals psi^H H psi of the dense vector and matrix for "
+                  "n <= 10; <psi|I|psi> = <psi|psi>",
+    "canonicalize": "round trip to the dense vector; right-canonical "
+                    "sites; lambda = Schmidt values of the dense vector",
+    "zipup": "without truncation equals the dense H psi (n <= 10); the "
+             "truncated result's lambda and norm follow the same rule",
 }
 
 
@@ -448,3 +456,184 @@ def run_layers(state, layers, cutoff, chimax, record=None):
             if record is not None:
                 record.append((w, k, disc, S))
     return state
+
+
+# ---------------------------------------------------------------------------
+# MPO x MPS (the build's zip-up path; src/mpo.jl:184-252 only wires an MPO
+# into a network, the compressed application is new)
+#
+# MPO site tensors W_j are (D_{j-1}, 2 out, 2 in, D_j), the layout of the
+# middle tensors of MPO(m) (src/mpo.jl:66); (W psi)[s'] = sum_s W[s', s]
+# psi[s] with wire j <-> bit j-1 of the state index.
+def tfim_mpo(n, J, h):
+    """H = -J sum_i Z_i Z_{i+1} - sum_i h_i X_i as an MPO of bond dim 3:
+    bulk W = [[I, 0, 0], [Z, 0, 0], [-h X, -J Z, I]] (row = left index),
+    first site its last row, last site its first column."""
+    I2 = np.eye(2, dtype=np.complex128)
+    X = np.array([[0, 1], [1, 0]], dtype=np.complex128)
+    Z = np.diag([1.0, -1.0]).astype(np.complex128)
+    h = np.asarray(h, dtype=np.float64)
+    out = []
+    for j in range(n):
+        W = np.zeros((3, 2, 2, 3), dtype=np.complex128)
+        W[0, :, :, 0] = I2
+        W[1, :, :, 0] = Z
+        W[2, :, :, 0] = -h[j] * X
+        W[2, :, :, 1] = -J * Z
+        W[2, :, :, 2] = I2
+        if j == 0:
+            W = W[2:3]
+        if j == n - 1:
+            W = W[:, :, :, 0:1]
+        out.append(W)
+    return out
+
+
+def tfim_dense(n, J, h):
+    """The same Hamiltonian as a dense 2^n x 2^n matrix from Kronecker
+    products (wire j <-> bit j-1, i.e. the last kron factor is wire 1)."""
+    X = np.array([[0, 1], [1, 0]], dtype=np.complex128)
+    Z = np.diag([1.0, -1.0]).astype(np.complex128)
+
+    def op(ops):  # ops: {wire: 2x2}
+        m = np.ones((1, 1), dtype=np.complex128)
+        for w in range(n, 0, -1):
+            m = np.kron(m, ops.get(w, np.eye(2)))
+        return m
+
+    H = np.zeros((2 ** n, 2 ** n), dtype=np.complex128)
+    for i in range(1, n):
+        H -= J * op({i: Z, i + 1: Z})
+    for i in range(1, n + 1):
+        H -= h[i - 1] * op({i: X})
+    return H
+
+
+def mpo_dense(W):
+    """Contract an MPO to its dense matrix [row, col] (row = out bits,
+    wire j <-> bit j-1)."""
+    M = W[0][0]  # (2 out, 2 in, D1)
+    for j in range(1, len(W)):
+        T = np.einsum("rcd,dsue->rscue", M, W[j])
+        R, S_, Cc, U, E = T.shape
+        # new row index r + R s, new column c + C u (earlier wires fastest)
+        M = T.reshape((R * S_, Cc * U, E), order="F")
+    return M[:, :, 0]
+
+
+def mps_dense(Bs):
+    """Dense vector of an MPS given as site tensors (chi_{j-1}, 2, chi_j)."""
+    v = np.ones((1, 1), dtype=np.complex128)
+    for B in Bs:
+        v = np.einsum("xa,asb->xsb", v, B).reshape((-1, B.shape[2]),
+                                                  order="F")
+    return v[:, 0]
+
+
+def expect_mpo(Bs, W, As=None):
+    """<A|W|B> by a left-to-right environment sweep (A = B by default):
+    L[b, d, b'] = sum conj(A[a, s', b]) L[a, d0, a'] W[d0, s', s, d]
+    B[a', s, b']."""
+    if As is None:
+        As = Bs
+    L = np.ones((1, 1, 1), dtype=np.complex128)
+    for A, B, Wj in zip(As, Bs, W):
+        T = np.einsum("adx,xsy->adsy", L, B)
+        T = np.einsum("adsy,dtse->atey", T, Wj)
+        L = np.einsum("atb,atey->bey", A.conj(), T)
+    return complex(L[0, 0, 0])
+
+
+def random_mps(n, chimax, seed):
+    """Complex Gaussian site tensors with bonds min(chimax, 2^j, 2^(n-j)),
+    then left-canonicalised by a QR sweep and normalised."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dims = [min(chimax, 2 ** j, 2 ** (n - j)) for j in range(n + 1)]
+    Bs = []
+    for j in range(n):
+        a, b = dims[j], dims[j + 1]
+        t = (rng.standard_normal((a, 2, b)) +
+             1j * rng.standard_normal((a, 2, b))) / np.sqrt(2.0)
+        Bs.append(t)
+    return left_canonical(Bs)
+
+
+def left_canonical(Bs):
+    """QR sweep left to right; the last site carries the norm, which is
+    then divided out."""
+    Bs = [b.copy() for b in Bs]
+    for j in range(len(Bs) - 1):
+        a, _, b = Bs[j].shape
+        Q, R = np.linalg.qr(Bs[j].reshape((2 * a, b), order="F"))
+        k = Q.shape[1]
+        Bs[j] = Q.reshape((a, 2, k), order="F")
+        nb = Bs[j + 1].shape[2]
+        Bs[j + 1] = (R @ Bs[j + 1].reshape((b, 2 * nb), order="F")
+                     ).reshape((k, 2, nb), order="F")
+    Bs[-1] = Bs[-1] / np.linalg.norm(Bs[-1])
+    return Bs
+
+
+def canonicalize(Bs, cutoff=0.0, chimax=0, left_done=False):
+    """Right-canonical form with bond spectra, the layout of libqtn's
+    device MPS: an SVD sweep left to right (left-orthonormal sites), then
+    one right to left truncating each bond with the build's rule.  Returns
+    (B, lam, norm): B right-canonical, lam[j] the normalised Schmidt values
+    of bond j (lam[0] = lam[n] = [1]), norm = |psi|."""
+    Bs = [b.copy() for b in Bs]
+    n = len(Bs)
+    if not left_done:
+        for j in range(n - 1):
+            a, _, b = Bs[j].shape
+            U, S, Vh = np.linalg.svd(Bs[j].reshape((2 * a, b), order="F"),
+                                     full_matrices=False)
+            k = S.size
+            Bs[j] = U.reshape((a, 2, k), order="F")
+            nb = Bs[j + 1].shape[2]
+            Bs[j + 1] = ((S[:, None] * Vh) @
+                         Bs[j + 1].reshape((b, 2 * nb), order="F")
+                         ).reshape((k, 2, nb), order="F")
+    lam = [None] * (n + 1)
+    lam[0] = np.ones(1)
+    lam[n] = np.ones(1)
+    for j in range(n - 1, 0, -1):
+        a, _, b = Bs[j].shape
+        U, S, Vh = np.linalg.svd(Bs[j].reshape((a, 2 * b), order="F"),
+                                 full_matrices=False)
+        k, _, _ = truncate(S, cutoff, True, chimax)
+        nrm = np.sqrt(np.sum(S[:k][::-1] ** 2))
+        Bs[j] = Vh[:k].reshape((k, 2, b), order="F")
+        lam[j] = S[:k] / nrm
+        pa = Bs[j - 1].shape[0]
+        Bs[j - 1] = (Bs[j - 1].reshape((2 * pa, a), order="F") @
+                     (U[:, :k] * S[:k])).reshape((pa, 2, k), order="F")
+    norm = np.linalg.norm(Bs[0])
+    Bs[0] = Bs[0] / norm
+    return Bs, lam, float(norm)
+
+
+def zipup(Bs, W, cutoff, chimax):
+    """Zip-up of W|psi> (Stoudenmire & White, New J. Phys. 12, 055026
+    (2010)) left to right with the build's truncation at every step, then
+    the right-to-left recompression of canonicalize.  Returns
+    (B, lam, norm) of the compressed W|psi>.  The truncations are near
+    optimal only for a right-canonical psi (the device MPS gauge)."""
+    n = len(Bs)
+    C = np.ones((1, 1, 1), dtype=np.complex128)  # (k, D, chi)
+    out = []
+    for j in range(n):
+        B, Wj = Bs[j], W[j]
+        k, D, chi = C.shape
+        cr, Dr = B.shape[2], Wj.shape[3]
+        T1 = np.einsum("kdx,xsb->kdsb", C, B)
+        T2 = np.einsum("kdsb,dtse->ktbe", T1, Wj)  # (k, s', b, d')
+        if j == n - 1:
+            out.append(T2.reshape((k, 2, cr * Dr), order="F"))
+            break
+        M = T2.reshape((2 * k, cr * Dr), order="F")
+        U, S, Vh = np.linalg.svd(M, full_matrices=False)
+        kk, _, _ = truncate(S, cutoff, True, chimax)
+        out.append(U[:, :kk].reshape((k, 2, kk), order="F"))
+        C = (S[:kk, None] * Vh[:kk]).reshape((kk, cr, Dr), order="F")
+        C = np.transpose(C, (0, 2, 1))
+    return canonicalize(out, cutoff, chimax, left_done=True)

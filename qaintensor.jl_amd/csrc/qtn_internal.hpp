@@ -113,3 +113,16 @@ struct qtn_ctx {
   int32_t buffer(int slot, size_t bytes, void** out);
   int32_t pinned_buffer(size_t bytes, void** out);
 };
+
+// Device-resident MPS (qtn_mps.hip): right-canonical site tensors with the
+// bond spectra beside them, in one arena.
+struct qtn_mps {
+  qtn_ctx* ctx = nullptr;
+  int n = 0;
+  int64_t cap = 0;
+  std::vector<int64_t> dims;     // chi_0..chi_n
+  std::vector<int64_t> maxb;     // capacity of every bond
+  std::vector<double2*> site;    // device site tensors (0-based site index)
+  std::vector<double*> lam;      // device bond spectra, lam[j] for bond j
+  void* arena = nullptr;         // one allocation for all of the above
+};

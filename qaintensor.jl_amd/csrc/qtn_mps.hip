@@ -22,16 +22,6 @@
 #include "qtn_qr.hpp"
 #include "qtn_svd.hpp"
 
-struct qtn_mps {
-  qtn_ctx* ctx = nullptr;
-  int n = 0;
-  int64_t cap = 0;
-  std::vector<int64_t> dims;     // chi_0..chi_n
-  std::vector<int64_t> maxb;     // capacity of every bond
-  std::vector<double2*> site;    // device site tensors (0-based site index)
-  std::vector<double*> lam;      // device bond spectra, lam[j] for bond j
-  void* arena = nullptr;         // one allocation for all of the above
-};
 
 namespace qtn {
 namespace {
@@ -1122,16 +1112,6 @@ int32_t qtn_mps_norm2(const qtn_mps* m, double* out) {
   QTN_TRY(qtn_mps_overlap(m, m, v));
   *out = v[0];
   return QTN_OK;
-}
-
-int32_t qtn_mps_set_sites(qtn_mps* m, const int64_t* dims,
-                          const double* const* sites) {
-  clear_error();
-  (void)dims;
-  (void)sites;
-  QTN_ARG(m != nullptr, "qtn_mps_set_sites: NULL argument");
-  set_error("qtn_mps_set_sites: not available in this build");
-  return QTN_ERR_ARG;
 }
 
 }  // extern "C"

@@ -36,7 +36,8 @@ EXPORTS = [
     "qtn_mps_bond_dims", "qtn_mps_bond_spectrum", "qtn_mps_get_site",
     "qtn_mps_apply_1q", "qtn_mps_apply_2q", "qtn_mps_apply_layer",
     "qtn_mps_amplitudes", "qtn_mps_expect_z", "qtn_mps_overlap",
-    "qtn_mps_norm2",
+    "qtn_mps_norm2", "qtn_mpo_create", "qtn_mpo_destroy",
+    "qtn_mps_expect_mpo", "qtn_mps_apply_mpo",
 ]
 
 
@@ -114,6 +115,11 @@ def _declare(lib):
         "qtn_mps_expect_z": (_i32, [_vp, _pd]),
         "qtn_mps_overlap": (_i32, [_vp, _vp, _pd]),
         "qtn_mps_norm2": (_i32, [_vp, _pd]),
+        "qtn_mpo_create": (_i32, [_vp, _i32, _pi64, ctypes.POINTER(_pd),
+                                  ctypes.POINTER(_vp)]),
+        "qtn_mpo_destroy": (_i32, [_vp]),
+        "qtn_mps_expect_mpo": (_i32, [_vp, _vp, _pd]),
+        "qtn_mps_apply_mpo": (_i32, [_vp, _vp, _dbl, _i64, _vp, _pd]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

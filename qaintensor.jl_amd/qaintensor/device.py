@@ -172,3 +172,85 @@ class DeviceMPS:
         out = ctypes.c_double(0.0)
         L.check(L.load().qtn_mps_norm2(self.handle, ctypes.byref(out)))
         return out.value
+
+
+def _set_sites(self, tensors):
+    """Replace the state by site tensors (chi_j, 2, chi_{j+1}) of any gauge
+    (qtn_mps_set_sites: brought to the right-canonical gauge on the GPU,
+    normalised)."""
+    if len(tensors) != self.n:
+        raise ValueError("need one tensor per site")
+    arrs = [L.as_c128(t, "F") for t in tensors]
+    dims = np.zeros(3 * self.n, dtype=np.int64)
+    for j, t in enumerate(arrs):
+        if t.ndim != 3:
+            raise ValueError("site tensors must have 3 legs")
+        dims[3 * j:3 * j + 3] = t.shape
+    ptrs = (ctypes.POINTER(ctypes.c_double) * self.n)(
+        *[L.dptr(t) for t in arrs])
+    L.check(L.load().qtn_mps_set_sites(self.handle, L.i64ptr(dims), ptrs))
+
+
+def _expect_mpo(self, mpo):
+    """<psi|W|psi> (complex) on the GPU."""
+    out = np.zeros(2, dtype=np.float64)
+    L.check(L.load().qtn_mps_expect_mpo(self.handle, mpo.handle,
+                                        L.dptr(out)))
+    return complex(out[0], out[1])
+
+
+def _apply_mpo(self, mpo, cutoff, chimax, out):
+    """out <- W|self> by zip-up and recompression; returns |W psi| of the
+    compressed state (out is normalised)."""
+    nrm = ctypes.c_double(0.0)
+    L.check(L.load().qtn_mps_apply_mpo(self.handle, mpo.handle,
+                                       float(cutoff), int(chimax),
+                                       out.handle, ctypes.byref(nrm)))
+    return nrm.value
+
+
+DeviceMPS.set_sites = _set_sites
+DeviceMPS.expect_mpo = _expect_mpo
+DeviceMPS.apply_mpo = _apply_mpo
+
+
+class DeviceMPO:
+    """An MPO on the GPU: site tensors (D_j, 2 out, 2 in, D_{j+1})."""
+
+    def __init__(self, tensors, ctx=None):
+        self.ctx = ctx if ctx is not None else L.default_context()
+        arrs = [L.as_c128(t, "F") for t in tensors]
+        self.n = len(arrs)
+        D = np.zeros(self.n + 1, dtype=np.int64)
+        for j, t in enumerate(arrs):
+            if t.ndim != 4 or t.shape[1:3] != (2, 2):
+                raise ValueError("MPO tensors must be (D, 2, 2, D')")
+            if j > 0 and t.shape[0] != D[j]:
+                raise ValueError("MPO bond dimensions do not match")
+            D[j] = t.shape[0]
+            D[j + 1] = t.shape[3]
+        ptrs = (ctypes.POINTER(ctypes.c_double) * self.n)(
+            *[L.dptr(t) for t in arrs])
+        h = ctypes.c_void_p()
+        L.check(L.load().qtn_mpo_create(self.ctx.handle, self.n,
+                                        L.i64ptr(D), ptrs, ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise L.QtnError(L.QTN_ERR_ARG, "DeviceMPO already freed")
+        return self._h
+
+    def free(self):
+        if self._h is not None:
+            L.check(L.load().qtn_mpo_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            if self._h is not None and L._lib is not None:
+                L._lib.qtn_mpo_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass

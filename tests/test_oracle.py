@@ -316,3 +316,70 @@ def test_fused_gate_equals_sequential_application():
     a = O.apply_mpo_dense(a, g, (2, 3))
     b = O.apply_mpo_dense(psi, g @ np.kron(u2, u1), (2, 3))
     assert np.allclose(a, b, atol=1e-14)
+
+
+# -- MPO x MPS oracle (tfim_mpo, expect_mpo, canonicalize, zipup) ----------
+def test_tfim_mpo_equals_kron_hamiltonian():
+    for n in (2, 3, 5, 8):
+        h = np.random.default_rng(n).uniform(0.5, 1.5, n)
+        Hm = O.mpo_dense(O.tfim_mpo(n, 1.0, h))
+        Hk = O.tfim_dense(n, 1.0, h)
+        assert np.abs(Hm - Hk).max() == 0.0
+        assert np.allclose(Hk, Hk.conj().T)
+    # closed form: at h = 0 the all-zero state has energy -J (n-1)
+    H = O.tfim_dense(4, 0.7, np.zeros(4))
+    assert np.isclose(H[0, 0].real, -0.7 * 3)
+
+
+def test_expect_mpo_matches_dense_and_identity():
+    n = 7
+    h = np.random.default_rng(2).uniform(0.5, 1.5, n)
+    W = O.tfim_mpo(n, 1.0, h)
+    Bs = O.random_mps(n, 8, 11)
+    psi = O.mps_dense(Bs)
+    assert abs(np.linalg.norm(psi) - 1) < 1e-12
+    ref = np.vdot(psi, O.tfim_dense(n, 1.0, h) @ psi)
+    assert abs(O.expect_mpo(Bs, W) - ref) < 1e-12
+    eye = [np.eye(2, dtype=complex).reshape((1, 2, 2, 1))] * n
+    assert abs(O.expect_mpo(Bs, eye) - 1.0) < 1e-12
+
+
+def test_canonicalize_right_form_and_schmidt_values():
+    n = 8
+    Bs = O.random_mps(n, 16, 3)
+    psi = O.mps_dense(Bs)
+    B, lam, nrm = O.canonicalize(Bs)
+    assert np.abs(O.mps_dense(B) * nrm - psi).max() < 1e-13
+    for j in range(n):
+        m = B[j].reshape((B[j].shape[0], -1), order="F")
+        assert np.abs(m @ m.conj().T - np.eye(m.shape[0])).max() < 1e-13
+    for cut in range(1, n):
+        s = np.linalg.svd(psi.reshape((2 ** cut, -1), order="F"),
+                          compute_uv=False)
+        s = s[s > 1e-12]
+        assert np.allclose(lam[cut], s / np.linalg.norm(s), atol=1e-12)
+
+
+def test_zipup_exact_and_truncated():
+    n = 8
+    h = np.random.default_rng(4).uniform(0.5, 1.5, n)
+    W = O.tfim_mpo(n, 1.0, h)
+    Bs = O.random_mps(n, 8, 5)
+    psi = O.mps_dense(Bs)
+    Hpsi = O.tfim_dense(n, 1.0, h) @ psi
+    B, lam, nrm = O.zipup(Bs, W, 0.0, 0)
+    assert np.abs(O.mps_dense(B) * nrm - Hpsi).max() < 1e-12
+    # enough bond dimension for H psi (3 x 8): exact again
+    B, lam, nrm = O.zipup(Bs, W, 0.0, 24)
+    assert np.abs(O.mps_dense(B) * nrm - Hpsi).max() < 1e-12
+    # truncated: normalised, norm below |H psi|, caps respected, and the
+    # overlap with the exact H psi grows with the cap
+    fids = []
+    for chi in (4, 6, 8, 12):
+        B, lam, nrm = O.zipup(Bs, W, 1e-3, chi)
+        phi = O.mps_dense(B)
+        assert abs(np.linalg.norm(phi) - 1) < 1e-12
+        assert nrm <= np.linalg.norm(Hpsi) * (1 + 1e-12)
+        assert max(b.shape[2] for b in B) <= chi
+        fids.append(abs(np.vdot(phi, Hpsi)) / np.linalg.norm(Hpsi))
+    assert all(a < b for a, b in zip(fids, fids[1:])) and fids[-1] < 1
