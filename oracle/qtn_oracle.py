@@ -52,6 +52,7 @@ PINNING = {
                   "n <= 10; <psi|I|psi> = <psi|psi>",
     "canonicalize": "round trip to the dense vector; right-canonical "
                     "sites; lambda = Schmidt values of the dense vector",
+    "expect_mpo_blas": "equals expect_mpo (einsum form) on random states",
     "zipup": "without truncation equals the dense H psi (n <= 10); the "
              "truncated result's lambda and norm follow the same rule",
 }
@@ -541,6 +542,26 @@ def expect_mpo(Bs, W, As=None):
         T = np.einsum("adx,xsy->adsy", L, B)
         T = np.einsum("adsy,dtse->atey", T, Wj)
         L = np.einsum("atb,atey->bey", A.conj(), T)
+    return complex(L[0, 0, 0])
+
+
+def expect_mpo_blas(Bs, W):
+    """expect_mpo with every contraction a BLAS matmul (for chi ~ 1e3):
+    L (a, d, a') -> T1 = L B -> (a, b', d, s) x W(d, s, t, e) -> sum over
+    (a, t) with conj(B)."""
+    L = np.ones((1, 1, 1), dtype=np.complex128)
+    for B, Wj in zip(Bs, W):
+        cl, _, cr = B.shape
+        Dl, Dr = Wj.shape[0], Wj.shape[3]
+        T1 = L.reshape((cl * Dl, cl), order="F") @ \
+            B.reshape((cl, 2 * cr), order="F")        # (a, d | s, b')
+        T1 = T1.reshape((cl, Dl, 2, cr), order="F").transpose(0, 3, 1, 2)
+        Wp = Wj.transpose(0, 2, 1, 3).reshape((Dl * 2, 2 * Dr), order="F")
+        T2 = T1.reshape((cl * cr, Dl * 2), order="F") @ Wp  # (a, b' | t, e)
+        T2 = T2.reshape((cl, cr, 2, Dr), order="F").transpose(0, 2, 3, 1)
+        L = B.reshape((2 * cl, cr), order="F").conj().T @ \
+            T2.reshape((2 * cl, Dr * cr), order="F")
+        L = L.reshape((cr, Dr, cr), order="F")
     return complex(L[0, 0, 0])
 
 

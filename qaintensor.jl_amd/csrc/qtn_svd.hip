@@ -780,7 +780,8 @@ __global__ __launch_bounds__(256) void jac_prepare(
     int64_t sX, double zrel,
     double* __restrict__ normF2, double* __restrict__ zthr,
     int* __restrict__ active, int* __restrict__ counters,
-    int* __restrict__ bad, const int* __restrict__ mask) {
+    int* __restrict__ bad, const int* __restrict__ mask,
+    int* __restrict__ scale_e) {
   const int bz = blockIdx.x;
   if (mask && !mask[bz]) {
     if (threadIdx.x == 0) {
@@ -788,36 +789,82 @@ __global__ __launch_bounds__(256) void jac_prepare(
       zthr[bz] = 0.0;
       active[bz] = 0;
       counters[bz] = 0;
+      scale_e[bz] = 0;
     }
     return;
   }
-  X += (int64_t)bz * sX;
-  double s = 0.0;
+  X += bz * sX;
+  // pass 1: largest |re|, |im| and finiteness
+  double amax = 0.0;
   int nonfinite = 0;
   for (int64_t i = threadIdx.x; i < rows * cols; i += 256) {
     const typename JT<R>::c_t v = X[(i % rows) + (i / rows) * ldx];
     if (!isfinite(v.x) || !isfinite(v.y)) nonfinite = 1;
-    s += (double)v.x * v.x + (double)v.y * v.y;
+    amax = fmax(amax, fmax(fabs((double)v.x), fabs((double)v.y)));
   }
   __shared__ double red[256];
   __shared__ int nf;
   if (threadIdx.x == 0) nf = 0;
-  red[threadIdx.x] = s;
+  red[threadIdx.x] = amax;
   __syncthreads();
   if (nonfinite) atomicOr(&nf, 1);
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  // The sweeps run on X * 2^-e with 2^e <= max |x| < 2^(e+1) (an exact
+  // scaling), so the squared Gram entries stay far from overflow and
+  // underflow whatever the input's magnitude; jac_scale applies it and
+  // undoes it after the sweeps.
+  const double am = red[0];
+  const int e = (am > 0.0 && isfinite(am)) ? ilogb(am) : 0;
+  __syncthreads();
+  // pass 2: |X 2^-e|_F^2
+  double ssum = 0.0;
+  if (!nf)
+    for (int64_t i = threadIdx.x; i < rows * cols; i += 256) {
+      const typename JT<R>::c_t v = X[(i % rows) + (i / rows) * ldx];
+      const double a = ldexp((double)v.x, -e), b = ldexp((double)v.y, -e);
+      ssum += a * a + b * b;
+    }
+  red[threadIdx.x] = ssum;
+  __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    const double f2 = red[0];
-    normF2[bz] = f2;
+    const double f2s = red[0];
+    scale_e[bz] = e;
+    normF2[bz] = f2s;
     // Columns with norm below rows*eps*|X|_F are rounding noise (the
     // backward error of the rotations themselves): never rotated.
-    zthr[bz] = zrel * zrel * f2;
-    active[bz] = (f2 > 0.0 && !nf) ? 1 : 0;
+    zthr[bz] = zrel * zrel * f2s;
+    active[bz] = (f2s > 0.0 && !nf) ? 1 : 0;
     counters[bz] = 0;
     if (nf) atomicOr(bad, 1);
+  }
+}
+
+// X *= 2^(sgn * e[bz]) over the rows x cols block (exact).
+// grid (tiles, batch).
+template <typename R>
+__global__ void jac_scale(typename JT<R>::c_t* __restrict__ X, int64_t rows,
+                          int64_t ldx, int64_t cols, int64_t sX,
+                          const int* __restrict__ scale_e, int sgn) {
+  const int bz = blockIdx.y;
+  const int e = scale_e[bz];
+  if (e == 0) return;
+  const R f = (R)ldexp(1.0, sgn * e);
+  typename JT<R>::c_t* x = X + (int64_t)bz * sX;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+       i < rows * cols; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i % rows, c = i / rows;
+    typename JT<R>::c_t v = x[r + c * ldx];
+    v.x *= f;
+    v.y *= f;
+    x[r + c * ldx] = v;
   }
 }
 
@@ -826,7 +873,7 @@ template <typename R>
 __global__ __launch_bounds__(256) void jac_colnorms(
     const typename JT<R>::c_t* __restrict__ X, int64_t rows, int64_t ldx,
     int64_t cols, int64_t sX, double* __restrict__ sig,
-    const double* __restrict__ floor2) {
+    const double* __restrict__ floor2, const int* __restrict__ scale_e) {
   const int bz = blockIdx.y;
   const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -840,7 +887,8 @@ __global__ __launch_bounds__(256) void jac_colnorms(
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   // columns the sweeps left alone (below the zero threshold) count as zero
   if (floor2 && s <= floor2[bz]) s = 0.0;
-  if (lane == 0) sig[(int64_t)bz * cols + c] = sqrt(s);
+  // X is still scaled by 2^-e here
+  if (lane == 0) sig[(int64_t)bz * cols + c] = ldexp(sqrt(s), scale_e[bz]);
 }
 
 // Sort sigma descending (bitonic in LDS) and apply the truncation rule of
@@ -853,7 +901,7 @@ __global__ __launch_bounds__(1024) void jac_sort_trunc(
     const int64_t* __restrict__ nsv, double cutoff, int relative,
     int64_t chimax, int64_t* __restrict__ kept, double* __restrict__ disc,
     double* __restrict__ total, double* __restrict__ keptw,
-    int* __restrict__ status) {
+    int* __restrict__ status, const int* __restrict__ scale_e) {
   const int bz = blockIdx.x;
   __shared__ double sv[kSortMax];
   __shared__ int si[kSortMax];
@@ -895,16 +943,25 @@ __global__ __launch_bounds__(1024) void jac_sort_trunc(
   __syncthreads();
   if (threadIdx.x == 0) {
     const int64_t n = nsv[bz];
+    // The sums run on sigma * 2^-e (the sweeps' exact scaling) so that
+    // squares of very large or small values neither overflow nor vanish;
+    // the rule is unchanged (er scales with the values), the weights are
+    // reported unscaled.
+    const int e = scale_e ? scale_e[bz] : 0;
     // cum[j] = sum_{l >= j} s_l^2, accumulated from the smallest upwards
     double c2 = 0.0;
-    for (int64_t j = n - 1; j >= 0; --j) c2 += sv[j] * sv[j];
+    for (int64_t j = n - 1; j >= 0; --j) {
+      const double v = ldexp(sv[j], -e);
+      c2 += v * v;
+    }
     const double tot = c2;
-    const double er = relative ? cutoff * sqrt(tot) : cutoff;
+    const double er = relative ? cutoff * sqrt(tot) : ldexp(cutoff, -e);
     c2 = 0.0;
     int64_t k = -1;
     double disc_w = 0.0;
     for (int64_t j = n - 1; j >= 0; --j) {
-      c2 += sv[j] * sv[j];
+      const double v = ldexp(sv[j], -e);
+      c2 += v * v;
       if (sqrt(c2) > er) {
         k = j + 1;
         break;
@@ -918,13 +975,19 @@ __global__ __launch_bounds__(1024) void jac_sort_trunc(
     if (chimax > 0 && k > chimax) k = chimax;
     // discarded weight: sum of squares of s[k..n-1], smallest first
     disc_w = 0.0;
-    for (int64_t j = n - 1; j >= k; --j) disc_w += sv[j] * sv[j];
+    for (int64_t j = n - 1; j >= k; --j) {
+      const double v = ldexp(sv[j], -e);
+      disc_w += v * v;
+    }
     double kw = 0.0;
-    for (int64_t j = k - 1; j >= 0; --j) kw += sv[j] * sv[j];
+    for (int64_t j = k - 1; j >= 0; --j) {
+      const double v = ldexp(sv[j], -e);
+      kw += v * v;
+    }
     kept[bz] = k;
-    disc[bz] = disc_w;
-    total[bz] = tot;
-    keptw[bz] = kw;
+    disc[bz] = ldexp(disc_w, 2 * e);
+    total[bz] = ldexp(tot, 2 * e);
+    keptw[bz] = ldexp(kw, 2 * e);
     status[bz] = st;
   }
 }
@@ -961,7 +1024,7 @@ int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
   Q = ptr;
   // small per-problem arrays packed in one slot
   const size_t small =
-      (size_t)batch * (p * sizeof(int) + 6 * sizeof(int) + 6 * sizeof(double) +
+      (size_t)batch * (p * sizeof(int) + 7 * sizeof(int) + 6 * sizeof(double) +
                        2 * sizeof(int64_t)) +
       1024;
   QTN_TRY(ctx->buffer(slot_base + 2, small, &ptr));
@@ -993,6 +1056,8 @@ int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
   want_w = (int*)c;
   c += batch * sizeof(int);
   mask = (int*)c;
+  c += batch * sizeof(int);
+  scale_e = (int*)c;
   c += batch * sizeof(int);
   misc = (int*)c;  // [0] = n_active, [1] = nonfinite flag
   QTN_TRY(ctx->buffer(slot_base + 3, (size_t)batch * cols_p * sizeof(double),
@@ -1199,7 +1264,12 @@ int32_t JacobiEngine::run_t(qtn_ctx* ctx, typename CType<R>::type* X,
   hipLaunchKernelGGL(jac_prepare<R>, dim3(batch), dim3(256), 0, s,
                      (const c_t*)X, rows_p, ldx, cols_p, sX, zrel, normF2,
                      zthr, active, counters, misc + 1,
-                     (const int*)(mask_host ? mask : nullptr));
+                     (const int*)(mask_host ? mask : nullptr), scale_e);
+  QTN_CHECK_HIP(hipGetLastError());
+  const unsigned sgrid = (unsigned)std::max<int64_t>(
+      1, std::min<int64_t>((rows_p * cols_p + 255) / 256, 2048));
+  hipLaunchKernelGGL(jac_scale<R>, dim3(sgrid, batch), dim3(256), 0, s, X,
+                     rows_p, ldx, cols_p, sX, (const int*)scale_e, -1);
   QTN_CHECK_HIP(hipGetLastError());
   if (Wm) {
     // W starts as the identity on the padded columns.
@@ -1253,14 +1323,18 @@ int32_t JacobiEngine::run_t(qtn_ctx* ctx, typename CType<R>::type* X,
   hipLaunchKernelGGL(jac_colnorms<R>,
                      dim3((unsigned)((cols_p + 3) / 4), batch), dim3(256), 0,
                      s, (const c_t*)X, rows_p, ldx, cols_p, sX, sig,
-                     (const double*)(dbl ? nullptr : zthr));
+                     (const double*)(dbl ? nullptr : zthr),
+                     (const int*)scale_e);
+  hipLaunchKernelGGL(jac_scale<R>, dim3(sgrid, batch), dim3(256), 0, s, X,
+                     rows_p, ldx, cols_p, sX, (const int*)scale_e, 1);
+  QTN_CHECK_HIP(hipGetLastError());
   if (cols_p > kSortMax) {
     set_error("svd: more than 4096 columns are not supported");
     return QTN_ERR_ARG;
   }
   hipLaunchKernelGGL(jac_sort_trunc, dim3(batch), dim3(1024), 0, s, sig, perm,
                      cols_p, (const int64_t*)nsv, cutoff, relative, chimax,
-                     kept, disc, total, keptw, status);
+                     kept, disc, total, keptw, status, (const int*)scale_e);
   QTN_CHECK_HIP(hipGetLastError());
   return QTN_OK;
 }

@@ -44,6 +44,7 @@ struct JacobiEngine {
   int* status = nullptr;
   int* want_w = nullptr;
   int* mask = nullptr;  // problems to solve (host mask of run*)
+  int* scale_e = nullptr;  // X is swept as X * 2^-scale_e (see jac_prepare)
   int* misc = nullptr;
   double* sig = nullptr;  // [batch][cols_p], sorted descending after run
   qtn_ctx* ctx_ = nullptr;

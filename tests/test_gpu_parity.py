@@ -276,3 +276,20 @@ def test_qr_preconditioned_jacobi_against_oracle(gpu_ctx, monkeypatch):
         a, ref = d.amplitudes(bits), st.amplitudes(bits)
         assert np.linalg.norm(a - ref) <= TOL * np.linalg.norm(ref)
         assert np.abs(d.expect_z() - st.expect_z()).max() <= TOL
+
+
+@pytest.mark.parametrize("scale", [1e160, 1e-160])
+def test_svd_extreme_magnitudes(gpu_ctx, scale):
+    """Squared Gram entries of X = 1e160 * A would overflow: the engine
+    sweeps a power-of-two rescaled copy, so the singular values come out
+    exact relative to the scale (regression for a 64-site MPS whose
+    unnormalised carry reached 1e92)."""
+    rng = np.random.default_rng(12)
+    A = crand(rng, 300, 200)
+    ref = np.linalg.svd(A, compute_uv=False)
+    S = ops.svd_values(A * scale)
+    assert np.abs(S / scale - ref).max() <= 1e-12 * ref[0]
+    U, S2, Vh, _ = ops.svd_trunc(A * scale, 1e-10, True, 0)
+    assert np.abs(S2 / scale - ref[:S2.size]).max() <= 1e-12 * ref[0]
+    R = (U * (S2 / scale)) @ Vh
+    assert np.linalg.norm(R - A) <= 1e-11 * np.linalg.norm(A)

@@ -383,3 +383,17 @@ def test_zipup_exact_and_truncated():
         assert max(b.shape[2] for b in B) <= chi
         fids.append(abs(np.vdot(phi, Hpsi)) / np.linalg.norm(Hpsi))
     assert all(a < b for a, b in zip(fids, fids[1:])) and fids[-1] < 1
+
+
+def test_expect_mpo_blas_equals_einsum_form_and_product_mpo():
+    import sys, os
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..",
+                                    "qaintensor.jl_amd"))
+    from qaintensor import hamiltonians as Hm
+    n = 9
+    h = Hm.tfim_fields(n, 3)
+    W = O.tfim_mpo(n, 1.0, h)
+    Wp = Hm.tfim_mpo(n, 1.0, h)
+    assert all(np.array_equal(a, b) for a, b in zip(W, Wp))
+    Bs = O.left_canonical(Hm.random_mps_tensors(n, 16, 2))
+    assert abs(O.expect_mpo_blas(Bs, W) - O.expect_mpo(Bs, W)) < 1e-12
