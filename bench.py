@@ -48,36 +48,58 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
 
-def roofline(prof, fp32=False):
-    """Roofline of the SVD kernel with the largest summed time in the timed
-    region (HIP events on the library stream), from its algorithmic bytes
-    and flops (counted by the library per launch):
-      jac_round  -- the fused round (Gram + rotations + update of one block
-                    pair per workgroup): both fractions are computed and the
-                    larger one names the bound
-      jac_inner  -- HBM (partial Grams in, rotation block out)
-      jac_gram / jac_update -- MFMA GEMMs
-    MFMA peak: FP64 78.6 TF, or FP32 157.3 TF in the chosen form."""
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summaries (lines 'traffic_bytes_per_launch <kernel> <bytes>' in
+    profiles/*.md, FETCH_SIZE + WRITE_SIZE scaled by a copy calibration),
+    or None."""
+    if not os.path.isdir(PROFILES):
+        return None
+    for fn in sorted(os.listdir(PROFILES)):
+        if not fn.endswith(".md"):
+            continue
+        with open(os.path.join(PROFILES, fn)) as f:
+            for line in f:
+                parts = line.split()
+                if (len(parts) == 3 and parts[0] == "traffic_bytes_per_launch"
+                        and parts[1] == kernel):
+                    return float(parts[2])
+    return None
+
+
+def roofline(prof):
+    """Roofline of the SVD kernel with the largest summed time in the
+    profiled pass (start / stop events stamped by each dispatch on the
+    library stream), from the algorithmic bytes and flops the library
+    counts per launch:
+      jac_inner[_fp32]  -- HBM (partial Grams in, rotation block out)
+      jac_gram / jac_update[_fp32] -- MFMA GEMMs (FP64 78.6 TF, FP32 157.3)
+      jac_round -- the fused round: the larger fraction names the bound"""
     name = max(prof, key=lambda k: prof[k]["ms"])
     p = prof[name]
     n = max(1, p["count"])
     avg_s = p["ms"] / n / 1e3
-    peak = FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS
+    peak = FP32_PEAK_TFLOPS if name.endswith("_fp32") else FP64_PEAK_TFLOPS
     gbs = p["bytes"] / n / avg_s / 1e9
     tfs = p["flops"] / n / avg_s / 1e12
+    traffic = measured_traffic(name)
     hbm = {"kernel": name, "bound": "hbm", "achieved": gbs,
            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-           "traffic": None, "avg_launch_us": avg_s * 1e6, "launches": n}
+           "traffic": traffic, "avg_launch_us": avg_s * 1e6, "launches": n,
+           "algorithmic_bytes_per_launch": p["bytes"] / n}
     mfma = {"kernel": name, "bound": "mfma", "achieved": tfs, "peak": peak,
-            "unit": "TFLOP/s", "frac": tfs / peak, "traffic": None,
+            "unit": "TFLOP/s", "frac": tfs / peak, "traffic": traffic,
             "avg_launch_us": avg_s * 1e6, "launches": n}
-    if name == "jac_inner":
+    if name.startswith("jac_inner"):
         return hbm
     if name == "jac_round":
         out = hbm if hbm["frac"] >= mfma["frac"] else mfma
-        out["other"] = {"bound": "mfma" if out is hbm else "hbm",
-                        "achieved": (mfma if out is hbm else hbm)["achieved"],
-                        "frac": (mfma if out is hbm else hbm)["frac"]}
+        other = mfma if out is hbm else hbm
+        out["other"] = {"bound": other["bound"],
+                        "achieved": other["achieved"], "frac": other["frac"]}
         return out
     return mfma
 
@@ -194,6 +216,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="circuit",
                     choices=["circuit", "tfim-zipup"])
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="after the run, copy a MIB-MiB device buffer 4 "
+                         "times (a kernel of known traffic for calibrating "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--only-batched", action="store_true",
                     help="time the headline form only (for profiler runs)")
     args = ap.parse_args()
@@ -230,31 +256,35 @@ def main():
         torch.cuda.synchronize()
         plain_s = (time.perf_counter() - t0) / args.plain_steps
 
-    def timed(form, profile):
+    def timed(form):
         for _ in range(args.warmup):
             run_circuit(dev, pristine, layers, batched=True, form=form)
         torch.cuda.synchronize()
         ctx.counters(reset=True)
-        if profile:
-            ctx.profile(True)
         t0 = time.perf_counter()
         for _ in range(args.steps):
             run_circuit(dev, pristine, layers, batched=True, form=form)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / args.steps
-        prof = ctx.profile_read() if profile else None
+        return dt, ctx.counters(), infidelity(ref, dev)
+
+    def profiled(form):
+        """One more circuit with per-dispatch kernel timing on (outside
+        the timed region)."""
+        ctx.profile(True)
+        run_circuit(dev, pristine, layers, batched=True, form=form)
+        torch.cuda.synchronize()
+        out = ctx.profile_read()
         ctx.profile(False)
-        return dt, prof, ctx.counters(), infidelity(ref, dev)
+        return out
 
     # batched FP64 form
-    batched_s, prof64, cnt, inf_batched = timed(FORM_PLAIN,
-                                                not args.only_batched)
+    batched_s, cnt, inf_batched = timed(FORM_PLAIN)
     # chosen form (FP32 sweeps, FP64 re-orthogonalisation): the headline
     # when it stays within 1e-6 infidelity of the FP64 state
-    chosen_s, prof, cnt32, inf_chosen = timed(FORM_CHOSEN, True)
+    chosen_s, cnt32, inf_chosen = timed(FORM_CHOSEN)
     use_chosen = inf_chosen <= 1e-6
-    if not use_chosen:
-        prof = prof64
+    prof = profiled(FORM_CHOSEN if use_chosen else FORM_PLAIN)
 
     # algorithmic flops of the circuit from the plain run's bond dims
     # replay the bond dimensions: kept ranks of every gate in order
@@ -310,7 +340,7 @@ def main():
                        "vs_plain": (None if plain_rate is None else
                                     chosen_rate / plain_rate)},
         },
-        "roofline": roofline(prof, fp32=use_chosen),
+        "roofline": roofline(prof),
         "pipeline_roofline": {
             "bound": "mfma", "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -323,6 +353,14 @@ def main():
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(layers)
+    if args.copy_probe:
+        nb = args.copy_probe * (1 << 20)
+        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        line["copy_probe_bytes"] = nb
     print(json.dumps(line))
 
 

@@ -73,8 +73,9 @@ int32_t qtn_ctx_counters(qtn_ctx* ctx, int64_t* out, int32_t n,
                          int32_t reset);
 /* Per-kernel timing with HIP events on the context stream (off by default).
  * Kernel ids: 0 jac_inner, 1 jac_gram, 2 jac_update (the three-launch
- * block Jacobi round), 3 jac_round (the fused round used for 32-column
- * block pairs).  qtn_ctx_profile(ctx, 1) enables and clears the accumulators.
+ * block Jacobi round, FP64 sweeps), 3 jac_round (the opt-in fused round),
+ * 4 jac_inner, 5 jac_gram, 6 jac_update (FP32 sweeps of the chosen form).
+ * Times come from start / stop events stamped by each dispatch.  qtn_ctx_profile(ctx, 1) enables and clears the accumulators.
  * qtn_ctx_profile_read returns, for one kernel id, the launch count, the
  * summed launch durations (ms), and the algorithmic HBM bytes and flops of
  * those launches (counted for the problems active in each launch).        */

@@ -99,12 +99,14 @@ struct qtn_ctx {
   // work counters (see qtn_ctx_counters)
   int64_t n_svd = 0, n_sweeps = 0, max_sweeps = 0, n_batches = 0;
   // per-kernel event timing (qtn_ctx_profile)
-  static constexpr int kProfKernels = 4;
+  // ids: 0 jac_inner, 1 jac_gram, 2 jac_update, 3 jac_round (FP64 sweeps),
+  // 4 jac_inner, 5 jac_gram, 6 jac_update (FP32 sweeps of the chosen form)
+  static constexpr int kProfKernels = 7;
   bool prof = false;
-  int64_t prof_count[kProfKernels] = {0, 0, 0, 0};
-  double prof_ms[kProfKernels] = {0, 0, 0, 0};
-  double prof_bytes[kProfKernels] = {0, 0, 0, 0};
-  double prof_flops[kProfKernels] = {0, 0, 0, 0};
+  int64_t prof_count[kProfKernels] = {};
+  double prof_ms[kProfKernels] = {};
+  double prof_bytes[kProfKernels] = {};
+  double prof_flops[kProfKernels] = {};
   std::vector<hipEvent_t> ev_pool;  // recycled events
   // Small pinned host staging buffer for scalar read-backs.
   void* pinned = nullptr;

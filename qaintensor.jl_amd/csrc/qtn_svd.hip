@@ -26,6 +26,8 @@
 #include <cmath>
 #include <cstdlib>
 
+#include <hip/hip_ext.h>
+
 #include "qtn_internal.hpp"
 #include "qtn_svd.hpp"
 
@@ -1079,9 +1081,12 @@ int32_t JacobiEngine::launch_round(hipStream_t s, void* Xv, int64_t ldx,
   c_t* Wm = (c_t*)Wv;
   c_t* G = (c_t*)Gp;
   c_t* Qc = (c_t*)Q;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // While profiling, every launch carries its own start / stop events,
+  // stamped by the dispatch itself (hipExtLaunchKernelGGL), so the times
+  // are kernel execution times without the gaps between launches.
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   if (ctx_ && ctx_->prof) {
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 6; ++i) {
       if (ctx_->ev_pool.empty()) {
         hipEvent_t e;
         QTN_CHECK_HIP(hipEventCreate(&e));
@@ -1092,35 +1097,36 @@ int32_t JacobiEngine::launch_round(hipStream_t s, void* Xv, int64_t ldx,
       prof_events.push_back(ev[i]);
     }
   }
-  if (ev[0]) QTN_CHECK_HIP(hipEventRecord(ev[0], s));
+#define QTN_TIMED_LAUNCH(e0, e1, kern, grid, ...)                           \
+  do {                                                                      \
+    if (e0)                                                                 \
+      hipExtLaunchKernelGGL(kern, grid, dim3(256), 0, s, e0, e1, 0,         \
+                            __VA_ARGS__);                                   \
+    else                                                                    \
+      hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, __VA_ARGS__);         \
+  } while (0)
   if (fused && WW == 32) {
-    hipLaunchKernelGGL((jac_round<R, 32>), dim3(p, batch), dim3(256), 0, s, X,
-                       ldx, sX, nchunk_x, Wm, ldw ? ldw : cols_p, sW, nchunk_w,
-                       (const int*)(per_problem_w ? want_w : nullptr), p, r,
-                       tol, zthr, n_inner, counters, active, offmax,
-                       misc + 2);
-    if (ev[1]) QTN_CHECK_HIP(hipEventRecord(ev[1], s));
-    if (ev[2]) QTN_CHECK_HIP(hipEventRecord(ev[2], s));
-    if (ev[3]) QTN_CHECK_HIP(hipEventRecord(ev[3], s));
+    QTN_TIMED_LAUNCH(ev[0], ev[1], (jac_round<R, 32>), dim3(p, batch), X, ldx,
+                     sX, nchunk_x, Wm, ldw ? ldw : cols_p, sW, nchunk_w,
+                     (const int*)(per_problem_w ? want_w : nullptr), p, r,
+                     tol, zthr, n_inner, counters, active, offmax, misc + 2);
     return QTN_OK;
   }
-  hipLaunchKernelGGL((jac_gram<R, WW>), dim3(nchunk_x, p, batch), dim3(256),
-                     0, s, (const c_t*)X, ldx, sX, p, r, G, active);
-  if (ev[1]) QTN_CHECK_HIP(hipEventRecord(ev[1], s));
-  hipLaunchKernelGGL((jac_inner<R, WW>), dim3(p, batch), dim3(256), 0, s,
-                     (const c_t*)G, nchunk_x, p, tol, zthr, n_inner, Qc,
-                     flags, counters, active, offmax, misc + 2);
-  if (ev[2]) QTN_CHECK_HIP(hipEventRecord(ev[2], s));
-  hipLaunchKernelGGL((jac_update<R, WW>), dim3(nchunk_x, p, batch),
-                     dim3(256), 0, s, X, ldx, sX, p, r, (const c_t*)Qc, flags,
-                     active, (const int*)nullptr);
+  QTN_TIMED_LAUNCH(ev[0], ev[1], (jac_gram<R, WW>), dim3(nchunk_x, p, batch),
+                   (const c_t*)X, ldx, sX, p, r, G, active);
+  QTN_TIMED_LAUNCH(ev[2], ev[3], (jac_inner<R, WW>), dim3(p, batch),
+                   (const c_t*)G, nchunk_x, p, tol, zthr, n_inner, Qc, flags,
+                   counters, active, offmax, misc + 2);
+  QTN_TIMED_LAUNCH(ev[4], ev[5], (jac_update<R, WW>),
+                   dim3(nchunk_x, p, batch), X, ldx, sX, p, r,
+                   (const c_t*)Qc, flags, active, (const int*)nullptr);
+#undef QTN_TIMED_LAUNCH
   if (Wm)
     hipLaunchKernelGGL((jac_update<R, WW>), dim3(nchunk_w, p, batch),
                        dim3(256), 0, s, Wm, ldw ? ldw : cols_p, sW, p, r,
                        (const c_t*)Qc,
                        flags, active,
                        (const int*)(per_problem_w ? want_w : nullptr));
-  if (ev[3]) QTN_CHECK_HIP(hipEventRecord(ev[3], s));
   return QTN_OK;
 }
 
@@ -1139,7 +1145,7 @@ int32_t JacobiEngine::account_sweep(int n_active, int qpairs, int nchunk) {
     // written once more for the update
     const double Wd = W, c16 = single ? 8.0 : 16.0;
     const double rows = (double)nchunk * 64.0;
-    for (size_t i = 0; i + 3 < prof_events.size(); i += 4) {
+    for (size_t i = 0; i + 5 < prof_events.size(); i += 6) {
       float t01;
       QTN_CHECK_HIP(hipEventElapsedTime(&t01, prof_events[i],
                                         prof_events[i + 1]));
@@ -1156,30 +1162,33 @@ int32_t JacobiEngine::account_sweep(int n_active, int qpairs, int nchunk) {
   }
   const double Wd = W, c16 = single ? 8.0 : 16.0;
   const double rounds = 2.0 * p - 1.0;
+  const int ki = single ? 4 : 0, kg = single ? 5 : 1, ku = single ? 6 : 2;
   const double gram_f = 0.75 * 8.0 * 64.0 * Wd * Wd;
   const double gram_b = (64.0 * Wd + 0.75 * Wd * Wd) * c16;
   const double inner_b = (nchunk * 0.75 * Wd * Wd + Wd * Wd) * c16;
   const double upd_f = 8.0 * 64.0 * Wd * Wd;
   const double upd_b = (2.0 * 64.0 * Wd + Wd * Wd) * c16;
   const double pairs_per_launch = (double)n_active * p;
-  for (size_t i = 0; i + 3 < prof_events.size(); i += 4) {
+  // per round: (start, stop) of jac_gram, jac_inner and the X update
+  // (the W update, when there is one, is not timed)
+  for (size_t i = 0; i + 5 < prof_events.size(); i += 6) {
     float t01, t12, t23;
     QTN_CHECK_HIP(hipEventElapsedTime(&t01, prof_events[i], prof_events[i + 1]));
-    QTN_CHECK_HIP(hipEventElapsedTime(&t12, prof_events[i + 1], prof_events[i + 2]));
-    QTN_CHECK_HIP(hipEventElapsedTime(&t23, prof_events[i + 2], prof_events[i + 3]));
-    ctx->prof_count[1] += 1;
-    ctx->prof_ms[1] += t01;
-    ctx->prof_flops[1] += pairs_per_launch * nchunk * gram_f;
-    ctx->prof_bytes[1] += pairs_per_launch * nchunk * gram_b;
-    ctx->prof_count[0] += 1;
-    ctx->prof_ms[0] += t12;
-    ctx->prof_bytes[0] += pairs_per_launch * inner_b;
-    ctx->prof_count[2] += 1;
-    ctx->prof_ms[2] += t23;
+    QTN_CHECK_HIP(hipEventElapsedTime(&t12, prof_events[i + 2], prof_events[i + 3]));
+    QTN_CHECK_HIP(hipEventElapsedTime(&t23, prof_events[i + 4], prof_events[i + 5]));
+    ctx->prof_count[kg] += 1;
+    ctx->prof_ms[kg] += t01;
+    ctx->prof_flops[kg] += pairs_per_launch * nchunk * gram_f;
+    ctx->prof_bytes[kg] += pairs_per_launch * nchunk * gram_b;
+    ctx->prof_count[ki] += 1;
+    ctx->prof_ms[ki] += t12;
+    ctx->prof_bytes[ki] += pairs_per_launch * inner_b;
+    ctx->prof_count[ku] += 1;
+    ctx->prof_ms[ku] += t23;
   }
   // rotated pairs are only known per sweep: spread over its launches
-  ctx->prof_flops[2] += (double)qpairs * nchunk * upd_f;
-  ctx->prof_bytes[2] += (double)qpairs * nchunk * upd_b;
+  ctx->prof_flops[ku] += (double)qpairs * nchunk * upd_f;
+  ctx->prof_bytes[ku] += (double)qpairs * nchunk * upd_b;
   (void)rounds;
   for (auto e : prof_events) ctx->ev_pool.push_back(e);
   prof_events.clear();

@@ -48,7 +48,7 @@ struct JacobiEngine {
   int* misc = nullptr;
   double* sig = nullptr;  // [batch][cols_p], sorted descending after run
   qtn_ctx* ctx_ = nullptr;
-  std::vector<hipEvent_t> prof_events;  // 4 per round while profiling
+  std::vector<hipEvent_t> prof_events;  // 6 per round while profiling
   int32_t account_sweep(int n_active, int qpairs, int nchunk);
   int* perm = nullptr;    // [batch][cols_p], column of X for each sigma
 

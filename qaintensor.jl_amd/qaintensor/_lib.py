@@ -208,7 +208,9 @@ class Context:
         return dict(svd=int(out[0]), sweeps=int(out[1]),
                     max_sweeps=int(out[2]), batches=int(out[3]))
 
-    PROFILED_KERNELS = ("jac_inner", "jac_gram", "jac_update", "jac_round")
+    PROFILED_KERNELS = ("jac_inner", "jac_gram", "jac_update", "jac_round",
+                        "jac_inner_fp32", "jac_gram_fp32",
+                        "jac_update_fp32")
 
     def profile(self, enable=True):
         """Enable (and clear) or disable per-kernel HIP-event timing."""
