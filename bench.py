@@ -16,6 +16,7 @@ stays resident in HBM.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
        python bench.py --workload tfim-zipup   (64-site TFIM MPO x MPS)
+       python bench.py --workload circuits-batch   (256 mixed circuits)
 """
 import argparse
 import json
@@ -207,6 +208,100 @@ def bench_tfim(args):
     print(json.dumps(line))
 
 
+def bench_batch(args):
+    """256 independent seeded brick-wall circuits, n ~ U[16, 48], depth ~
+    U[8, 24], chi_max ~ {32, 64, 128}, cutoff 1e-8, all advanced together:
+    at every step the layer-t gates of all circuits go through
+    qtn_mps_apply_1q_multi and qtn_mps_apply_layer_multi (Jacobi batches
+    grouped by size).  Reports circuits/s and p50 / p99 step (layer) times
+    for the plain FP64 form and the chosen form; the chosen form's
+    infidelity is the largest over the circuits.  The CPU leg is the
+    oracle on the first 8 circuits."""
+    import torch
+    from qaintensor import (DeviceMPS, FORM_CHOSEN, FORM_PLAIN,
+                            circuits as C, default_context, run_circuits)
+    ctx = default_context()
+    cutoff, count, seed = 1e-8, 256, 3
+    circs = C.batch_circuits(count, seed)
+    layers = [c[3] for c in circs]
+    chis = [c[2] for c in circs]
+    n2q = sum(C.count_2q(l) for l in layers)
+
+    def fresh():
+        return [DeviceMPS(n, chi, ctx) for (n, _, chi, _) in circs]
+
+    def timed(form):
+        for _ in range(args.warmup):
+            run_circuits(fresh(), layers, chis, cutoff, form)
+        torch.cuda.synchronize()
+        best, states, steps = None, None, None
+        for _ in range(max(1, args.steps)):
+            st = fresh()
+            torch.cuda.synchronize()
+            times = []
+            t0 = time.perf_counter()
+            run_circuits(st, layers, chis, cutoff, form, step_times=times)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            if best is None or dt < best:
+                best, states, steps = dt, st, times
+        return best, states, steps
+
+    t_plain, s_plain, st_plain = timed(FORM_PLAIN)
+    print("plain %.2f s" % t_plain, file=sys.stderr, flush=True)
+    t_ch, s_ch, st_ch = timed(FORM_CHOSEN)
+    print("chosen %.2f s" % t_ch, file=sys.stderr, flush=True)
+    inf = max(1 - abs(a.overlap(b)) ** 2 / (a.norm2() * b.norm2())
+              for a, b in zip(s_plain, s_ch))
+    use_chosen = inf <= 1e-6
+    t_head = t_ch if use_chosen else t_plain
+    steps = st_ch if use_chosen else st_plain
+
+    def pct(v, q):
+        return float(np.percentile(np.asarray(v) * 1e3, q))
+
+    line = {
+        "metric": "circuits/s, 256 mixed brick-wall circuits batched",
+        "value": count / t_head, "unit": "circuits/s", "n_gpus": 1,
+        "steps": max(1, args.steps), "warmup": args.warmup,
+        "ms_per_step": t_head * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64-sweeps/c128-core" if use_chosen else "c128",
+        "data": "synthetic (seeded Haar brick-wall circuits)",
+        "config": {"workload": "256 circuits, n~U[16,48], depth~U[8,24], "
+                               "chi_max~{32,64,128}, cutoff 1e-8",
+                   "two_qubit_gates": n2q,
+                   "form": "chosen" if use_chosen else "plain"},
+        "forms": {
+            "plain": {"circuits_per_s": count / t_plain,
+                      "gates_per_s": n2q / t_plain,
+                      "layer_ms_p50": pct(st_plain, 50),
+                      "layer_ms_p99": pct(st_plain, 99)},
+            "chosen": {"circuits_per_s": count / t_ch,
+                       "gates_per_s": n2q / t_ch,
+                       "layer_ms_p50": pct(st_ch, 50),
+                       "layer_ms_p99": pct(st_ch, 99),
+                       "max_infidelity_vs_fp64": inf},
+        },
+        "layer_ms_p50": pct(steps, 50), "layer_ms_p99": pct(steps, 99),
+    }
+    if not args.no_cpu:
+        from oracle import qtn_oracle as O
+        t0 = time.perf_counter()
+        for (n, _, chi, lay) in circs[:8]:
+            O.run_layers(O.TebdState(n), lay, cutoff, chi)
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {
+            "value": 8 / dt, "unit": "circuits/s",
+            "cores": int(os.environ.get("OPENBLAS_NUM_THREADS",
+                                        os.environ.get("OMP_NUM_THREADS",
+                                                       "1"))),
+            "kind": "port",
+            "sample": "oracle on the first 8 circuits of the batch, "
+                      "%.1f s" % dt}
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,7 +310,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="circuit",
-                    choices=["circuit", "tfim-zipup"])
+                    choices=["circuit", "tfim-zipup", "circuits-batch"])
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, copy a MIB-MiB device buffer 4 "
                          "times (a kernel of known traffic for calibrating "
@@ -230,6 +325,9 @@ def main():
         sys.exit(2)
     if args.workload == "tfim-zipup":
         bench_tfim(args)
+        return
+    if args.workload == "circuits-batch":
+        bench_batch(args)
         return
 
     import torch

@@ -51,6 +51,7 @@ extern "C" {
 /* precision forms of the two-site update */
 #define QTN_FORM_PLAIN      0  /* ComplexF64 end to end                       */
 #define QTN_FORM_CHOSEN     1  /* complex FP32 bulk, FP64 core (see docs)     */
+#define QTN_FORM_LEARNED    2  /* rank-hinted randomized SVD, full fallback   */
 
 typedef struct qtn_ctx qtn_ctx;
 typedef struct qtn_mps qtn_mps;
@@ -221,6 +222,35 @@ int32_t qtn_mps_apply_layer(qtn_mps* mps, int32_t ngates,
                             double cutoff, int64_t chimax, int32_t form,
                             int64_t* kept, double* disc_weight);
 
+/* A layer as qtn_mps_apply_layer in the learned form: gate i whose
+ * predicted rank rank_hint[i] lies in (0, chimax) and whose sketch
+ * (rank_hint[i] + 16 columns) is at most 3/4 of the full rank first takes the
+ * randomized range finder (2 power iterations, complex FP64); its exact
+ * discarded weight |theta|^2 - sum of the kept sketch values^2 must meet
+ * the cutoff, else the gate falls back to the full Jacobi SVD.
+ * accepted[i] = 1 for the gates the range finder decided (may be NULL).   */
+int32_t qtn_mps_apply_layer_learned(qtn_mps* mps, int32_t ngates,
+                                    const int32_t* sites, const double* gates,
+                                    double cutoff, int64_t chimax,
+                                    const int64_t* rank_hint, int64_t* kept,
+                                    double* disc_weight, int32_t* accepted);
+/* Many states at once (the circuits of a batch): gate i acts on sites
+ * (sites[i], sites[i]+1) of states[i] with the bond cap chimax[i] (<= 0:
+ * that state's capacity); gates on one state must be disjoint, all states
+ * share one context.  The gates are grouped by the padded size of their
+ * two-site matrix and every group is one batched pass (contraction, gate,
+ * Jacobi SVD, split), as in qtn_mps_apply_layer.  No counterpart in the
+ * reference (it applies gates to one network at a time).                  */
+int32_t qtn_mps_apply_layer_multi(int32_t ngates, qtn_mps* const* states,
+                                  const int32_t* sites, const double* gates,
+                                  const int64_t* chimax, double cutoff,
+                                  int32_t form, int64_t* kept,
+                                  double* disc_weight);
+/* Single-qubit gates us[8*i..] (2x2 column-major) on (states[i],
+ * sites[i]), one launch.                                                   */
+int32_t qtn_mps_apply_1q_multi(int32_t ngates, qtn_mps* const* states,
+                               const int32_t* sites, const double* us);
+
 /* Amplitudes <b|psi> for nb bitstrings; bits is nb x nsites bytes (0/1),
  * row-major per bitstring (bits[i*nsites + w-1] is wire w).               */
 int32_t qtn_mps_amplitudes(const qtn_mps* mps, int64_t nb,
@@ -252,6 +282,29 @@ int32_t qtn_mps_expect_mpo(const qtn_mps* mps, const qtn_mpo* mpo,
 int32_t qtn_mps_apply_mpo(const qtn_mps* in, const qtn_mpo* mpo,
                           double cutoff, int64_t chimax, qtn_mps* out,
                           double* norm);
+
+/* ---- learned rank predictor (no counterpart in the reference) ---------- */
+/* A 3-layer FP32 MLP, y = W3 gelu(W2 gelu(W1 a + b1) + b2) + b3, whose
+ * inputs are columns of nin features (the first min(64, nin) enter as
+ * log10(max(x, 1e-16)) / 8).  params: W1 (width x nin), b1 (width), W2 (width x
+ * width), b2 (width), W3 (1 x width), b3 (1), column-major, concatenated.
+ * Forward, backward (mean squared error) and Adam run in the library's own
+ * kernels on the GPU.                                                      */
+typedef struct qtn_mlp qtn_mlp;
+int32_t qtn_mlp_create(qtn_ctx* ctx, int32_t nin, int32_t width,
+                       const float* params, qtn_mlp** mlp);
+int32_t qtn_mlp_destroy(qtn_mlp* mlp);
+int32_t qtn_mlp_get_params(const qtn_mlp* mlp, float* params);
+/* Y (nb) = mlp(X), X nin x nb column-major (host).                         */
+int32_t qtn_mlp_forward(qtn_mlp* mlp, int64_t nb, const float* X, float* Y);
+/* loss = mean (y - T)^2 and its gradient (same layout as params).          */
+int32_t qtn_mlp_loss_grad(qtn_mlp* mlp, int64_t nb, const float* X,
+                          const float* T, float* loss, float* grad);
+/* `steps` Adam steps (beta 0.9 / 0.999) over contiguous minibatches of the
+ * (pre-shuffled) training set; loss_hist (steps floats, may be NULL).       */
+int32_t qtn_mlp_train(qtn_mlp* mlp, int64_t nb, const float* X,
+                      const float* T, int32_t steps, int64_t minibatch,
+                      float lr, float* loss_hist);
 
 #ifdef __cplusplus
 }

@@ -658,3 +658,83 @@ def zipup(Bs, W, cutoff, chimax):
         C = (S[:kk, None] * Vh[:kk]).reshape((kk, cr, Dr), order="F")
         C = np.transpose(C, (0, 2, 1))
     return canonicalize(out, cutoff, chimax, left_done=True)
+
+
+# ---------------------------------------------------------------------------
+# Rank-predictor MLP (the build's learned part; no counterpart in the
+# reference).  Parameters are one flat float array, column-major blocks:
+#   W1 (width x nin), b1 (width), W2 (width x width), b2 (width),
+#   W3 (1 x width), b3 (1).
+# Inputs are columns x_b (nin); features 0..63 enter as
+# log10(max(x, 1e-16)) / 8.
+def gelu(z):
+    from scipy.special import erf
+    return 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
+
+
+def gelu_grad(z):
+    from scipy.special import erf
+    return 0.5 * (1.0 + erf(z / np.sqrt(2.0))) + \
+        z * np.exp(-0.5 * z * z) / np.sqrt(2.0 * np.pi)
+
+
+def mlp_unpack(p, nin, width):
+    p = np.asarray(p, dtype=np.float64)
+    o = 0
+    W1 = p[o:o + width * nin].reshape((width, nin), order="F"); o += width * nin
+    b1 = p[o:o + width]; o += width
+    W2 = p[o:o + width * width].reshape((width, width), order="F")
+    o += width * width
+    b2 = p[o:o + width]; o += width
+    W3 = p[o:o + width].reshape((1, width), order="F"); o += width
+    b3 = p[o:o + 1]; o += 1
+    assert o == p.size
+    return W1, b1, W2, b2, W3, b3
+
+
+def mlp_nparams(nin, width):
+    return width * nin + width + width * width + width + width + 1
+
+
+def mlp_prepare(X, nlog=64):
+    X = np.array(X, dtype=np.float64)
+    X[:nlog] = np.log10(np.maximum(X[:nlog], 1e-16)) / 8.0
+    return X
+
+
+def mlp_forward(p, X, nin, width):
+    """Y (1 x B) for inputs X (nin x B), float64 arithmetic."""
+    W1, b1, W2, b2, W3, b3 = mlp_unpack(p, nin, width)
+    A0 = mlp_prepare(X)
+    Z1 = W1 @ A0 + b1[:, None]
+    A1 = gelu(Z1)
+    Z2 = W2 @ A1 + b2[:, None]
+    A2 = gelu(Z2)
+    return W3 @ A2 + b3[:, None]
+
+
+def mlp_loss_grad(p, X, T, nin, width):
+    """(loss, flat gradient) of the mean squared error 1/B sum (y - t)^2."""
+    W1, b1, W2, b2, W3, b3 = mlp_unpack(p, nin, width)
+    B = X.shape[1]
+    A0 = mlp_prepare(X)
+    Z1 = W1 @ A0 + b1[:, None]
+    A1 = gelu(Z1)
+    Z2 = W2 @ A1 + b2[:, None]
+    A2 = gelu(Z2)
+    Y = W3 @ A2 + b3[:, None]
+    R = Y - np.asarray(T, dtype=np.float64).reshape(1, -1)
+    loss = float(np.sum(R * R) / B)
+    dY = 2.0 * R / B
+    dW3 = dY @ A2.T
+    db3 = dY.sum(axis=1)
+    dZ2 = (W3.T @ dY) * gelu_grad(Z2)
+    dW2 = dZ2 @ A1.T
+    db2 = dZ2.sum(axis=1)
+    dZ1 = (W2.T @ dZ2) * gelu_grad(Z1)
+    dW1 = dZ1 @ A0.T
+    db1 = dZ1.sum(axis=1)
+    g = np.concatenate([dW1.reshape(-1, order="F"), db1,
+                        dW2.reshape(-1, order="F"), db2,
+                        dW3.reshape(-1, order="F"), db3])
+    return loss, g

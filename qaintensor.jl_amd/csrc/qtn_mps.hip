@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "qtn_internal.hpp"
@@ -26,6 +27,7 @@
 namespace qtn {
 namespace {
 
+constexpr int kSlot1q = 47;
 constexpr int kSlotPhi = 30, kSlotXm = 31, kSlotVk = 32, kSlotDesc = 33,
               kSlotGates = 34, kSlotEnv = 35, kSlotT1 = 36, kSlotT2 = 37,
               kSlotRed = 38, kSlotBits = 39, kSlotAmp = 40, kSlotW2 = 41,
@@ -181,6 +183,30 @@ __global__ void apply_1q_kernel(double2* __restrict__ B, int64_t chiL,
   }
 }
 
+// A batch of single-qubit gates, one per descriptor (grid (tiles, batch)).
+struct OneQDesc {
+  double2* B;
+  int64_t chiL, chiR;
+  double2 u[4];
+};
+
+__global__ void apply_1q_multi_kernel(const OneQDesc* __restrict__ dd) {
+  const OneQDesc d = dd[blockIdx.y];
+  const double2 u00 = d.u[0], u10 = d.u[1], u01 = d.u[2], u11 = d.u[3];
+  const int64_t total = d.chiL * d.chiR;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+       idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t a = idx % d.chiL, b = idx / d.chiL;
+    double2* p0 = d.B + a + d.chiL * (0 + 2 * b);
+    double2* p1 = d.B + a + d.chiL * (1 + 2 * b);
+    const double2 x0 = *p0, x1 = *p1;
+    *p0 = make_double2(u00.x * x0.x - u00.y * x0.y + u01.x * x1.x - u01.y * x1.y,
+                       u00.x * x0.y + u00.y * x0.x + u01.x * x1.y + u01.y * x1.x);
+    *p1 = make_double2(u10.x * x0.x - u10.y * x0.y + u11.x * x1.x - u11.y * x1.y,
+                       u10.x * x0.y + u10.y * x0.x + u11.x * x1.y + u11.y * x1.x);
+  }
+}
+
 // Amplitude contraction step: V'[i][c] = sum_a V[i][a] B[a, s_i, c].
 // V is nb x chiL (ld nb), V' is nb x chiR.
 __global__ void amp_step(const double2* __restrict__ V, int64_t nb,
@@ -267,6 +293,7 @@ __global__ void extract_vk_mixed(const GateDesc* __restrict__ gd,
                                  int64_t cols_p, int64_t cols32,
                                  int64_t cols64) {
   const int bz = blockIdx.y;
+  if (src[bz] == 2) return;  // V_k written by the randomized range finder
   const GateDesc d = gd[bz];
   const int64_t k = kept[bz];
   const int64_t N = 2 * d.chiR;
@@ -369,6 +396,144 @@ __global__ __launch_bounds__(256) void left_finish(
       d.lamM[j] = s[j] * inv;
 }
 
+// ---- learned form: randomized range finder ------------------------------
+// Per-gate view of the range finder's operands: theta is M x N; its X is
+// theta^H (trans) or theta, ld ldx.
+struct RandDesc {
+  const double2* X;  // the gate's X
+  int64_t ldx;
+  int trans;
+  int64_t M, N, l;   // theta dims, sketch width
+  double2* Om;       // N x l sketch (ld N)
+  double2* Y;        // M x l (ld M)
+  double2* Z;        // N x l (ld N)
+  double2* Q;        // max(M, N) x l (ld max(M, N)): orthonormal basis
+};
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+// Om = complex Gaussian (N(0,1/2) + i N(0,1/2)) from a counter hash and the
+// Box-Muller transform; deterministic in (seed, gate, element).
+__global__ void gauss_fill(const RandDesc* __restrict__ rd, uint64_t seed) {
+  const int bz = blockIdx.y;
+  const RandDesc d = rd[bz];
+  const int64_t total = d.N * d.l;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t h1 = mix64(seed ^ mix64(((uint64_t)bz << 40) ^ (uint64_t)i));
+    const uint64_t h2 = mix64(h1);
+    const double u1 = ((double)(h1 >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    const double u2 = (double)(h2 >> 11) * (1.0 / 9007199254740992.0);
+    const double r = sqrt(-log(u1));  // sqrt(-2 ln u) / sqrt(2)
+    const double t = 6.283185307179586 * u2;
+    d.Om[i] = make_double2(r * cos(t), r * sin(t));
+  }
+}
+
+// B = [I_l; 0] (rows x l, ld ld) for every gate (rows = M or N by `which`).
+__global__ void eye_fill(const RandDesc* __restrict__ rd, int which) {
+  const int bz = blockIdx.y;
+  const RandDesc d = rd[bz];
+  const int64_t rows = which ? d.N : d.M;
+  const int64_t ld = d.M > d.N ? d.M : d.N;
+  const int64_t total = rows * d.l;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i % rows, c = i / rows;
+    d.Q[r + ld * c] = make_double2(r == c ? 1.0 : 0.0, 0.0);
+  }
+}
+
+// |theta|_F^2 per gate (one workgroup per gate) from X.
+__global__ __launch_bounds__(256) void theta_norm2(
+    const RandDesc* __restrict__ rd, double* __restrict__ out) {
+  const RandDesc d = rd[blockIdx.x];
+  const int64_t rows = d.trans ? d.N : d.M, cols = d.trans ? d.M : d.N;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < rows * cols; i += 256) {
+    const double2 v = d.X[(i % rows) + (i / rows) * d.ldx];
+    acc += v.x * v.x + v.y * v.y;
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// The exact check: with sigma_j the singular values of B = Q^H theta
+// (sorted), the weight theta loses when truncated to the top k of them is
+// exactly |theta|_F^2 - sum_{j<=k} sigma_j^2.  k = the fewest values with
+// that loss <= (cutoff |theta|_F)^2; accepted when such k exists below the
+// sketch width l and the cap chimax, else the gate falls back to the full
+// SVD.  One thread per gate.
+__global__ void rand_decide(int ng, const double* __restrict__ sig,
+                            int64_t cols_p, const double* __restrict__ total,
+                            const int64_t* __restrict__ lw, double cutoff,
+                            const int64_t* __restrict__ chimax,
+                            const int* __restrict__ attempt,
+                            int64_t* __restrict__ kept,
+                            int* __restrict__ accepted,
+                            double* __restrict__ keptw,
+                            double* __restrict__ discw) {
+  const int bz = blockIdx.x * blockDim.x + threadIdx.x;
+  if (bz >= ng) return;
+  accepted[bz] = 0;
+  if (!attempt[bz]) return;
+  const double tot = total[bz];
+  const double er2 = cutoff * cutoff * tot;
+  const double* s = sig + (int64_t)bz * cols_p;
+  const int64_t l = lw[bz];
+  double kw = 0.0;
+  int64_t k = -1;
+  for (int64_t j = 0; j < l && j < chimax[bz]; ++j) {
+    kw += s[j] * s[j];
+    if (tot - kw <= er2) {
+      k = j + 1;
+      break;
+    }
+  }
+  if (k > 0 && k < l) {
+    accepted[bz] = 1;
+    kept[bz] = k;
+    keptw[bz] = kw;
+    discw[bz] = tot - kw > 0.0 ? tot - kw : 0.0;
+  }
+}
+
+// V_k (N x k, ld N) = X_B[:, perm[j]] / sig[j] for accepted gates, X_B =
+// B^H = theta^H Q (N x l, ld ldb).
+__global__ void extract_vk_rand(const GateDesc* __restrict__ gd,
+                                const int* __restrict__ accepted,
+                                const double2* __restrict__ XB, int64_t ldb,
+                                int64_t sB, const double* __restrict__ sig,
+                                const int* __restrict__ perm, int64_t cols_p,
+                                const int64_t* __restrict__ kept) {
+  const int bz = blockIdx.y;
+  if (!accepted[bz]) return;
+  const GateDesc d = gd[bz];
+  const int64_t k = kept[bz];
+  const int64_t N = 2 * d.chiR;
+  const double* s = sig + (int64_t)bz * cols_p;
+  const int* pm = perm + (int64_t)bz * cols_p;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+       idx < N * k; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = idx % N, j = idx / N;
+    const double sj = s[j];
+    const double inv = sj > 0.0 ? 1.0 / sj : 0.0;
+    const double2 x = XB[bz * sB + c + ldb * (int64_t)pm[j]];
+    d.Vk[c + N * j] = make_double2(x.x * inv, x.y * inv);
+  }
+}
+
 __global__ void fill_one(double2* p) { *p = make_double2(1.0, 0.0); }
 __global__ void fill_lambda_one(double* p) { *p = 1.0; }
 
@@ -376,22 +541,246 @@ __global__ void fill_lambda_one(double* p) { *p = 1.0; }
 
 // ---------------------------------------------------------------------------
 // Batched two-site update of disjoint gates (host orchestration).
-int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
-                        const double* gates, double cutoff, int64_t chimax,
-                        int form, int64_t* kept_out, double* disc_out,
-                        double* spec_out, int64_t* spec_len) {
-  const bool chosen = form == QTN_FORM_CHOSEN;
-  qtn_ctx* ctx = m->ctx;
+// Randomized range finder (Halko, Martinsson & Tropp, SIAM Rev. 53, 217
+// (2011), algorithm 4.4: q = 2 power iterations with re-orthonormalisation)
+// for the gates with attempt[g]: Q (M x l) spans theta Omega, B^H = theta^H
+// Q is split by the Jacobi engine, and rand_decide applies the exact
+// discarded-weight check.  Accepted gates get V_k written into their
+// descriptor's Vk; kept / keptw / disc / total are filled for them.
+constexpr int kSlotROm = 140, kSlotRY = 141, kSlotRZ = 142, kSlotRQ = 143,
+              kSlotRDesc = 144, kSlotRSmall = 145, kSlotRDesc2 = 146;
+
+int32_t rand_range_finder(qtn_ctx* ctx, int ng,
+                          const std::vector<GateDesc>& gd,
+                          const GateDesc* d_gd, int64_t ldx,
+                          const std::vector<int>& attempt,
+                          const std::vector<int64_t>& lw, const int64_t* chis,
+                          double cutoff, uint64_t seed,
+                          std::vector<int>& accepted,
+                          std::vector<int64_t>& kept,
+                          std::vector<double>& keptw,
+                          std::vector<double>& disc,
+                          std::vector<double>& total) {
   hipStream_t s = ctx->stream;
-  std::vector<char> used(m->n + 1, 0);
+  int64_t maxM = 1, maxN = 1, maxl = 1;
+  for (int g = 0; g < ng; ++g) {
+    if (!attempt[g]) continue;
+    maxM = std::max(maxM, 2 * gd[g].chiL);
+    maxN = std::max(maxN, 2 * gd[g].chiR);
+    maxl = std::max(maxl, lw[g]);
+  }
+  const int64_t ldq = std::max(maxM, maxN);
+  const int64_t rowsB = round_up64(maxN), colsB = round_up64(maxl);
+  const int64_t om_sz = maxN * maxl, y_sz = maxM * maxl,
+                z_sz = rowsB * colsB, q_sz = ldq * maxl;
+  void *pOm, *pY, *pZ, *pQ, *pD, *pS;
+  QTN_TRY(ctx->buffer(kSlotROm, (size_t)ng * om_sz * sizeof(double2), &pOm));
+  QTN_TRY(ctx->buffer(kSlotRY, (size_t)ng * y_sz * sizeof(double2), &pY));
+  QTN_TRY(ctx->buffer(kSlotRZ, (size_t)ng * z_sz * sizeof(double2), &pZ));
+  QTN_TRY(ctx->buffer(kSlotRQ, (size_t)ng * q_sz * sizeof(double2), &pQ));
+  QTN_TRY(ctx->buffer(kSlotRDesc, (size_t)ng * sizeof(RandDesc), &pD));
+  QTN_TRY(ctx->buffer(kSlotRSmall,
+                      (size_t)ng * (4 * sizeof(int64_t) + 4 * sizeof(double) +
+                                    2 * sizeof(int)),
+                      &pS));
+  std::vector<RandDesc> rd(ng);
+  std::vector<QrDims> qdM(ng), qdN(ng);
+  for (int g = 0; g < ng; ++g) {
+    RandDesc& r = rd[g];
+    r.X = gd[g].X;
+    r.ldx = ldx;
+    r.trans = gd[g].trans;
+    r.M = attempt[g] ? 2 * gd[g].chiL : 0;
+    r.N = attempt[g] ? 2 * gd[g].chiR : 0;
+    r.l = attempt[g] ? lw[g] : 0;
+    r.Om = (double2*)pOm + (size_t)g * om_sz;
+    r.Y = (double2*)pY + (size_t)g * y_sz;
+    r.Z = (double2*)pZ + (size_t)g * z_sz;
+    r.Q = (double2*)pQ + (size_t)g * q_sz;
+    qdM[g] = QrDims{r.M, r.l, attempt[g] ? 1 : 0};
+    qdN[g] = QrDims{r.N, r.l, attempt[g] ? 1 : 0};
+  }
+  RandDesc* d_rd = (RandDesc*)pD;
+  QTN_CHECK_HIP(hipMemcpyAsync(d_rd, rd.data(), ng * sizeof(RandDesc),
+                               hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(gauss_fill, dim3(grid_for(maxN * maxl), (unsigned)ng),
+                     dim3(256), 0, s, (const RandDesc*)d_rd, seed);
+  QTN_CHECK_HIP(hipGetLastError());
+  // descriptors of the four products, split by the orientation of X:
+  //   Y = theta Om | theta Q   (M x l): trans X^H B, else X B
+  //   Z = theta^H Q            (N x l): trans X Q,   else X^H Q
+  void* pG;
+  QTN_TRY(ctx->buffer(kSlotRDesc2, (size_t)10 * ng * sizeof(GemmDesc<double>),
+                      &pG));
+  GemmDesc<double>* dG = (GemmDesc<double>*)pG;
+  // host descriptor arrays live until the stream is synchronised below
+  std::vector<std::vector<GemmDesc<double>>> keep;
+  keep.reserve(16);
+  auto run_products = [&](bool toY, const double2* Bbase, int64_t bstride,
+                          int64_t ldb, double2* Cbase, int64_t cstride,
+                          int64_t ldc, int slot_off) -> int32_t {
+    keep.emplace_back();
+    std::vector<GemmDesc<double>>& gt = keep.back();
+    keep.emplace_back();
+    std::vector<GemmDesc<double>>& gn = keep.back();
+    for (int g = 0; g < ng; ++g) {
+      if (!attempt[g]) continue;
+      const RandDesc& r = rd[g];
+      GemmDesc<double> e;
+      e.M = toY ? r.M : r.N;
+      e.N = r.l;
+      e.K = toY ? r.N : r.M;
+      e.alpha = make_double2(1.0, 0.0);
+      e.beta = make_double2(0.0, 0.0);
+      e.A = r.X;
+      e.lda = ldx;
+      e.B = Bbase + (size_t)g * bstride;
+      e.ldb = ldb;
+      e.C = Cbase + (size_t)g * cstride;
+      e.ldc = ldc;
+      // toY: trans -> X^H (op C); Z: non-trans -> X^H (op C)
+      const bool conj = toY ? r.trans : !r.trans;
+      (conj ? gt : gn).push_back(e);
+    }
+    GemmDesc<double>* d1 = dG + (size_t)slot_off * 2 * ng;
+    GemmDesc<double>* d2 = d1 + ng;
+    const int64_t mm = toY ? maxM : maxN;
+    if (!gt.empty()) {
+      QTN_CHECK_HIP(hipMemcpyAsync(d1, gt.data(),
+                                   gt.size() * sizeof(GemmDesc<double>),
+                                   hipMemcpyHostToDevice, s));
+      QTN_TRY(launch_zgemm_grouped(s, 2, 0, d1, (int)gt.size(), mm, maxl));
+    }
+    if (!gn.empty()) {
+      QTN_CHECK_HIP(hipMemcpyAsync(d2, gn.data(),
+                                   gn.size() * sizeof(GemmDesc<double>),
+                                   hipMemcpyHostToDevice, s));
+      QTN_TRY(launch_zgemm_grouped(s, 0, 0, d2, (int)gn.size(), mm, maxl));
+    }
+    return QTN_OK;
+  };
+  // orthonormal basis of the first l columns of A (rows x l, ld lda) into
+  // Q (ld ldq): Householder QR, then Q [I; 0]
+  auto orth = [&](double2* A, int64_t lda, int64_t stride, int which,
+                  const std::vector<QrDims>& dims) -> int32_t {
+    BatchedQR qr;
+    qr.slot_base = 150;
+    QTN_TRY(qr.factor(ctx, A, lda, stride, maxl, dims, true));
+    hipLaunchKernelGGL(eye_fill, dim3(grid_for(ldq * maxl), (unsigned)ng),
+                       dim3(256), 0, s, (const RandDesc*)d_rd, which);
+    QTN_CHECK_HIP(hipGetLastError());
+    std::vector<int64_t> ldu(ng, ldq), uc(ng);
+    for (int g = 0; g < ng; ++g) uc[g] = attempt[g] ? lw[g] : 0;
+    return qr.apply_q(ctx, (double2*)pQ, ldu, q_sz, uc, dims);
+  };
+  // Y = theta Om
+  QTN_TRY(run_products(true, (const double2*)pOm, om_sz, maxN, (double2*)pY,
+                       y_sz, maxM, 0));
+  for (int it = 0; it < 2; ++it) {
+    QTN_TRY(orth((double2*)pY, maxM, y_sz, 0, qdM));           // Q = orth(Y)
+    QTN_TRY(run_products(false, (const double2*)pQ, q_sz, ldq, (double2*)pZ,
+                         z_sz, rowsB, 1 + 2 * it));             // Z = th^H Q
+    QTN_TRY(orth((double2*)pZ, rowsB, z_sz, 1, qdN));          // Q = orth(Z)
+    QTN_TRY(run_products(true, (const double2*)pQ, q_sz, ldq, (double2*)pY,
+                         y_sz, maxM, 2 + 2 * it));              // Y = th Q
+  }
+  QTN_TRY(orth((double2*)pY, maxM, y_sz, 0, qdM));
+  // B^H = theta^H Q (N x l) into a zeroed rowsB x colsB block per gate
+  QTN_CHECK_HIP(hipMemsetAsync(pZ, 0, (size_t)ng * z_sz * sizeof(double2), s));
+  QTN_TRY(run_products(false, (const double2*)pQ, q_sz, ldq, (double2*)pZ,
+                       z_sz, rowsB, 3));
+  JacobiEngine eng;
+  eng.slot_base = 160;
+  QTN_TRY(eng.reserve(ctx, ng, rowsB, colsB));
+  std::vector<int64_t> nsv(ng);
+  for (int g = 0; g < ng; ++g) nsv[g] = attempt[g] ? lw[g] : 1;
+  int sweeps = 0;
+  QTN_TRY(eng.run(ctx, (double2*)pZ, rowsB, z_sz, nullptr, 0, nullptr,
+                  nsv.data(), 0.0, 1, 0, 60, &sweeps, attempt.data()));
+  int64_t* d_lw = (int64_t*)pS;
+  int64_t* d_ch = d_lw + ng;
+  int64_t* d_kept = d_ch + ng;
+  double* d_tot = (double*)(d_kept + ng);
+  double* d_kw = d_tot + ng;
+  double* d_dw = d_kw + ng;
+  int* d_att = (int*)(d_dw + ng);
+  int* d_acc = d_att + ng;
+  QTN_CHECK_HIP(hipMemcpyAsync(d_lw, lw.data(), ng * sizeof(int64_t),
+                               hipMemcpyHostToDevice, s));
+  QTN_CHECK_HIP(hipMemcpyAsync(d_ch, chis, ng * sizeof(int64_t),
+                               hipMemcpyHostToDevice, s));
+  QTN_CHECK_HIP(hipMemcpyAsync(d_att, attempt.data(), ng * sizeof(int),
+                               hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(theta_norm2, dim3(ng), dim3(256), 0, s,
+                     (const RandDesc*)d_rd, d_tot);
+  hipLaunchKernelGGL(rand_decide, dim3((ng + 63) / 64), dim3(64), 0, s, ng,
+                     (const double*)eng.sig, eng.cols_p,
+                     (const double*)d_tot, (const int64_t*)d_lw, cutoff,
+                     (const int64_t*)d_ch, (const int*)d_att, d_kept, d_acc,
+                     d_kw, d_dw);
+  hipLaunchKernelGGL(extract_vk_rand,
+                     dim3(grid_for(maxN * maxl), (unsigned)ng), dim3(256), 0,
+                     s, d_gd, (const int*)d_acc, (const double2*)pZ, rowsB,
+                     z_sz, (const double*)eng.sig, (const int*)eng.perm,
+                     eng.cols_p, (const int64_t*)d_kept);
+  QTN_CHECK_HIP(hipGetLastError());
+  accepted.assign(ng, 0);
+  std::vector<int64_t> k(ng, 0);
+  std::vector<double> kw(ng, 0.0), dw(ng, 0.0), tt(ng, 0.0);
+  QTN_CHECK_HIP(hipMemcpyAsync(accepted.data(), d_acc, ng * sizeof(int),
+                               hipMemcpyDeviceToHost, s));
+  QTN_CHECK_HIP(hipMemcpyAsync(k.data(), d_kept, ng * sizeof(int64_t),
+                               hipMemcpyDeviceToHost, s));
+  QTN_CHECK_HIP(hipMemcpyAsync(kw.data(), d_kw, ng * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+  QTN_CHECK_HIP(hipMemcpyAsync(dw.data(), d_dw, ng * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+  QTN_CHECK_HIP(hipMemcpyAsync(tt.data(), d_tot, ng * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+  QTN_CHECK_HIP(hipStreamSynchronize(s));
+  for (int g = 0; g < ng; ++g) {
+    if (!accepted[g]) continue;
+    kept[g] = k[g];
+    keptw[g] = kw[g];
+    disc[g] = dw[g];
+    total[g] = tt[g];
+  }
+  return QTN_OK;
+}
+
+// Gate g acts on sites (sites[g], sites[g] + 1) of state ms[g] with the
+// bond cap chis[g]; the states may differ (one batch over many circuits)
+// but share ctx.
+// hint (learned form, may be NULL): predicted kept rank per gate; gates
+// with 0 < hint < cap try the randomized range finder first (sketch width
+// hint + 16) and fall back to the full SVD when its exact check fails;
+// rand_accepted (may be NULL) receives 1 for the gates it decided.
+int32_t apply_gates_core(qtn_ctx* ctx, int ng, qtn_mps* const* ms,
+                         const int32_t* sites, const double* gates,
+                         double cutoff, const int64_t* chis, int form,
+                         int64_t* kept_out, double* disc_out,
+                         double* spec_out, int64_t* spec_len,
+                         const int64_t* hint = nullptr,
+                         int* rand_accepted = nullptr) {
+  const bool chosen = form == QTN_FORM_CHOSEN;
+  const bool learned = form == QTN_FORM_LEARNED;
+  hipStream_t s = ctx->stream;
+  std::map<const qtn_mps*, std::vector<char>> used;
   int64_t maxM = 0, maxN = 0, maxXr = 0, maxXc = 0;
   bool any_w = false;
   std::vector<int> trans(ng);
   for (int g = 0; g < ng; ++g) {
+    const qtn_mps* m = ms[g];
     const int i = sites[g];
+    QTN_ARG(m != nullptr, "apply_layer: NULL state");
+    QTN_ARG(m->ctx == ctx, "apply_layer: states on different contexts");
     QTN_ARG(i >= 1 && i < m->n, "apply_2q: site out of range");
-    QTN_ARG(!used[i] && !used[i + 1], "apply_layer: gates overlap");
-    used[i] = used[i + 1] = 1;
+    QTN_ARG(chis[g] >= 1 && chis[g] <= m->cap,
+            "apply_layer: chimax above capacity");
+    std::vector<char>& u = used[m];
+    if (u.empty()) u.assign(m->n + 1, 0);
+    QTN_ARG(!u[i] && !u[i + 1], "apply_layer: gates overlap");
+    u[i] = u[i + 1] = 1;
     const int64_t M = 2 * m->dims[i - 1], N = 2 * m->dims[i + 1];
     maxM = std::max(maxM, M);
     maxN = std::max(maxN, N);
@@ -408,7 +797,7 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
   // about what it saves at chi = 256; off unless QTN_JACOBI_PRECOND=1.
   bool precond = false;
   if (const char* e = getenv("QTN_JACOBI_PRECOND")) precond = atoi(e) != 0;
-  if (chosen) precond = false;
+  if (chosen || learned) precond = false;
   const int64_t w_sz = cols_p * cols_p;
   void *dphi, *dX, *dVk, *ddesc, *dg, *dW = nullptr;
   if (any_w)
@@ -427,6 +816,7 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
   std::vector<int64_t> nsv(ng);
   for (int g = 0; g < ng; ++g) {
     const int i = sites[g];
+    const qtn_mps* m = ms[g];
     GateDesc& d = gd[g];
     d.chiL = m->dims[i - 1];
     d.chiM = m->dims[i];
@@ -553,7 +943,7 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
     std::vector<int> cand(ng, 0);
     bool any32 = false;
     for (int g = 0; g < ng; ++g) {
-      cand[g] = nsv[g] > chimax ? 1 : 0;
+      cand[g] = nsv[g] > chis[g] ? 1 : 0;
       any32 = any32 || cand[g];
     }
     if (any32) {
@@ -568,15 +958,16 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
                          (int64_t)ng * x_sz);
       QTN_CHECK_HIP(hipGetLastError());
       QTN_TRY(eng32.reserve(ctx, ng, rows_p, cols_p, true));
+      eng32.chimax_per = chis;
       QTN_TRY(eng32.run32(ctx, (float2*)dX32, rows_p, x_sz, (float2*)dW32,
                           w_sz, any_w ? wantw.data() : nullptr, nsv.data(),
-                          cutoff, 1, chimax, 60, &sweeps, cand.data()));
+                          cutoff, 1, 0, 60, &sweeps, cand.data()));
       std::vector<int64_t> k32(ng);
       std::vector<int> st32(ng);
       std::vector<double> d32(ng), t32(ng), kw32(ng);
       QTN_TRY(read_results(eng32, k32, st32, d32, t32, kw32));
       for (int g = 0; g < ng; ++g) {
-        if (cand[g] && st32[g] == 0 && k32[g] == chimax) {
+        if (cand[g] && st32[g] == 0 && k32[g] == chis[g]) {
           src[g] = 1;
           need64[g] = 0;
           kept[g] = k32[g];
@@ -587,12 +978,42 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
       }
     }
   }
+  if (learned && hint) {
+    std::vector<int> attempt(ng, 0);
+    std::vector<int64_t> lw(ng, 0);
+    bool any = false;
+    for (int g = 0; g < ng; ++g) {
+      const int64_t l = std::min<int64_t>(hint[g] + 16, nsv[g]);
+      // worth trying when the predicted rank stays below the cap (the
+      // cutoff can decide it) and the sketch is well below the full rank
+      if (hint[g] > 0 && hint[g] < chis[g] && 4 * l <= 3 * nsv[g]) {
+        attempt[g] = 1;
+        lw[g] = l;
+        any = true;
+      }
+    }
+    if (any) {
+      std::vector<int> acc;
+      static uint64_t seed = 0x5eed;
+      seed = seed * 6364136223846793005ULL + 1442695040888963407ULL;
+      QTN_TRY(rand_range_finder(ctx, ng, gd, d_gd, rows_p, attempt, lw, chis,
+                                cutoff, seed, acc, kept, keptw, disc,
+                                total));
+      for (int g = 0; g < ng; ++g)
+        if (acc[g]) {
+          src[g] = 2;
+          need64[g] = 0;
+        }
+    }
+  }
+  if (rand_accepted)
+    for (int g = 0; g < ng; ++g) rand_accepted[g] = src[g] == 2 ? 1 : 0;
   bool any64 = false;
   for (int g = 0; g < ng; ++g) any64 = any64 || need64[g];
   if (any64) {
     // every problem preconditioned: only the top cols_p rows are non-zero
     int64_t rows_eng = all_pre ? cols_p : rows_p, cols_eng = cols_p;
-    if (chosen) {
+    if (chosen || learned) {
       // the FP64 engine is padded to the gates it solves, not to the batch
       int64_t r64 = 1, c64 = 1;
       for (int g = 0; g < ng; ++g) {
@@ -606,9 +1027,11 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
     }
     eng.ldw = cols_p;  // W keeps the batch layout
     QTN_TRY(eng.reserve(ctx, ng, rows_eng, cols_eng));
+    eng.chimax_per = chis;
     QTN_TRY(eng.run(ctx, (double2*)dX, rows_p, x_sz, (double2*)dW, w_sz,
                     any_w ? wantw.data() : nullptr, nsv.data(), cutoff, 1,
-                    chimax, 60, &sweeps, chosen ? need64.data() : nullptr));
+                    0, 60, &sweeps,
+                    (chosen || learned) ? need64.data() : nullptr));
     std::vector<int64_t> k64(ng);
     std::vector<int> st64(ng);
     std::vector<double> d64(ng), t64(ng), kw64(ng);
@@ -630,14 +1053,20 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
     }
   }
   if (spec_out && ng == 1) {
-    QTN_CHECK_HIP(hipMemcpy(spec_out, src[0] ? eng32.sig : eng.sig,
-                            nsv[0] * sizeof(double), hipMemcpyDeviceToHost));
-    if (spec_len) *spec_len = nsv[0];
+    if (src[0] == 2) {
+      // the range finder only resolves the sketch's values
+      if (spec_len) *spec_len = 0;
+    } else {
+      QTN_CHECK_HIP(hipMemcpy(spec_out, src[0] ? eng32.sig : eng.sig,
+                              nsv[0] * sizeof(double),
+                              hipMemcpyDeviceToHost));
+      if (spec_len) *spec_len = nsv[0];
+    }
   }
   // per-gate kept rank and source engine on the device (chosen form)
   int64_t* d_kept = eng.kept;
   int* d_src = nullptr;
-  if (chosen) {
+  if (chosen || learned) {
     void* pm;
     QTN_TRY(ctx->buffer(kSlotMix, (size_t)ng * (sizeof(int64_t) + sizeof(int)),
                         &pm));
@@ -651,7 +1080,7 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
   // 4. V_k, then B_{i+1} = V_k^H and lambda_i
   int64_t maxk = 0;
   for (int g = 0; g < ng; ++g) maxk = std::max(maxk, kept[g]);
-  if (chosen) {
+  if (chosen || learned) {
     hipLaunchKernelGGL(extract_vk_mixed,
                        dim3(grid_for(maxN * maxk), (unsigned)ng), dim3(256),
                        0, s, (const GateDesc*)d_gd, (const int*)d_src,
@@ -661,6 +1090,8 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
                        (const int64_t*)d_kept, cols_p, eng32.cols_p,
                        eng.cols_p);
     QTN_CHECK_HIP(hipGetLastError());
+  }
+  if (chosen) {
     // Newton-Schulz in FP64: V <- V (3 I - V^H V) / 2, three steps take the
     // FP32 orthogonality error (<~1e-3 on the smallest kept columns) below
     // FP64 rounding.
@@ -696,7 +1127,7 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
                          (const double2*)dT, t_sz, (const int64_t*)d_kept);
       QTN_CHECK_HIP(hipGetLastError());
     }
-  } else {
+  } else if (!learned) {
     hipLaunchKernelGGL(extract_vk, dim3(grid_for(maxN * maxk), (unsigned)ng),
                        dim3(256), 0, s, (const GateDesc*)d_gd,
                        (const double*)eng.sig, (const int*)eng.perm,
@@ -716,7 +1147,8 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
   hipLaunchKernelGGL(write_right, dim3(grid_for(maxN * maxk), (unsigned)ng),
                      dim3(256), 0, s, (const GateDesc*)d_gd,
                      (const double*)eng.sig, (const int64_t*)d_kept,
-                     (const double*)eng.keptw, cols_p, chosen ? 0 : 1);
+                     (const double*)eng.keptw, cols_p,
+                     (chosen || learned) ? 0 : 1);
   QTN_CHECK_HIP(hipGetLastError());
   // 5. B_i = Phi V_k / |S_k|
   std::vector<GemmDesc<double>> g2(ng);
@@ -728,7 +1160,9 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
     e.N = kept[g];
     e.K = 2 * d.chiR;
     // the chosen form normalises after recomputing |S_k| in FP64
-    e.alpha = make_double2(chosen ? 1.0 : (nrm > 0 ? 1.0 / nrm : 0.0), 0.0);
+    e.alpha = make_double2((chosen || learned) ? 1.0
+                                               : (nrm > 0 ? 1.0 / nrm : 0.0),
+                           0.0);
     e.beta = make_double2(0.0, 0.0);
     e.A = d.phi;
     e.lda = 2 * d.chiL;
@@ -740,7 +1174,7 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
   QTN_CHECK_HIP(hipMemcpyAsync(d_g2, g2.data(), ng * sizeof(GemmDesc<double>),
                                hipMemcpyHostToDevice, s));
   QTN_TRY(launch_zgemm_grouped(s, 0, 0, d_g2, ng, maxM, maxk));
-  if (chosen) {
+  if (chosen || learned) {
     // S_k = column norms of diag(lambda_{i-1}) Phi V_k in FP64; B_i and
     // lambda_i are normalised by |S_k| (overwriting what write_right put in
     // lambda_i from the FP32 values)
@@ -757,12 +1191,24 @@ int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
     QTN_CHECK_HIP(hipGetLastError());
   }
   for (int g = 0; g < ng; ++g) {
-    m->dims[sites[g]] = kept[g];
+    ms[g]->dims[sites[g]] = kept[g];
     if (kept_out) kept_out[g] = kept[g];
     if (disc_out) disc_out[g] = total[g] > 0 ? disc[g] / total[g] : 0.0;
   }
   QTN_CHECK_HIP(hipStreamSynchronize(s));
   return QTN_OK;
+}
+
+// All gates on one state with one cap.
+int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
+                        const double* gates, double cutoff, int64_t chimax,
+                        int form, int64_t* kept_out, double* disc_out,
+                        double* spec_out, int64_t* spec_len) {
+  std::vector<qtn_mps*> ms(ng, m);
+  std::vector<int64_t> chis(ng, chimax);
+  return apply_gates_core(m->ctx, ng, ms.data(), sites, gates, cutoff,
+                          chis.data(), form, kept_out, disc_out, spec_out,
+                          spec_len);
 }
 
 // ---------------------------------------------------------------------------
@@ -1111,6 +1557,126 @@ int32_t qtn_mps_norm2(const qtn_mps* m, double* out) {
   double v[2];
   QTN_TRY(qtn_mps_overlap(m, m, v));
   *out = v[0];
+  return QTN_OK;
+}
+
+int32_t qtn_mps_apply_1q_multi(int32_t ngates, qtn_mps* const* states,
+                               const int32_t* sites, const double* us) {
+  clear_error();
+  QTN_ARG(ngates >= 0, "qtn_mps_apply_1q_multi: negative count");
+  if (ngates == 0) return QTN_OK;
+  QTN_ARG(states && sites && us, "qtn_mps_apply_1q_multi: NULL argument");
+  qtn_ctx* ctx = states[0] ? states[0]->ctx : nullptr;
+  QTN_ARG(ctx != nullptr, "qtn_mps_apply_1q_multi: NULL state");
+  std::vector<OneQDesc> dd(ngates);
+  int64_t maxt = 1;
+  for (int g = 0; g < ngates; ++g) {
+    const qtn_mps* m = states[g];
+    QTN_ARG(m && m->ctx == ctx, "qtn_mps_apply_1q_multi: bad state");
+    const int site = sites[g];
+    QTN_ARG(site >= 1 && site <= m->n, "qtn_mps_apply_1q_multi: bad site");
+    dd[g].B = m->site[site - 1];
+    dd[g].chiL = m->dims[site - 1];
+    dd[g].chiR = m->dims[site];
+    for (int k = 0; k < 4; ++k)
+      dd[g].u[k] = make_double2(us[8 * g + 2 * k], us[8 * g + 2 * k + 1]);
+    maxt = std::max(maxt, dd[g].chiL * dd[g].chiR);
+  }
+  QTN_CHECK_HIP(hipSetDevice(ctx->device));
+  void* dptr;
+  QTN_TRY(ctx->buffer(kSlot1q, (size_t)ngates * sizeof(OneQDesc), &dptr));
+  QTN_CHECK_HIP(hipMemcpyAsync(dptr, dd.data(), ngates * sizeof(OneQDesc),
+                               hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(apply_1q_multi_kernel,
+                     dim3(grid_for(maxt), (unsigned)ngates), dim3(256), 0,
+                     ctx->stream, (const OneQDesc*)dptr);
+  QTN_CHECK_HIP(hipGetLastError());
+  QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return QTN_OK;
+}
+
+int32_t qtn_mps_apply_layer_multi(int32_t ngates, qtn_mps* const* states,
+                                  const int32_t* sites, const double* gates,
+                                  const int64_t* chimax, double cutoff,
+                                  int32_t form, int64_t* kept,
+                                  double* disc_weight) {
+  clear_error();
+  QTN_ARG(ngates >= 0, "qtn_mps_apply_layer_multi: negative count");
+  if (ngates == 0) return QTN_OK;
+  QTN_ARG(states && sites && gates && chimax,
+          "qtn_mps_apply_layer_multi: NULL argument");
+  QTN_ARG(cutoff >= 0.0, "Error must be positive");
+  QTN_ARG(form == QTN_FORM_PLAIN || form == QTN_FORM_CHOSEN,
+          "qtn_mps_apply_layer_multi: unknown form");
+  qtn_ctx* ctx = states[0] ? states[0]->ctx : nullptr;
+  QTN_ARG(ctx != nullptr, "qtn_mps_apply_layer_multi: NULL state");
+  QTN_CHECK_HIP(hipSetDevice(ctx->device));
+  // one batched pass per padded column count of X (64, 128, ...): gates of
+  // similar size share a Jacobi batch without padding small problems to the
+  // largest
+  std::map<int64_t, std::vector<int>> bucket;
+  std::map<const qtn_mps*, std::vector<char>> used;
+  for (int g = 0; g < ngates; ++g) {
+    const qtn_mps* m = states[g];
+    QTN_ARG(m && m->ctx == ctx, "qtn_mps_apply_layer_multi: bad state");
+    const int i = sites[g];
+    QTN_ARG(i >= 1 && i < m->n, "apply_2q: site out of range");
+    std::vector<char>& u = used[m];
+    if (u.empty()) u.assign(m->n + 1, 0);
+    QTN_ARG(!u[i] && !u[i + 1], "apply_layer: gates overlap");
+    u[i] = u[i + 1] = 1;
+    const int64_t M = 2 * m->dims[i - 1], N = 2 * m->dims[i + 1];
+    bucket[round_up64(std::min(M, N))].push_back(g);
+  }
+  for (auto& kv : bucket) {
+    const std::vector<int>& idx = kv.second;
+    const int nb = (int)idx.size();
+    std::vector<qtn_mps*> ms(nb);
+    std::vector<int32_t> st(nb);
+    std::vector<double> gs((size_t)nb * 32);
+    std::vector<int64_t> ch(nb), kp(nb);
+    std::vector<double> dw(nb);
+    for (int b = 0; b < nb; ++b) {
+      const int g = idx[b];
+      ms[b] = states[g];
+      st[b] = sites[g];
+      std::memcpy(&gs[(size_t)b * 32], gates + (size_t)g * 32,
+                  32 * sizeof(double));
+      ch[b] = chimax[g] > 0 ? chimax[g] : states[g]->cap;
+    }
+    QTN_TRY(apply_gates_core(ctx, nb, ms.data(), st.data(), gs.data(),
+                             cutoff, ch.data(), form, kp.data(), dw.data(),
+                             nullptr, nullptr));
+    for (int b = 0; b < nb; ++b) {
+      if (kept) kept[idx[b]] = kp[b];
+      if (disc_weight) disc_weight[idx[b]] = dw[b];
+    }
+  }
+  return QTN_OK;
+}
+
+int32_t qtn_mps_apply_layer_learned(qtn_mps* m, int32_t ngates,
+                                    const int32_t* sites, const double* gates,
+                                    double cutoff, int64_t chimax,
+                                    const int64_t* rank_hint, int64_t* kept,
+                                    double* disc_weight, int32_t* accepted) {
+  clear_error();
+  QTN_ARG(m && (ngates == 0 || (sites && gates && rank_hint)),
+          "qtn_mps_apply_layer_learned: NULL argument");
+  QTN_ARG(cutoff >= 0.0, "Error must be positive");
+  QTN_ARG(chimax <= m->cap, "qtn_mps_apply_layer_learned: chimax above "
+                            "capacity");
+  if (ngates == 0) return QTN_OK;
+  QTN_CHECK_HIP(hipSetDevice(m->ctx->device));
+  const int64_t cm = (chimax > 0) ? chimax : m->cap;
+  std::vector<qtn_mps*> ms(ngates, m);
+  std::vector<int64_t> chis(ngates, cm);
+  std::vector<int> acc(ngates, 0);
+  QTN_TRY(apply_gates_core(m->ctx, ngates, ms.data(), sites, gates, cutoff,
+                           chis.data(), QTN_FORM_LEARNED, kept, disc_weight,
+                           nullptr, nullptr, rank_hint, acc.data()));
+  if (accepted)
+    for (int g = 0; g < ngates; ++g) accepted[g] = acc[g];
   return QTN_OK;
 }
 

@@ -773,80 +773,128 @@ __global__ void jac_sweep_end(int* __restrict__ active,
   if (threadIdx.x == 0) *n_active = cnt;
 }
 
-// Frobenius norm^2 per problem, finiteness check, zero-column threshold.
-// grid (batch), 256 threads.
+// Frobenius norm, finiteness check, scaling exponent and zero-column
+// threshold per problem, in two steps.  jac_prep_partial: every block of
+// grid (kPrepBlocks, batch) reduces its slice to the scaled sum of squares
+// (scale = max |re|, |im| seen, ssq = sum (x / scale)^2, combined with the
+// rescaling of LAPACK's nrm2), so nothing overflows or underflows.
+constexpr int kPrepBlocks = 32;
+
+__device__ __forceinline__ void ssq_add(double a, double& scale, double& ssq) {
+  if (a > 0.0) {
+    if (scale < a) {
+      const double r = scale / a;
+      ssq = 1.0 + ssq * r * r;
+      scale = a;
+    } else {
+      const double r = a / scale;
+      ssq += r * r;
+    }
+  }
+}
+
+__device__ __forceinline__ void ssq_merge(double s2, double q2, double& scale,
+                                          double& ssq) {
+  if (s2 <= 0.0) return;
+  if (scale < s2) {
+    const double r = scale / s2;
+    ssq = q2 + ssq * r * r;
+    scale = s2;
+  } else {
+    const double r = s2 / scale;
+    ssq += q2 * r * r;
+  }
+}
+
 template <typename R>
-__global__ __launch_bounds__(256) void jac_prepare(
+__global__ __launch_bounds__(256) void jac_prep_partial(
     const typename JT<R>::c_t* __restrict__ X, int64_t rows, int64_t ldx,
-    int64_t cols,
-    int64_t sX, double zrel,
-    double* __restrict__ normF2, double* __restrict__ zthr,
-    int* __restrict__ active, int* __restrict__ counters,
-    int* __restrict__ bad, const int* __restrict__ mask,
-    int* __restrict__ scale_e) {
-  const int bz = blockIdx.x;
+    int64_t cols, int64_t sX, const int* __restrict__ mask,
+    double* __restrict__ part_scale, double* __restrict__ part_ssq,
+    int* __restrict__ part_bad) {
+  const int bz = blockIdx.y;
+  const int slot = bz * kPrepBlocks + blockIdx.x;
   if (mask && !mask[bz]) {
     if (threadIdx.x == 0) {
-      normF2[bz] = 0.0;
-      zthr[bz] = 0.0;
-      active[bz] = 0;
-      counters[bz] = 0;
-      scale_e[bz] = 0;
+      part_scale[slot] = 0.0;
+      part_ssq[slot] = 0.0;
+      part_bad[slot] = 0;
     }
     return;
   }
   X += bz * sX;
-  // pass 1: largest |re|, |im| and finiteness
-  double amax = 0.0;
+  double scale = 0.0, ssq = 0.0;
   int nonfinite = 0;
-  for (int64_t i = threadIdx.x; i < rows * cols; i += 256) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < rows * cols;
+       i += (int64_t)kPrepBlocks * 256) {
     const typename JT<R>::c_t v = X[(i % rows) + (i / rows) * ldx];
-    if (!isfinite(v.x) || !isfinite(v.y)) nonfinite = 1;
-    amax = fmax(amax, fmax(fabs((double)v.x), fabs((double)v.y)));
+    if (!isfinite(v.x) || !isfinite(v.y)) {
+      nonfinite = 1;
+      continue;
+    }
+    ssq_add(fabs((double)v.x), scale, ssq);
+    ssq_add(fabs((double)v.y), scale, ssq);
   }
-  __shared__ double red[256];
+  __shared__ double rs[256], rq[256];
   __shared__ int nf;
   if (threadIdx.x == 0) nf = 0;
-  red[threadIdx.x] = amax;
+  rs[threadIdx.x] = scale;
+  rq[threadIdx.x] = ssq;
   __syncthreads();
   if (nonfinite) atomicOr(&nf, 1);
   for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w)
-      red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
-  // The sweeps run on X * 2^-e with 2^e <= max |x| < 2^(e+1) (an exact
-  // scaling), so the squared Gram entries stay far from overflow and
-  // underflow whatever the input's magnitude; jac_scale applies it and
-  // undoes it after the sweeps.
-  const double am = red[0];
-  const int e = (am > 0.0 && isfinite(am)) ? ilogb(am) : 0;
-  __syncthreads();
-  // pass 2: |X 2^-e|_F^2
-  double ssum = 0.0;
-  if (!nf)
-    for (int64_t i = threadIdx.x; i < rows * cols; i += 256) {
-      const typename JT<R>::c_t v = X[(i % rows) + (i / rows) * ldx];
-      const double a = ldexp((double)v.x, -e), b = ldexp((double)v.y, -e);
-      ssum += a * a + b * b;
+    if (threadIdx.x < w) {
+      double sc = rs[threadIdx.x], q = rq[threadIdx.x];
+      ssq_merge(rs[threadIdx.x + w], rq[threadIdx.x + w], sc, q);
+      rs[threadIdx.x] = sc;
+      rq[threadIdx.x] = q;
     }
-  red[threadIdx.x] = ssum;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    const double f2s = red[0];
-    scale_e[bz] = e;
-    normF2[bz] = f2s;
-    // Columns with norm below rows*eps*|X|_F are rounding noise (the
-    // backward error of the rotations themselves): never rotated.
-    zthr[bz] = zrel * zrel * f2s;
-    active[bz] = (f2s > 0.0 && !nf) ? 1 : 0;
-    counters[bz] = 0;
-    if (nf) atomicOr(bad, 1);
+    part_scale[slot] = rs[0];
+    part_ssq[slot] = rq[0];
+    part_bad[slot] = nf;
   }
+}
+
+// jac_prep_final: one thread per problem merges its partials in order.
+// The sweeps run on X * 2^-e with 2^e <= max |x| < 2^(e+1) (an exact
+// scaling), so the squared Gram entries stay far from overflow and
+// underflow whatever the input's magnitude; jac_scale applies it and undoes
+// it after the sweeps.  Columns with norm below rows*eps*|X|_F are rounding
+// noise (the backward error of the rotations themselves): never rotated.
+__global__ void jac_prep_final(int batch, double zrel,
+                               const double* __restrict__ part_scale,
+                               const double* __restrict__ part_ssq,
+                               const int* __restrict__ part_bad,
+                               const int* __restrict__ mask,
+                               double* __restrict__ normF2,
+                               double* __restrict__ zthr,
+                               int* __restrict__ active,
+                               int* __restrict__ counters,
+                               int* __restrict__ bad,
+                               int* __restrict__ scale_e) {
+  const int bz = blockIdx.x * blockDim.x + threadIdx.x;
+  if (bz >= batch) return;
+  double scale = 0.0, ssq = 0.0;
+  int nf = 0;
+  for (int b = 0; b < kPrepBlocks; ++b) {
+    ssq_merge(part_scale[bz * kPrepBlocks + b], part_ssq[bz * kPrepBlocks + b],
+              scale, ssq);
+    nf |= part_bad[bz * kPrepBlocks + b];
+  }
+  const bool on = !(mask && !mask[bz]);
+  const int e = (on && scale > 0.0 && isfinite(scale)) ? ilogb(scale) : 0;
+  // |X 2^-e|_F^2 = ssq (scale 2^-e)^2 with scale 2^-e in [1, 2)
+  const double r = ldexp(scale, -e);
+  const double f2s = on ? ssq * r * r : 0.0;
+  scale_e[bz] = e;
+  normF2[bz] = f2s;
+  zthr[bz] = zrel * zrel * f2s;
+  active[bz] = (on && f2s > 0.0 && !nf) ? 1 : 0;
+  counters[bz] = 0;
+  if (on && nf) atomicOr(bad, 1);
 }
 
 // X *= 2^(sgn * e[bz]) over the rows x cols block (exact).
@@ -903,7 +951,8 @@ __global__ __launch_bounds__(1024) void jac_sort_trunc(
     const int64_t* __restrict__ nsv, double cutoff, int relative,
     int64_t chimax, int64_t* __restrict__ kept, double* __restrict__ disc,
     double* __restrict__ total, double* __restrict__ keptw,
-    int* __restrict__ status, const int* __restrict__ scale_e) {
+    int* __restrict__ status, const int* __restrict__ scale_e,
+    const int64_t* __restrict__ chimax_arr) {
   const int bz = blockIdx.x;
   __shared__ double sv[kSortMax];
   __shared__ int si[kSortMax];
@@ -974,7 +1023,8 @@ __global__ __launch_bounds__(1024) void jac_sort_trunc(
       st = QTN_ERR_TRUNC;
       k = 0;
     }
-    if (chimax > 0 && k > chimax) k = chimax;
+    const int64_t cm = chimax_arr ? chimax_arr[bz] : chimax;
+    if (cm > 0 && k > cm) k = cm;
     // discarded weight: sum of squares of s[k..n-1], smallest first
     disc_w = 0.0;
     for (int64_t j = n - 1; j >= k; --j) {
@@ -1027,7 +1077,8 @@ int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
   // small per-problem arrays packed in one slot
   const size_t small =
       (size_t)batch * (p * sizeof(int) + 7 * sizeof(int) + 6 * sizeof(double) +
-                       2 * sizeof(int64_t)) +
+                       3 * sizeof(int64_t) +
+                       kPrepBlocks * (2 * sizeof(double) + sizeof(int))) +
       1024;
   QTN_TRY(ctx->buffer(slot_base + 2, small, &ptr));
   char* c = (char*)ptr;
@@ -1043,7 +1094,13 @@ int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
   c += batch * sizeof(double);
   offmax = (double*)c;
   c += batch * sizeof(double);
+  prep_scale = (double*)c;
+  c += (size_t)batch * kPrepBlocks * sizeof(double);
+  prep_ssq = (double*)c;
+  c += (size_t)batch * kPrepBlocks * sizeof(double);
   kept = (int64_t*)c;
+  c += batch * sizeof(int64_t);
+  chimax_dev = (int64_t*)c;
   c += batch * sizeof(int64_t);
   nsv = (int64_t*)c;
   c += batch * sizeof(int64_t);
@@ -1061,6 +1118,8 @@ int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
   c += batch * sizeof(int);
   scale_e = (int*)c;
   c += batch * sizeof(int);
+  prep_bad = (int*)c;
+  c += (size_t)batch * kPrepBlocks * sizeof(int);
   misc = (int*)c;  // [0] = n_active, [1] = nonfinite flag
   QTN_TRY(ctx->buffer(slot_base + 3, (size_t)batch * cols_p * sizeof(double),
                       &ptr));
@@ -1260,6 +1319,10 @@ int32_t JacobiEngine::run_t(qtn_ctx* ctx, typename CType<R>::type* X,
   if (mask_host)
     QTN_CHECK_HIP(hipMemcpyAsync(mask, mask_host, batch * sizeof(int),
                                  hipMemcpyHostToDevice, s));
+  if (chimax_per)
+    QTN_CHECK_HIP(hipMemcpyAsync(chimax_dev, chimax_per,
+                                 batch * sizeof(int64_t),
+                                 hipMemcpyHostToDevice, s));
   QTN_CHECK_HIP(hipMemsetAsync(misc, 0, 3 * sizeof(int), s));
   ctx_ = ctx;
   prof_events.clear();
@@ -1270,10 +1333,15 @@ int32_t JacobiEngine::run_t(qtn_ctx* ctx, typename CType<R>::type* X,
   }
   const int n_solved = n_act;
   QTN_CHECK_HIP(hipMemsetAsync(offmax, 0, batch * sizeof(double), s));
-  hipLaunchKernelGGL(jac_prepare<R>, dim3(batch), dim3(256), 0, s,
-                     (const c_t*)X, rows_p, ldx, cols_p, sX, zrel, normF2,
-                     zthr, active, counters, misc + 1,
-                     (const int*)(mask_host ? mask : nullptr), scale_e);
+  hipLaunchKernelGGL(jac_prep_partial<R>, dim3(kPrepBlocks, batch),
+                     dim3(256), 0, s, (const c_t*)X, rows_p, ldx, cols_p, sX,
+                     (const int*)(mask_host ? mask : nullptr), prep_scale,
+                     prep_ssq, prep_bad);
+  hipLaunchKernelGGL(jac_prep_final, dim3((batch + 63) / 64), dim3(64), 0, s,
+                     batch, zrel, (const double*)prep_scale,
+                     (const double*)prep_ssq, (const int*)prep_bad,
+                     (const int*)(mask_host ? mask : nullptr), normF2, zthr,
+                     active, counters, misc + 1, scale_e);
   QTN_CHECK_HIP(hipGetLastError());
   const unsigned sgrid = (unsigned)std::max<int64_t>(
       1, std::min<int64_t>((rows_p * cols_p + 255) / 256, 2048));
@@ -1343,7 +1411,8 @@ int32_t JacobiEngine::run_t(qtn_ctx* ctx, typename CType<R>::type* X,
   }
   hipLaunchKernelGGL(jac_sort_trunc, dim3(batch), dim3(1024), 0, s, sig, perm,
                      cols_p, (const int64_t*)nsv, cutoff, relative, chimax,
-                     kept, disc, total, keptw, status, (const int*)scale_e);
+                     kept, disc, total, keptw, status, (const int*)scale_e,
+                     (const int64_t*)(chimax_per ? chimax_dev : nullptr));
   QTN_CHECK_HIP(hipGetLastError());
   return QTN_OK;
 }

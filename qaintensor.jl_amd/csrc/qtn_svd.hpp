@@ -26,6 +26,8 @@ struct JacobiEngine {
   int slot_base = kSlotGram;  // first of the 5 context slots it uses
   bool fused = false;  // W = 32 rounds as one jac_round launch
   int64_t ldw = 0;     // leading dimension of W (0: cols_p)
+  const int64_t* chimax_per = nullptr;  // host, per-problem cap (optional)
+  int64_t* chimax_dev = nullptr;
   // device arrays (valid after reserve); Gram partials and rotation blocks
   // are complex double, or complex float when `single`
   void* Gp = nullptr;
@@ -44,7 +46,10 @@ struct JacobiEngine {
   int* status = nullptr;
   int* want_w = nullptr;
   int* mask = nullptr;  // problems to solve (host mask of run*)
-  int* scale_e = nullptr;  // X is swept as X * 2^-scale_e (see jac_prepare)
+  int* scale_e = nullptr;  // X is swept as X * 2^-scale_e (jac_prep_final)
+  double* prep_scale = nullptr;  // per-block partial norms (jac_prep_*)
+  double* prep_ssq = nullptr;
+  int* prep_bad = nullptr;
   int* misc = nullptr;
   double* sig = nullptr;  // [batch][cols_p], sorted descending after run
   qtn_ctx* ctx_ = nullptr;

@@ -3,7 +3,8 @@ API over libqtn (hand-written gfx950 kernels behind a C ABI)."""
 from ._lib import (Context, QtnError, QtnLibraryError, default_context,
                    load as load_library, lib_path)
 from . import circuits
-from .device import DeviceMPO, DeviceMPS, FORM_PLAIN, FORM_CHOSEN
+from .device import (DeviceMPO, DeviceMPS, FORM_PLAIN, FORM_CHOSEN,
+                     apply_1q_multi, apply_layer_multi, run_circuits)
 from .network import (ErrorException, GeneralTensorNetwork, Summation,
                       Tensor, TensorNetwork, contract, contract_rep,
                       is_power_two, shift_pair, shift_summation)

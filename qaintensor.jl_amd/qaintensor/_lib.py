@@ -23,6 +23,7 @@ QTN_ERR_NOTFINITE = -6
 
 QTN_FORM_PLAIN = 0
 QTN_FORM_CHOSEN = 1
+QTN_FORM_LEARNED = 2
 
 # Every symbol include/qtn.h declares (checked by the CPU test suite).
 EXPORTS = [
@@ -35,9 +36,13 @@ EXPORTS = [
     "qtn_mps_create", "qtn_mps_destroy", "qtn_mps_copy", "qtn_mps_set_sites",
     "qtn_mps_bond_dims", "qtn_mps_bond_spectrum", "qtn_mps_get_site",
     "qtn_mps_apply_1q", "qtn_mps_apply_2q", "qtn_mps_apply_layer",
+    "qtn_mps_apply_layer_multi", "qtn_mps_apply_1q_multi",
+    "qtn_mps_apply_layer_learned",
     "qtn_mps_amplitudes", "qtn_mps_expect_z", "qtn_mps_overlap",
     "qtn_mps_norm2", "qtn_mpo_create", "qtn_mpo_destroy",
     "qtn_mps_expect_mpo", "qtn_mps_apply_mpo",
+    "qtn_mlp_create", "qtn_mlp_destroy", "qtn_mlp_get_params",
+    "qtn_mlp_forward", "qtn_mlp_loss_grad", "qtn_mlp_train",
 ]
 
 
@@ -65,6 +70,7 @@ _pi64 = ctypes.POINTER(ctypes.c_int64)
 _pi32 = ctypes.POINTER(ctypes.c_int32)
 _pd = ctypes.POINTER(ctypes.c_double)
 _pu8 = ctypes.POINTER(ctypes.c_uint8)
+_pf = ctypes.POINTER(ctypes.c_float)
 
 
 def _declare(lib):
@@ -111,6 +117,14 @@ def _declare(lib):
                                     _pd, _pd, _pi64]),
         "qtn_mps_apply_layer": (_i32, [_vp, _i32, _pi32, _pd, _dbl, _i64,
                                        _i32, _pi64, _pd]),
+        "qtn_mps_apply_layer_multi": (_i32, [_i32, ctypes.POINTER(_vp), _pi32,
+                                             _pd, _pi64, _dbl, _i32, _pi64,
+                                             _pd]),
+        "qtn_mps_apply_1q_multi": (_i32, [_i32, ctypes.POINTER(_vp), _pi32,
+                                          _pd]),
+        "qtn_mps_apply_layer_learned": (_i32, [_vp, _i32, _pi32, _pd, _dbl,
+                                               _i64, _pi64, _pi64, _pd,
+                                               _pi32]),
         "qtn_mps_amplitudes": (_i32, [_vp, _i64, _pu8, _pd]),
         "qtn_mps_expect_z": (_i32, [_vp, _pd]),
         "qtn_mps_overlap": (_i32, [_vp, _vp, _pd]),
@@ -120,6 +134,13 @@ def _declare(lib):
         "qtn_mpo_destroy": (_i32, [_vp]),
         "qtn_mps_expect_mpo": (_i32, [_vp, _vp, _pd]),
         "qtn_mps_apply_mpo": (_i32, [_vp, _vp, _dbl, _i64, _vp, _pd]),
+        "qtn_mlp_create": (_i32, [_vp, _i32, _i32, _pf, ctypes.POINTER(_vp)]),
+        "qtn_mlp_destroy": (_i32, [_vp]),
+        "qtn_mlp_get_params": (_i32, [_vp, _pf]),
+        "qtn_mlp_forward": (_i32, [_vp, _i64, _pf, _pf]),
+        "qtn_mlp_loss_grad": (_i32, [_vp, _i64, _pf, _pf, _pf, _pf]),
+        "qtn_mlp_train": (_i32, [_vp, _i64, _pf, _pf, _i32, _i64,
+                                 ctypes.c_float, _pf]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -55,3 +55,19 @@ def random_bitstrings(n, count, seed):
     """count x n array of 0/1 (uint8); column w-1 is wire w."""
     rng = np.random.default_rng(seed)
     return rng.integers(0, 2, size=(count, n), dtype=np.uint8)
+
+
+def batch_circuits(count, seed, n_range=(16, 48), depth_range=(8, 24),
+                   chis=(32, 64, 128)):
+    """count independent seeded circuits for many-circuit batching:
+    [(n, depth, chi_max, layers)], n and depth uniform on the inclusive
+    ranges, chi_max uniform over chis, circuit i seeded by (seed, i)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.integers(n_range[0], n_range[1] + 1))
+        depth = int(rng.integers(depth_range[0], depth_range[1] + 1))
+        chi = int(chis[int(rng.integers(0, len(chis)))])
+        out.append((n, depth, chi, brickwall_layers(n, depth,
+                                                    seed * 100003 + i)))
+    return out

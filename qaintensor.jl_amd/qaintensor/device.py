@@ -254,3 +254,99 @@ class DeviceMPO:
                 self._h = None
         except Exception:
             pass
+
+
+def _state_array(states):
+    arr = (ctypes.c_void_p * len(states))()
+    for i, st in enumerate(states):
+        arr[i] = st.handle.value if isinstance(st.handle, ctypes.c_void_p) \
+            else st.handle
+    return arr
+
+
+def apply_1q_multi(states, sites, us):
+    """Single-qubit gates us[i] on (states[i], sites[i]) in one launch
+    (qtn_mps_apply_1q_multi)."""
+    n = len(states)
+    if n == 0:
+        return
+    st = np.ascontiguousarray(sites, dtype=np.int32)
+    u = np.ascontiguousarray(
+        np.stack([L.as_c128(x, "F").reshape(-1, order="F") for x in us]))
+    L.check(L.load().qtn_mps_apply_1q_multi(
+        n, _state_array(states), L.i32ptr(st),
+        u.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+
+def apply_layer_multi(states, sites, gates, chimax, cutoff,
+                      form=FORM_PLAIN):
+    """Two-qubit gates on many states in batched passes grouped by size
+    (qtn_mps_apply_layer_multi).  Returns (kept, discarded weight)."""
+    n = len(states)
+    kept = np.zeros(n, dtype=np.int64)
+    disc = np.zeros(n, dtype=np.float64)
+    if n == 0:
+        return kept, disc
+    st = np.ascontiguousarray(sites, dtype=np.int32)
+    g = np.ascontiguousarray(
+        np.stack([L.as_c128(x, "F").reshape(-1, order="F") for x in gates]))
+    cm = np.ascontiguousarray(chimax, dtype=np.int64)
+    L.check(L.load().qtn_mps_apply_layer_multi(
+        n, _state_array(states), L.i32ptr(st),
+        g.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+        L.i64ptr(cm), float(cutoff), int(form), L.i64ptr(kept),
+        L.dptr(disc)))
+    return kept, disc
+
+
+def run_circuits(states, circuits, chimax, cutoff, form=FORM_PLAIN,
+                 step_times=None):
+    """Run many brick-wall circuits (circuits[i] = list of (singles, pairs)
+    layers for states[i]) step by step: at step t the singles of layer t of
+    every circuit that has one go in one launch, then all its pairs in one
+    qtn_mps_apply_layer_multi call.  step_times, when a list, receives the
+    wall time of every step (synchronised)."""
+    import time
+    depth = max(len(c) for c in circuits)
+    for t in range(depth):
+        t0 = time.perf_counter()
+        s1, w1, u1, s2, w2, g2, c2 = [], [], [], [], [], [], []
+        for st, circ, cm in zip(states, circuits, chimax):
+            if t >= len(circ):
+                continue
+            singles, pairs = circ[t]
+            for (w, u) in singles:
+                s1.append(st)
+                w1.append(w)
+                u1.append(u)
+            for (w, gm) in pairs:
+                s2.append(st)
+                w2.append(w)
+                g2.append(gm)
+                c2.append(cm)
+        apply_1q_multi(s1, w1, u1)
+        apply_layer_multi(s2, w2, g2, c2, cutoff, form)
+        if step_times is not None:
+            step_times.append(time.perf_counter() - t0)
+
+
+def _apply_layer_learned(self, sites, gates, cutoff, chimax, hints):
+    """A layer in the learned form (qtn_mps_apply_layer_learned): returns
+    (kept, discarded weight, accepted) per gate."""
+    n = len(sites)
+    st = np.ascontiguousarray(sites, dtype=np.int32)
+    g = np.ascontiguousarray(
+        np.stack([L.as_c128(x, "F").reshape(-1, order="F") for x in gates]))
+    h = np.ascontiguousarray(hints, dtype=np.int64)
+    kept = np.zeros(n, dtype=np.int64)
+    disc = np.zeros(n, dtype=np.float64)
+    acc = np.zeros(n, dtype=np.int32)
+    L.check(L.load().qtn_mps_apply_layer_learned(
+        self.handle, n, L.i32ptr(st),
+        g.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+        float(cutoff), int(chimax), L.i64ptr(h), L.i64ptr(kept),
+        L.dptr(disc), L.i32ptr(acc)))
+    return kept, disc, acc
+
+
+DeviceMPS.apply_layer_learned = _apply_layer_learned

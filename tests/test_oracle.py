@@ -397,3 +397,24 @@ def test_expect_mpo_blas_equals_einsum_form_and_product_mpo():
     assert all(np.array_equal(a, b) for a, b in zip(W, Wp))
     Bs = O.left_canonical(Hm.random_mps_tensors(n, 16, 2))
     assert abs(O.expect_mpo_blas(Bs, W) - O.expect_mpo(Bs, W)) < 1e-12
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    nin, w = 69, 16
+    rng = np.random.default_rng(0)
+    p = rng.standard_normal(O.mlp_nparams(nin, w)) * 0.1
+    X = np.abs(rng.standard_normal((nin, 7))) + 1e-3
+    T = rng.standard_normal(7)
+    _, g = O.mlp_loss_grad(p, X, T, nin, w)
+    eps = 1e-6
+    for i in rng.integers(0, p.size, 25):
+        pp, pm = p.copy(), p.copy()
+        pp[i] += eps
+        pm[i] -= eps
+        lp, _ = O.mlp_loss_grad(pp, X, T, nin, w)
+        lm, _ = O.mlp_loss_grad(pm, X, T, nin, w)
+        assert abs(g[i] - (lp - lm) / (2 * eps)) <= 1e-7 + 1e-5 * abs(g[i])
+    # forward agrees with the loss: loss = mean (y - t)^2
+    y = O.mlp_forward(p, X, nin, w)[0]
+    l, _ = O.mlp_loss_grad(p, X, T, nin, w)
+    assert abs(l - np.mean((y - T) ** 2)) < 1e-12
