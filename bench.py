@@ -302,6 +302,161 @@ def bench_batch(args):
     print(json.dumps(line))
 
 
+def bench_learned(args):
+    """The learned rank predictor.  Training pairs (features, kept rank /
+    chi_max at cutoff 1e-8) come from seeded plain FP64 runs of 40-qubit
+    chi=256 brick-walls: Haar depth 20 (seeds 101, 102) and weakly
+    entangling depth 24 (tau 0.5, seeds 201, 202); the MLP (72 -> 128 ->
+    128 -> 1, GELU, FP32) trains on the GPU with its own kernels.
+    Evaluation: 64-qubit depth-24 chi=256 brick-walls, Haar (seed 7, the
+    headline) and weak (seed 8), each in the plain FP64 form (batched
+    layers; for Haar also one gate at a time), the chosen form, the learned
+    form with the trained network, and the learned form with the sketch
+    sized by the bond's rank before the gate (no network).  The learned
+    times include the per-layer device feature gather and MLP inference;
+    the gates' operator-Schmidt table (one batched launch, state
+    independent) is built before the timed region and its time is
+    reported.  The fallback rate is sketches rejected / sketches tried."""
+    import torch
+    from qaintensor import (DeviceMPS, FORM_CHOSEN, FORM_PLAIN,
+                            circuits as C, default_context)
+    from qaintensor import learned as Lrn
+    ctx = default_context()
+    cutoff, chi, margin = 1e-8, 256, 0.02
+    t0 = time.perf_counter()
+    Xs, Ts = [], []
+    for kind, seed in (("haar", 101), ("haar", 102), ("weak", 201),
+                       ("weak", 202)):
+        lay = (C.brickwall_layers(40, 20, seed) if kind == "haar"
+               else C.weak_brickwall_layers(40, 24, seed, 0.5))
+        x, t = Lrn.harvest(lay, 40, chi, cutoff, ctx=ctx)
+        Xs.append(x)
+        Ts.append(t)
+    X = np.concatenate(Xs, axis=1)
+    T = np.concatenate(Ts)
+    t_harvest = time.perf_counter() - t0
+    print("harvest %d pairs %.1f s" % (T.size, t_harvest), file=sys.stderr,
+          flush=True)
+    mlp = Lrn.RankMLP(Lrn.init_params(1), ctx=ctx)
+    perm = np.random.default_rng(0).permutation(T.size)
+    t0 = time.perf_counter()
+    hist = mlp.train(X[:, perm], T[perm], steps=3000, minibatch=256,
+                     lr=2e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    base = float(np.mean((T - T.mean()) ** 2))
+    print("train %.1f s loss %.3g (variance %.3g)" % (
+        t_train, float(hist[-50:].mean()), base), file=sys.stderr,
+        flush=True)
+
+    def evaluate(kind, layers):
+        n, n2q = 64, C.count_2q(layers)
+        t0 = time.perf_counter()
+        sch = Lrn.gate_schmidt_table(layers)
+        t_sch = time.perf_counter() - t0
+        out = {"two_qubit_gates": n2q, "schmidt_table_s": t_sch}
+        states = {}
+
+        def run(name, fn, reps):
+            best = None
+            for _ in range(reps):
+                dev = DeviceMPS(n, chi, ctx)
+                stats = {}
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                fn(dev, stats)
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                if best is None or dt < best[0]:
+                    best = (dt, dev, stats)
+            dt, dev, stats = best
+            states[name] = dev
+            r = {"gates_per_s": n2q / dt, "s": dt}
+            if stats:
+                tried = stats["tried"]
+                r["sketch_tried"] = tried
+                r["sketch_accepted"] = stats["accepted"]
+                r["fallback_rate"] = ((tried - stats["accepted"]) / tried
+                                      if tried else 0.0)
+                r["decided_by_sketch"] = stats["accepted"] / n2q
+            if name != "plain":
+                a, b = states["plain"], dev
+                r["infidelity_vs_fp64"] = 1 - abs(a.overlap(b)) ** 2 / (
+                    a.norm2() * b.norm2())
+            out[name] = r
+            print("%s %s %.2f s" % (kind, name, dt), file=sys.stderr,
+                  flush=True)
+
+        reps = max(1, args.steps)
+        run("plain", lambda d, st: d.run_layers(layers, cutoff, chi,
+                                                FORM_PLAIN, batched=True),
+            max(1, args.plain_steps))
+        if kind == "haar":
+            # the reference's order: one two-site update at a time
+            run("plain_per_gate", lambda d, st: d.run_layers(
+                layers, cutoff, chi, FORM_PLAIN, batched=False), 1)
+        run("chosen", lambda d, st: d.run_layers(layers, cutoff, chi,
+                                                 FORM_CHOSEN, batched=True),
+            reps)
+        run("learned", lambda d, st: Lrn.run_layers_learned(
+            d, layers, cutoff, chi, mlp, sch=sch, margin=margin, stats=st),
+            reps)
+        run("previous_rank", lambda d, st: Lrn.run_layers_learned(
+            d, layers, cutoff, chi, None, stats=st, hint_rule="previous"),
+            reps)
+        return out
+
+    for _ in range(args.warmup):
+        warm = C.brickwall_layers(64, 4, 3)
+        d = DeviceMPS(64, chi, ctx)
+        Lrn.run_layers_learned(d, warm, cutoff, chi, mlp, margin=margin)
+    haar = evaluate("haar", C.brickwall_layers(64, 24, 7))
+    weak = evaluate("weak", C.weak_brickwall_layers(64, 24, 8, 0.5))
+    head = haar["learned"]
+    ok = head["infidelity_vs_fp64"] <= 1e-6
+    line = {
+        "metric": "2q-gate applies/s, 64q depth-24 chi=256 Haar brick-wall, "
+                  "learned form (MLP-sized randomized SVD, full-SVD fallback)",
+        "value": head["gates_per_s"] if ok else haar["plain"]["gates_per_s"],
+        "unit": "2q-gates/s", "n_gpus": 1, "steps": max(1, args.steps),
+        "warmup": args.warmup, "ms_per_step": head["s"] * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64-sweeps/c128-core" if ok else "c128",
+        "data": "synthetic (seeded brick-wall circuits)",
+        "config": {"workload": "64 qubits, depth 24, chi_max 256, cutoff "
+                               "1e-8; MLP trained on 40-qubit runs",
+                   "margin": margin,
+                   "form": "learned" if ok else "plain"},
+        "training": {"pairs": int(T.size), "harvest_s": t_harvest,
+                     "train_s": t_train, "steps": 3000,
+                     "final_loss": float(hist[-50:].mean()),
+                     "target_variance": base},
+        "haar": haar, "weak_tau0.5": weak,
+    }
+    if not args.no_cpu:
+        from oracle import qtn_oracle as O
+        layers = C.brickwall_layers(64, 24, 7)
+        st = O.TebdState(64)
+        t0 = time.perf_counter()
+        done = 0
+        for li in range(len(layers)):
+            O.run_layers(st, layers[li:li + 1], cutoff, chi)
+            done += len(layers[li][1])
+            if time.perf_counter() - t0 > 15.0:
+                break
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {
+            "value": done / dt, "unit": "2q-gates/s",
+            "cores": int(os.environ.get("OPENBLAS_NUM_THREADS",
+                                        os.environ.get("OMP_NUM_THREADS",
+                                                       "1"))),
+            "kind": "port",
+            "sample": "oracle, plain FP64, first %d layers (%d gates) of the "
+                      "Haar circuit, %.1f s" % (li + 1, done, dt)}
+    mlp.free()
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -310,7 +465,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="circuit",
-                    choices=["circuit", "tfim-zipup", "circuits-batch"])
+                    choices=["circuit", "tfim-zipup", "circuits-batch",
+                             "learned"])
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, copy a MIB-MiB device buffer 4 "
                          "times (a kernel of known traffic for calibrating "
@@ -328,6 +484,9 @@ def main():
         return
     if args.workload == "circuits-batch":
         bench_batch(args)
+        return
+    if args.workload == "learned":
+        bench_learned(args)
         return
 
     import torch

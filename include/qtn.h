@@ -51,7 +51,7 @@ extern "C" {
 /* precision forms of the two-site update */
 #define QTN_FORM_PLAIN      0  /* ComplexF64 end to end                       */
 #define QTN_FORM_CHOSEN     1  /* complex FP32 bulk, FP64 core (see docs)     */
-#define QTN_FORM_LEARNED    2  /* rank-hinted randomized SVD, full fallback   */
+#define QTN_FORM_LEARNED    2  /* chosen + rank-hinted randomized SVD        */
 
 typedef struct qtn_ctx qtn_ctx;
 typedef struct qtn_mps qtn_mps;
@@ -226,9 +226,15 @@ int32_t qtn_mps_apply_layer(qtn_mps* mps, int32_t ngates,
  * predicted rank rank_hint[i] lies in (0, chimax) and whose sketch
  * (rank_hint[i] + 16 columns) is at most 3/4 of the full rank first takes the
  * randomized range finder (2 power iterations, complex FP64); its exact
- * discarded weight |theta|^2 - sum of the kept sketch values^2 must meet
- * the cutoff, else the gate falls back to the full Jacobi SVD.
- * accepted[i] = 1 for the gates the range finder decided (may be NULL).   */
+ * discarded weight, |theta - Q Q^H theta|_F^2 summed entry by entry plus
+ * the dropped sketch values^2, must meet the cutoff, else the gate falls
+ * back to the full Jacobi SVD.  Gates
+ * that skip the range finder are solved as in QTN_FORM_CHOSEN (FP32
+ * sweeps when the cap binds, FP64 otherwise), so hints of 0 everywhere
+ * give the chosen form.
+ * accepted (may be NULL): 1 for the gates the range finder decided, -1
+ * for those whose sketch failed the check (solved by the full SVD), 0 where
+ * no sketch was tried.                                                    */
 int32_t qtn_mps_apply_layer_learned(qtn_mps* mps, int32_t ngates,
                                     const int32_t* sites, const double* gates,
                                     double cutoff, int64_t chimax,
@@ -305,6 +311,35 @@ int32_t qtn_mlp_loss_grad(qtn_mlp* mlp, int64_t nb, const float* X,
 int32_t qtn_mlp_train(qtn_mlp* mlp, int64_t nb, const float* X,
                       const float* T, int32_t steps, int64_t minibatch,
                       float lr, float* loss_hist);
+
+/* Operator-Schmidt coefficients of ngates 4x4 gates (gates[32*i..] as in
+ * qtn_mps_apply_2q): the singular values, descending, of each gate split
+ * into (s_i, s_i') x (s_{i+1}, s_{i+1}') factors, out[4*i..].  The split is
+ * the one of src/mpo.jl:40-56 (MPO(m) of a two-site operator).            */
+int32_t qtn_gate_schmidt(qtn_ctx* ctx, int32_t ngates, const double* gates,
+                         double* out);
+/* The predictor's input for the gates on (sites[i], sites[i]+1) of a state:
+ * out (72 x ngates floats, column-major, host) holds per gate the bond
+ * spectrum before the gate (top 64 values, zero padded), the gate's four
+ * operator-Schmidt coefficients schmidt[4*i..] (as given), the bond
+ * position sites[i]/n and chi_{i-1}, chi_i, chi_{i+1} / chimax (chimax
+ * <= 0: the state's capacity).  No counterpart in the reference.          */
+int32_t qtn_mps_gate_features(qtn_mps* mps, int32_t ngates,
+                              const int32_t* sites, const double* schmidt,
+                              int64_t chimax, float* out);
+/* A layer in the learned form with the predictor in the library: the
+ * features above are gathered on the device, the MLP (nin 72, same
+ * context) predicts y per gate, and rank_hint = ceil((y + margin) chimax)
+ * clipped to [0, chimax] drives qtn_mps_apply_layer_learned.  hints (may
+ * be NULL) receives the hints.  No counterpart in the reference.           */
+int32_t qtn_mps_apply_layer_predicted(qtn_mps* mps, int32_t ngates,
+                                      const int32_t* sites,
+                                      const double* gates,
+                                      const double* schmidt, double cutoff,
+                                      int64_t chimax, qtn_mlp* mlp,
+                                      double margin, int64_t* kept,
+                                      double* disc_weight, int32_t* accepted,
+                                      int64_t* hints);
 
 #ifdef __cplusplus
 }

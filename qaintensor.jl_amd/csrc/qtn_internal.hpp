@@ -89,10 +89,18 @@ int32_t launch_cgemm_grouped(hipStream_t s, int opA, int opB,
 int32_t launch_permute(hipStream_t s, const double2* in, int nd,
                        const int64_t* dims, const int* perm0, double2* out);
 
+// Rank-predictor MLP (qtn_mlp.hip): prediction on device features, Y
+// points into the MLP's own scratch.
+int32_t mlp_predict_device(qtn_mlp* m, int64_t nb, const float* X,
+                           float** Y);
+int mlp_nin(const qtn_mlp* m);
+qtn_ctx* mlp_ctx(const qtn_mlp* m);
+
 }  // namespace qtn
 
 struct qtn_ctx {
   int device = 0;
+  uint64_t sketch_seed = 0x5eed;  // range-finder sketches (learned form)
   hipStream_t stream = nullptr;
   // Named scratch buffers grown on demand (never shrunk until destroy).
   std::vector<qtn::DevBuf> bufs;

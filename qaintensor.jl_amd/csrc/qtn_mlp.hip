@@ -316,6 +316,9 @@ int32_t mlp_predict_device(qtn_mlp* m, int64_t nb, const float* X,
   return QTN_OK;
 }
 
+int mlp_nin(const qtn_mlp* m) { return m->nin; }
+qtn_ctx* mlp_ctx(const qtn_mlp* m) { return m->ctx; }
+
 }  // namespace qtn
 
 using namespace qtn;

@@ -407,7 +407,7 @@ struct RandDesc {
   double2* Om;       // N x l sketch (ld N)
   double2* Y;        // M x l (ld M)
   double2* Z;        // N x l (ld N)
-  double2* Q;        // max(M, N) x l (ld max(M, N)): orthonormal basis
+  double2* Q;        // rows x l (ld: the batch's largest max(M, N))
 };
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
@@ -435,12 +435,13 @@ __global__ void gauss_fill(const RandDesc* __restrict__ rd, uint64_t seed) {
   }
 }
 
-// B = [I_l; 0] (rows x l, ld ld) for every gate (rows = M or N by `which`).
-__global__ void eye_fill(const RandDesc* __restrict__ rd, int which) {
+// Q = [I_l; 0] (rows x l, ld ldq, the batch-wide leading dimension of Q)
+// for every gate (rows = M or N by `which`).
+__global__ void eye_fill(const RandDesc* __restrict__ rd, int which,
+                         int64_t ld) {
   const int bz = blockIdx.y;
   const RandDesc d = rd[bz];
   const int64_t rows = which ? d.N : d.M;
-  const int64_t ld = d.M > d.N ? d.M : d.N;
   const int64_t total = rows * d.l;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -469,14 +470,44 @@ __global__ __launch_bounds__(256) void theta_norm2(
   if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
-// The exact check: with sigma_j the singular values of B = Q^H theta
-// (sorted), the weight theta loses when truncated to the top k of them is
-// exactly |theta|_F^2 - sum_{j<=k} sigma_j^2.  k = the fewest values with
-// that loss <= (cutoff |theta|_F)^2; accepted when such k exists below the
-// sketch width l and the cap chimax, else the gate falls back to the full
-// SVD.  One thread per gate.
+// |X - P|_F^2 per gate (one workgroup per gate): the weight of theta
+// outside the range of Q, P = Q Q^H theta in X's orientation (rows_x x
+// cols_x, ld ldp, batch stride sP).  Summed entry by entry, so it keeps its
+// relative accuracy however small it is (|theta|^2 - |Q^H theta|^2 would
+// cancel to rounding noise at the 1e-16 level a cutoff of 1e-8 asks for).
+__global__ __launch_bounds__(256) void resid_norm2(
+    const RandDesc* __restrict__ rd, const double2* __restrict__ P,
+    int64_t ldp, int64_t sP, double* __restrict__ out) {
+  const RandDesc d = rd[blockIdx.x];
+  const int64_t rows = d.trans ? d.N : d.M, cols = d.trans ? d.M : d.N;
+  const double2* p = P + (int64_t)blockIdx.x * sP;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < rows * cols; i += 256) {
+    const int64_t r = i % rows, c = i / rows;
+    const double2 x = d.X[r + c * d.ldx], y = p[r + c * ldp];
+    const double dx = x.x - y.x, dy = x.y - y.y;
+    acc += dx * dx + dy * dy;
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// The exact check.  With sigma_j the singular values of B = Q^H theta
+// (sorted) and r = |theta - Q B|_F^2, truncating theta to the top k
+// directions of Q B loses exactly tail(k) = r + sum_{j >= k} sigma_j^2.
+// k = the fewest values with tail(k) <= (cutoff |theta|_F)^2 (the rule of
+// the full SVD, sums taken from the smallest term up); accepted when such k
+// exists below the sketch width l and within the cap chimax, else the gate
+// falls back to the full SVD.  One thread per gate.
 __global__ void rand_decide(int ng, const double* __restrict__ sig,
                             int64_t cols_p, const double* __restrict__ total,
+                            const double* __restrict__ resid,
                             const int64_t* __restrict__ lw, double cutoff,
                             const int64_t* __restrict__ chimax,
                             const int* __restrict__ attempt,
@@ -488,24 +519,26 @@ __global__ void rand_decide(int ng, const double* __restrict__ sig,
   if (bz >= ng) return;
   accepted[bz] = 0;
   if (!attempt[bz]) return;
-  const double tot = total[bz];
-  const double er2 = cutoff * cutoff * tot;
+  const double er2 = cutoff * cutoff * total[bz];
   const double* s = sig + (int64_t)bz * cols_p;
   const int64_t l = lw[bz];
-  double kw = 0.0;
-  int64_t k = -1;
-  for (int64_t j = 0; j < l && j < chimax[bz]; ++j) {
-    kw += s[j] * s[j];
-    if (tot - kw <= er2) {
+  double tail = resid[bz];
+  int64_t k = 0;
+  for (int64_t j = l - 1; j >= 0; --j) {
+    const double t = tail + s[j] * s[j];
+    if (t > er2) {
       k = j + 1;
       break;
     }
+    tail = t;
   }
-  if (k > 0 && k < l) {
+  if (k > 0 && k < l && k <= chimax[bz]) {
+    double kw = 0.0;
+    for (int64_t j = k - 1; j >= 0; --j) kw += s[j] * s[j];
     accepted[bz] = 1;
     kept[bz] = k;
     keptw[bz] = kw;
-    discw[bz] = tot - kw > 0.0 ? tot - kw : 0.0;
+    discw[bz] = tail;
   }
 }
 
@@ -534,6 +567,99 @@ __global__ void extract_vk_rand(const GateDesc* __restrict__ gd,
   }
 }
 
+// Features of the learned rank predictor, one column of kGateFeatures
+// floats per gate: the bond spectrum before the gate (top 64 values, zero
+// padded; the MLP takes their logarithm), the gate's four operator-Schmidt
+// coefficients, the bond position i / n and the bond dimensions chi_{i-1},
+// chi_i, chi_{i+1} / chimax.
+constexpr int kGateFeatures = 72;
+struct FeatDesc {
+  const double* lam;
+  int64_t len;
+  float sch[4];
+  float pos;
+  float dims[3];
+};
+
+__global__ __launch_bounds__(128) void gate_features(
+    const FeatDesc* __restrict__ fd, float* __restrict__ X) {
+  const FeatDesc d = fd[blockIdx.x];
+  const int t = threadIdx.x;
+  float v = 0.f;
+  if (t < 64)
+    v = t < d.len ? (float)d.lam[t] : 0.f;
+  else if (t < 68)
+    v = d.sch[t - 64];
+  else if (t == 68)
+    v = d.pos;
+  else if (t < kGateFeatures)
+    v = d.dims[t - 69];
+  if (t < kGateFeatures) X[(int64_t)blockIdx.x * kGateFeatures + t] = v;
+}
+
+// Operator-Schmidt coefficients of 4x4 gates (index s_w + 2 s_{w+1}), one
+// thread per gate: the singular values of M[a + 2c, b + 2d] = g[a + 2b,
+// c + 2d] (the split of the gate into (s_w, s_w') x (s_{w+1}, s_{w+1}')
+// factors) by one-sided Jacobi (Hestenes) on M's four columns in FP64
+// registers, sorted descending.
+__global__ void gate_schmidt(int ng, const double2* __restrict__ gates,
+                             double* __restrict__ out) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  const double2* G = gates + (int64_t)g * 16;
+  double2 a[4][4];  // a[col][row]
+  for (int b = 0; b < 2; ++b)
+    for (int d = 0; d < 2; ++d)
+      for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 2; ++c)
+          a[b + 2 * d][r + 2 * c] = G[(r + 2 * b) + 4 * (c + 2 * d)];
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    bool rotated = false;
+    for (int p = 0; p < 3; ++p)
+      for (int q = p + 1; q < 4; ++q) {
+        double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
+        for (int i = 0; i < 4; ++i) {
+          al += a[p][i].x * a[p][i].x + a[p][i].y * a[p][i].y;
+          be += a[q][i].x * a[q][i].x + a[q][i].y * a[q][i].y;
+          // conj(a_p) a_q
+          gr += a[p][i].x * a[q][i].x + a[p][i].y * a[q][i].y;
+          gi += a[p][i].x * a[q][i].y - a[p][i].y * a[q][i].x;
+        }
+        const double gm = sqrt(gr * gr + gi * gi);
+        if (gm <= 1e-15 * sqrt(al * be) || gm == 0.0) continue;
+        rotated = true;
+        // a_q <- e^{-i phi} a_q makes the inner product real (= gm)
+        const double er = gr / gm, ei = -gi / gm;
+        const double zeta = (be - al) / (2.0 * gm);
+        const double t = (zeta >= 0.0 ? 1.0 : -1.0) /
+                         (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+        for (int i = 0; i < 4; ++i) {
+          const double2 x = a[p][i];
+          const double2 y = make_double2(a[q][i].x * er - a[q][i].y * ei,
+                                         a[q][i].x * ei + a[q][i].y * er);
+          a[p][i] = make_double2(cs * x.x - sn * y.x, cs * x.y - sn * y.y);
+          a[q][i] = make_double2(sn * x.x + cs * y.x, sn * x.y + cs * y.y);
+        }
+      }
+    if (!rotated) break;
+  }
+  double sv[4];
+  for (int c = 0; c < 4; ++c) {
+    double s2 = 0.0;
+    for (int i = 0; i < 4; ++i)
+      s2 += a[c][i].x * a[c][i].x + a[c][i].y * a[c][i].y;
+    sv[c] = sqrt(s2);
+  }
+  for (int i = 1; i < 4; ++i)
+    for (int j = i; j > 0 && sv[j] > sv[j - 1]; --j) {
+      const double t = sv[j];
+      sv[j] = sv[j - 1];
+      sv[j - 1] = t;
+    }
+  for (int c = 0; c < 4; ++c) out[(int64_t)g * 4 + c] = sv[c];
+}
+
 __global__ void fill_one(double2* p) { *p = make_double2(1.0, 0.0); }
 __global__ void fill_lambda_one(double* p) { *p = 1.0; }
 
@@ -544,11 +670,12 @@ __global__ void fill_lambda_one(double* p) { *p = 1.0; }
 // Randomized range finder (Halko, Martinsson & Tropp, SIAM Rev. 53, 217
 // (2011), algorithm 4.4: q = 2 power iterations with re-orthonormalisation)
 // for the gates with attempt[g]: Q (M x l) spans theta Omega, B^H = theta^H
-// Q is split by the Jacobi engine, and rand_decide applies the exact
-// discarded-weight check.  Accepted gates get V_k written into their
+// Q is split by the Jacobi engine, the residual |theta - Q B|^2 is summed
+// explicitly, and rand_decide applies the exact discarded-weight check.  Accepted gates get V_k written into their
 // descriptor's Vk; kept / keptw / disc / total are filled for them.
 constexpr int kSlotROm = 140, kSlotRY = 141, kSlotRZ = 142, kSlotRQ = 143,
-              kSlotRDesc = 144, kSlotRSmall = 145, kSlotRDesc2 = 146;
+              kSlotRDesc = 144, kSlotRSmall = 145, kSlotRDesc2 = 146,
+              kSlotRP = 147, kSlotRRes = 148;
 
 int32_t rand_range_finder(qtn_ctx* ctx, int ng,
                           const std::vector<GateDesc>& gd,
@@ -610,7 +737,7 @@ int32_t rand_range_finder(qtn_ctx* ctx, int ng,
   //   Y = theta Om | theta Q   (M x l): trans X^H B, else X B
   //   Z = theta^H Q            (N x l): trans X Q,   else X^H Q
   void* pG;
-  QTN_TRY(ctx->buffer(kSlotRDesc2, (size_t)10 * ng * sizeof(GemmDesc<double>),
+  QTN_TRY(ctx->buffer(kSlotRDesc2, (size_t)11 * ng * sizeof(GemmDesc<double>),
                       &pG));
   GemmDesc<double>* dG = (GemmDesc<double>*)pG;
   // host descriptor arrays live until the stream is synchronised below
@@ -667,7 +794,7 @@ int32_t rand_range_finder(qtn_ctx* ctx, int ng,
     qr.slot_base = 150;
     QTN_TRY(qr.factor(ctx, A, lda, stride, maxl, dims, true));
     hipLaunchKernelGGL(eye_fill, dim3(grid_for(ldq * maxl), (unsigned)ng),
-                       dim3(256), 0, s, (const RandDesc*)d_rd, which);
+                       dim3(256), 0, s, (const RandDesc*)d_rd, which, ldq);
     QTN_CHECK_HIP(hipGetLastError());
     std::vector<int64_t> ldu(ng, ldq), uc(ng);
     for (int g = 0; g < ng; ++g) uc[g] = attempt[g] ? lw[g] : 0;
@@ -689,6 +816,51 @@ int32_t rand_range_finder(qtn_ctx* ctx, int ng,
   QTN_CHECK_HIP(hipMemsetAsync(pZ, 0, (size_t)ng * z_sz * sizeof(double2), s));
   QTN_TRY(run_products(false, (const double2*)pQ, q_sz, ldq, (double2*)pZ,
                        z_sz, rowsB, 3));
+  // P = Q Q^H theta in X's orientation (non-trans: Q Z^H, trans: Z Q^H),
+  // before the sweeps rotate Z; the residual |X - P|^2 feeds the check
+  int64_t maxRX = 1, maxCX = 1;
+  for (int g = 0; g < ng; ++g) {
+    if (!attempt[g]) continue;
+    maxRX = std::max(maxRX, rd[g].trans ? rd[g].N : rd[g].M);
+    maxCX = std::max(maxCX, rd[g].trans ? rd[g].M : rd[g].N);
+  }
+  const int64_t p_sz = maxRX * maxCX;
+  void* pP;
+  QTN_TRY(ctx->buffer(kSlotRP, (size_t)ng * p_sz * sizeof(double2), &pP));
+  {
+    keep.emplace_back();
+    std::vector<GemmDesc<double>>& gp = keep.back();
+    for (int g = 0; g < ng; ++g) {
+      if (!attempt[g]) continue;
+      const RandDesc& r = rd[g];
+      const double2* Qg = (const double2*)pQ + (size_t)g * q_sz;
+      const double2* Zg = (const double2*)pZ + (size_t)g * z_sz;
+      GemmDesc<double> e;
+      e.M = r.trans ? r.N : r.M;
+      e.N = r.trans ? r.M : r.N;
+      e.K = r.l;
+      e.alpha = make_double2(1.0, 0.0);
+      e.beta = make_double2(0.0, 0.0);
+      e.A = r.trans ? Zg : Qg;
+      e.lda = r.trans ? rowsB : ldq;
+      e.B = r.trans ? Qg : Zg;
+      e.ldb = r.trans ? ldq : rowsB;
+      e.C = (double2*)pP + (size_t)g * p_sz;
+      e.ldc = maxRX;
+      gp.push_back(e);
+    }
+    GemmDesc<double>* dp = dG + (size_t)10 * ng;
+    QTN_CHECK_HIP(hipMemcpyAsync(dp, gp.data(),
+                                 gp.size() * sizeof(GemmDesc<double>),
+                                 hipMemcpyHostToDevice, s));
+    QTN_TRY(launch_zgemm_grouped(s, 0, 2, dp, (int)gp.size(), maxRX, maxCX));
+  }
+  void* pR;
+  QTN_TRY(ctx->buffer(kSlotRRes, (size_t)ng * sizeof(double), &pR));
+  hipLaunchKernelGGL(resid_norm2, dim3(ng), dim3(256), 0, s,
+                     (const RandDesc*)d_rd, (const double2*)pP, maxRX, p_sz,
+                     (double*)pR);
+  QTN_CHECK_HIP(hipGetLastError());
   JacobiEngine eng;
   eng.slot_base = 160;
   QTN_TRY(eng.reserve(ctx, ng, rowsB, colsB));
@@ -715,7 +887,8 @@ int32_t rand_range_finder(qtn_ctx* ctx, int ng,
                      (const RandDesc*)d_rd, d_tot);
   hipLaunchKernelGGL(rand_decide, dim3((ng + 63) / 64), dim3(64), 0, s, ng,
                      (const double*)eng.sig, eng.cols_p,
-                     (const double*)d_tot, (const int64_t*)d_lw, cutoff,
+                     (const double*)d_tot, (const double*)pR,
+                     (const int64_t*)d_lw, cutoff,
                      (const int64_t*)d_ch, (const int*)d_att, d_kept, d_acc,
                      d_kw, d_dw);
   hipLaunchKernelGGL(extract_vk_rand,
@@ -754,7 +927,8 @@ int32_t rand_range_finder(qtn_ctx* ctx, int ng,
 // hint (learned form, may be NULL): predicted kept rank per gate; gates
 // with 0 < hint < cap try the randomized range finder first (sketch width
 // hint + 16) and fall back to the full SVD when its exact check fails;
-// rand_accepted (may be NULL) receives 1 for the gates it decided.
+// rand_accepted (may be NULL) receives 1 for the gates it decided, -1 for
+// those whose sketch failed the check, 0 where no sketch was tried.
 int32_t apply_gates_core(qtn_ctx* ctx, int ng, qtn_mps* const* ms,
                          const int32_t* sites, const double* gates,
                          double cutoff, const int64_t* chis, int form,
@@ -938,12 +1112,38 @@ int32_t apply_gates_core(qtn_ctx* ctx, int ng, qtn_mps* const* ms,
     QTN_CHECK_HIP(hipStreamSynchronize(s));
     return QTN_OK;
   };
+  // The learned form is the chosen form plus the range finder: gates whose
+  // predicted rank stays below the cap try the sketch (sketch width
+  // hint + 16, well below the full rank), the other capped gates take the
+  // FP32 route, and the rest (and every rejected sketch) the FP64 engine.
+  std::vector<int> attempt(ng, 0);
+  std::vector<int64_t> lw(ng, 0);
+  bool any_attempt = false;
+  if (learned && hint) {
+    for (int g = 0; g < ng; ++g) {
+      const int64_t l = std::min<int64_t>(hint[g] + 16, nsv[g]);
+      if (hint[g] > 0 && hint[g] < chis[g] && 4 * l <= 3 * nsv[g]) {
+        attempt[g] = 1;
+        lw[g] = l;
+        any_attempt = true;
+      }
+    }
+  }
   void *dX32 = nullptr, *dW32 = nullptr;
-  if (chosen) {
+  if (chosen || learned) {
+    // The FP32 route pays only where the cap binds.  Without a prediction
+    // that is taken to be where the bond sat at the cap before the gate or
+    // theta is at least twice the cap (a cutoff-bound spectrum below the
+    // cap would send the FP32 result back to the FP64 engine); a predicted
+    // rank at the cap decides it in the learned form.
     std::vector<int> cand(ng, 0);
     bool any32 = false;
     for (int g = 0; g < ng; ++g) {
-      cand[g] = nsv[g] > chis[g] ? 1 : 0;
+      const bool capped =
+          (learned && hint && hint[g] > 0)
+              ? hint[g] >= chis[g]
+              : (ms[g]->dims[sites[g]] >= chis[g] || nsv[g] >= 2 * chis[g]);
+      cand[g] = nsv[g] > chis[g] && !attempt[g] && capped ? 1 : 0;
       any32 = any32 || cand[g];
     }
     if (any32) {
@@ -978,24 +1178,13 @@ int32_t apply_gates_core(qtn_ctx* ctx, int ng, qtn_mps* const* ms,
       }
     }
   }
-  if (learned && hint) {
-    std::vector<int> attempt(ng, 0);
-    std::vector<int64_t> lw(ng, 0);
-    bool any = false;
-    for (int g = 0; g < ng; ++g) {
-      const int64_t l = std::min<int64_t>(hint[g] + 16, nsv[g]);
-      // worth trying when the predicted rank stays below the cap (the
-      // cutoff can decide it) and the sketch is well below the full rank
-      if (hint[g] > 0 && hint[g] < chis[g] && 4 * l <= 3 * nsv[g]) {
-        attempt[g] = 1;
-        lw[g] = l;
-        any = true;
-      }
-    }
-    if (any) {
+  if (any_attempt) {
+    {
       std::vector<int> acc;
-      static uint64_t seed = 0x5eed;
-      seed = seed * 6364136223846793005ULL + 1442695040888963407ULL;
+      // each call draws a fresh sketch from the context's generator
+      ctx->sketch_seed =
+          ctx->sketch_seed * 6364136223846793005ULL + 1442695040888963407ULL;
+      const uint64_t seed = ctx->sketch_seed;
       QTN_TRY(rand_range_finder(ctx, ng, gd, d_gd, rows_p, attempt, lw, chis,
                                 cutoff, seed, acc, kept, keptw, disc,
                                 total));
@@ -1007,7 +1196,8 @@ int32_t apply_gates_core(qtn_ctx* ctx, int ng, qtn_mps* const* ms,
     }
   }
   if (rand_accepted)
-    for (int g = 0; g < ng; ++g) rand_accepted[g] = src[g] == 2 ? 1 : 0;
+    for (int g = 0; g < ng; ++g)
+      rand_accepted[g] = src[g] == 2 ? 1 : (attempt[g] ? -1 : 0);
   bool any64 = false;
   for (int g = 0; g < ng; ++g) any64 = any64 || need64[g];
   if (any64) {
@@ -1091,7 +1281,9 @@ int32_t apply_gates_core(qtn_ctx* ctx, int ng, qtn_mps* const* ms,
                        eng.cols_p);
     QTN_CHECK_HIP(hipGetLastError());
   }
-  if (chosen) {
+  bool any_fp32 = false;
+  for (int g = 0; g < ng; ++g) any_fp32 = any_fp32 || src[g] == 1;
+  if (chosen || (learned && any_fp32)) {
     // Newton-Schulz in FP64: V <- V (3 I - V^H V) / 2, three steps take the
     // FP32 orthogonality error (<~1e-3 on the smallest kept columns) below
     // FP64 rounding.
@@ -1653,6 +1845,134 @@ int32_t qtn_mps_apply_layer_multi(int32_t ngates, qtn_mps* const* states,
     }
   }
   return QTN_OK;
+}
+
+}  // extern "C"
+
+namespace qtn {
+namespace {
+
+constexpr int kSlotFeatDesc = 170, kSlotFeat = 171;
+
+// Device feature columns (kGateFeatures x ngates floats, in context slot
+// kSlotFeat) of the gates on (sites[i], sites[i] + 1).
+int32_t gather_features(qtn_mps* m, int ng, const int32_t* sites,
+                        const double* schmidt, int64_t chimax,
+                        float** dX_out) {
+  std::vector<FeatDesc> fd(ng);
+  for (int g = 0; g < ng; ++g) {
+    const int w = sites[g];
+    QTN_ARG(w >= 1 && w < m->n, "gate features: site out of range");
+    FeatDesc& d = fd[g];
+    d.lam = m->lam[w];
+    d.len = m->dims[w];
+    for (int j = 0; j < 4; ++j) d.sch[j] = (float)schmidt[4 * g + j];
+    d.pos = (float)((double)w / (double)m->n);
+    for (int j = 0; j < 3; ++j)
+      d.dims[j] = (float)((double)m->dims[w - 1 + j] / (double)chimax);
+  }
+  void *pd, *px;
+  QTN_TRY(m->ctx->buffer(kSlotFeatDesc, (size_t)ng * sizeof(FeatDesc), &pd));
+  QTN_TRY(m->ctx->buffer(kSlotFeat,
+                         (size_t)ng * kGateFeatures * sizeof(float), &px));
+  hipStream_t s = m->ctx->stream;
+  QTN_CHECK_HIP(hipMemcpyAsync(pd, fd.data(), ng * sizeof(FeatDesc),
+                               hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(gate_features, dim3(ng), dim3(128), 0, s,
+                     (const FeatDesc*)pd, (float*)px);
+  QTN_CHECK_HIP(hipGetLastError());
+  *dX_out = (float*)px;
+  return QTN_OK;
+}
+
+}  // namespace
+}  // namespace qtn
+
+extern "C" {
+
+int32_t qtn_gate_schmidt(qtn_ctx* ctx, int32_t ngates, const double* gates,
+                         double* out) {
+  clear_error();
+  QTN_ARG(ctx && (ngates == 0 || (gates && out)),
+          "qtn_gate_schmidt: NULL argument");
+  QTN_ARG(ngates >= 0, "qtn_gate_schmidt: negative count");
+  if (ngates == 0) return QTN_OK;
+  QTN_CHECK_HIP(hipSetDevice(ctx->device));
+  void *pg, *po;
+  QTN_TRY(ctx->buffer(kSlotFeatDesc, (size_t)ngates * 16 * sizeof(double2),
+                      &pg));
+  QTN_TRY(ctx->buffer(kSlotFeat, (size_t)ngates * 4 * sizeof(double), &po));
+  hipStream_t s = ctx->stream;
+  QTN_CHECK_HIP(hipMemcpyAsync(pg, gates,
+                               (size_t)ngates * 16 * sizeof(double2),
+                               hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(gate_schmidt, dim3((ngates + 63) / 64), dim3(64), 0, s,
+                     ngates, (const double2*)pg, (double*)po);
+  QTN_CHECK_HIP(hipGetLastError());
+  QTN_CHECK_HIP(hipMemcpyAsync(out, po, (size_t)ngates * 4 * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+  QTN_CHECK_HIP(hipStreamSynchronize(s));
+  return QTN_OK;
+}
+
+int32_t qtn_mps_gate_features(qtn_mps* m, int32_t ngates,
+                              const int32_t* sites, const double* schmidt,
+                              int64_t chimax, float* out) {
+  clear_error();
+  QTN_ARG(m && (ngates == 0 || (sites && schmidt && out)),
+          "qtn_mps_gate_features: NULL argument");
+  QTN_ARG(ngates >= 0, "qtn_mps_gate_features: negative count");
+  if (ngates == 0) return QTN_OK;
+  QTN_CHECK_HIP(hipSetDevice(m->ctx->device));
+  const int64_t cm = chimax > 0 ? chimax : m->cap;
+  float* dX;
+  QTN_TRY(gather_features(m, ngates, sites, schmidt, cm, &dX));
+  QTN_CHECK_HIP(hipMemcpyAsync(out, dX,
+                               (size_t)ngates * kGateFeatures * sizeof(float),
+                               hipMemcpyDeviceToHost, m->ctx->stream));
+  QTN_CHECK_HIP(hipStreamSynchronize(m->ctx->stream));
+  return QTN_OK;
+}
+
+int32_t qtn_mps_apply_layer_predicted(qtn_mps* m, int32_t ngates,
+                                      const int32_t* sites,
+                                      const double* gates,
+                                      const double* schmidt, double cutoff,
+                                      int64_t chimax, qtn_mlp* mlp,
+                                      double margin, int64_t* kept,
+                                      double* disc_weight, int32_t* accepted,
+                                      int64_t* hints) {
+  clear_error();
+  QTN_ARG(m && mlp && (ngates == 0 || (sites && gates && schmidt)),
+          "qtn_mps_apply_layer_predicted: NULL argument");
+  QTN_ARG(mlp_nin(mlp) == kGateFeatures,
+          "qtn_mps_apply_layer_predicted: the MLP must take 72 features");
+  QTN_ARG(mlp_ctx(mlp) == m->ctx,
+          "qtn_mps_apply_layer_predicted: MLP and state on different "
+          "contexts");
+  QTN_ARG(chimax <= m->cap, "qtn_mps_apply_layer_predicted: chimax above "
+                            "capacity");
+  if (ngates == 0) return QTN_OK;
+  QTN_CHECK_HIP(hipSetDevice(m->ctx->device));
+  const int64_t cm = chimax > 0 ? chimax : m->cap;
+  float* dX;
+  QTN_TRY(gather_features(m, ngates, sites, schmidt, cm, &dX));
+  float* dY;
+  QTN_TRY(mlp_predict_device(mlp, ngates, dX, &dY));
+  std::vector<float> y(ngates);
+  QTN_CHECK_HIP(hipMemcpyAsync(y.data(), dY, ngates * sizeof(float),
+                               hipMemcpyDeviceToHost, m->ctx->stream));
+  QTN_CHECK_HIP(hipStreamSynchronize(m->ctx->stream));
+  std::vector<int64_t> h(ngates);
+  for (int g = 0; g < ngates; ++g) {
+    const double v = std::ceil(((double)y[g] + margin) * (double)cm);
+    h[g] = std::isfinite(v) ? (int64_t)std::min<double>(
+                                  std::max<double>(v, 0.0), (double)cm)
+                            : 0;
+    if (hints) hints[g] = h[g];
+  }
+  return qtn_mps_apply_layer_learned(m, ngates, sites, gates, cutoff, chimax,
+                                     h.data(), kept, disc_weight, accepted);
 }
 
 int32_t qtn_mps_apply_layer_learned(qtn_mps* m, int32_t ngates,

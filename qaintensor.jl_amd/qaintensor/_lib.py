@@ -37,7 +37,9 @@ EXPORTS = [
     "qtn_mps_bond_dims", "qtn_mps_bond_spectrum", "qtn_mps_get_site",
     "qtn_mps_apply_1q", "qtn_mps_apply_2q", "qtn_mps_apply_layer",
     "qtn_mps_apply_layer_multi", "qtn_mps_apply_1q_multi",
-    "qtn_mps_apply_layer_learned",
+    "qtn_mps_apply_layer_learned", "qtn_mps_gate_features",
+    "qtn_gate_schmidt",
+    "qtn_mps_apply_layer_predicted",
     "qtn_mps_amplitudes", "qtn_mps_expect_z", "qtn_mps_overlap",
     "qtn_mps_norm2", "qtn_mpo_create", "qtn_mpo_destroy",
     "qtn_mps_expect_mpo", "qtn_mps_apply_mpo",
@@ -125,6 +127,12 @@ def _declare(lib):
         "qtn_mps_apply_layer_learned": (_i32, [_vp, _i32, _pi32, _pd, _dbl,
                                                _i64, _pi64, _pi64, _pd,
                                                _pi32]),
+        "qtn_gate_schmidt": (_i32, [_vp, _i32, _pd, _pd]),
+        "qtn_mps_gate_features": (_i32, [_vp, _i32, _pi32, _pd, _i64,
+                                         _pf]),
+        "qtn_mps_apply_layer_predicted": (_i32, [_vp, _i32, _pi32, _pd, _pd,
+                                                 _dbl, _i64, _vp, _dbl,
+                                                 _pi64, _pd, _pi32, _pi64]),
         "qtn_mps_amplitudes": (_i32, [_vp, _i64, _pu8, _pd]),
         "qtn_mps_expect_z": (_i32, [_vp, _pd]),
         "qtn_mps_overlap": (_i32, [_vp, _vp, _pd]),

@@ -47,6 +47,37 @@ def brickwall_layers(n, depth, seed):
     return layers
 
 
+def weak_brickwall_layers(n, depth, seed, tau=0.5):
+    """The brick-wall pattern with weakly entangling pair gates: each pair
+    gate is exp(-i tau H) for a GUE Hamiltonian H (4x4, (Z + Z^H)/2 with Z
+    complex Ginibre) scaled to spectral norm 1, fused with Haar single-qubit
+    gates as in brickwall_layers.  Its entanglement grows slowly, so at a
+    relative cutoff the kept ranks stay well below the full rank (the
+    regime where a rank-sized randomized SVD pays)."""
+    rng = np.random.default_rng(seed)
+    layers = []
+    for layer in range(depth):
+        us = [haar_unitary(rng, 2) for _ in range(n)]
+        start = 1 if layer % 2 == 0 else 2
+        pair_wires = list(range(start, n, 2))
+        pairs = []
+        covered = set()
+        for w in pair_wires:
+            z = (rng.standard_normal((4, 4)) + 1j * rng.standard_normal(
+                (4, 4))) / np.sqrt(2.0)
+            h = (z + z.conj().T) / 2
+            e, v = np.linalg.eigh(h)
+            e = e / np.max(np.abs(e))
+            g = (v * np.exp(-1j * tau * e)[None, :]) @ v.conj().T
+            pairs.append((w, g @ np.kron(us[w], us[w - 1])))
+            covered.add(w)
+            covered.add(w + 1)
+        singles = [(w, us[w - 1]) for w in range(1, n + 1)
+                   if w not in covered]
+        layers.append((singles, pairs))
+    return layers
+
+
 def count_2q(layers):
     return sum(len(p) for _, p in layers)
 

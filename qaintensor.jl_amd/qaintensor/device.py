@@ -332,7 +332,9 @@ def run_circuits(states, circuits, chimax, cutoff, form=FORM_PLAIN,
 
 def _apply_layer_learned(self, sites, gates, cutoff, chimax, hints):
     """A layer in the learned form (qtn_mps_apply_layer_learned): returns
-    (kept, discarded weight, accepted) per gate."""
+    (kept, discarded weight, accepted) per gate; accepted is 1 where the
+    sketch decided the gate, -1 where it failed the check and the full SVD
+    ran, 0 where no sketch was tried."""
     n = len(sites)
     st = np.ascontiguousarray(sites, dtype=np.int32)
     g = np.ascontiguousarray(
@@ -349,4 +351,42 @@ def _apply_layer_learned(self, sites, gates, cutoff, chimax, hints):
     return kept, disc, acc
 
 
+def _gate_features(self, sites, schmidt, chimax):
+    """Predictor features (72 x len(sites) float32) of the gates on
+    (sites[i], sites[i] + 1), gathered on the device
+    (qtn_mps_gate_features); schmidt is len(sites) x 4."""
+    n = len(sites)
+    out = np.zeros((72, n), dtype=np.float32, order="F")
+    st = np.ascontiguousarray(sites, dtype=np.int32)
+    sc = np.ascontiguousarray(schmidt, dtype=np.float64).reshape(n, 4)
+    L.check(L.load().qtn_mps_gate_features(
+        self.handle, n, L.i32ptr(st), L.dptr(sc), int(chimax),
+        out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+    return out
+
+
+def _apply_layer_predicted(self, sites, gates, schmidt, cutoff, chimax, mlp,
+                           margin=0.0):
+    """A layer in the learned form with the predictor inside the library
+    (qtn_mps_apply_layer_predicted): returns (kept, discarded weight,
+    accepted, hints) per gate."""
+    n = len(sites)
+    st = np.ascontiguousarray(sites, dtype=np.int32)
+    g = np.ascontiguousarray(
+        np.stack([L.as_c128(x, "F").reshape(-1, order="F") for x in gates]))
+    sc = np.ascontiguousarray(schmidt, dtype=np.float64).reshape(n, 4)
+    kept = np.zeros(n, dtype=np.int64)
+    disc = np.zeros(n, dtype=np.float64)
+    acc = np.zeros(n, dtype=np.int32)
+    hints = np.zeros(n, dtype=np.int64)
+    L.check(L.load().qtn_mps_apply_layer_predicted(
+        self.handle, n, L.i32ptr(st),
+        g.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+        L.dptr(sc), float(cutoff), int(chimax), mlp.handle, float(margin),
+        L.i64ptr(kept), L.dptr(disc), L.i32ptr(acc), L.i64ptr(hints)))
+    return kept, disc, acc, hints
+
+
 DeviceMPS.apply_layer_learned = _apply_layer_learned
+DeviceMPS.gate_features = _gate_features
+DeviceMPS.apply_layer_predicted = _apply_layer_predicted
