@@ -105,6 +105,21 @@ def roofline(prof):
     return mfma
 
 
+def autotune(ctx, args, rows_cols, batch):
+    """Tune the Jacobi tilings of the buckets and batch sizes a workload
+    meets (outside the timed region); returns the table for the JSON line
+    (None without --autotune)."""
+    if not args.autotune:
+        return None
+    from qaintensor import autotune as A
+    t0 = time.perf_counter()
+    table = A.tune_jacobi(ctx, buckets=rows_cols, batches=batch)
+    print("autotune %.1f s: %s" % (time.perf_counter() - t0, {
+        k: v["best"] for k, v in sorted(table.items())}), file=sys.stderr,
+        flush=True)
+    return table
+
+
 def infidelity(a, b):
     """1 - |<a|b>|^2 / (<a|a><b|b>) (the truncated states are not exactly
     normalised)."""
@@ -221,6 +236,8 @@ def bench_batch(args):
     from qaintensor import (DeviceMPS, FORM_CHOSEN, FORM_PLAIN,
                             circuits as C, default_context, run_circuits)
     ctx = default_context()
+    tuned = autotune(ctx, args, ((64, 64), (128, 128), (256, 256)),
+                     (4, 16, 64))
     cutoff, count, seed = 1e-8, 256, 3
     circs = C.batch_circuits(count, seed)
     layers = [c[3] for c in circs]
@@ -299,6 +316,7 @@ def bench_batch(args):
             "kind": "port",
             "sample": "oracle on the first 8 circuits of the batch, "
                       "%.1f s" % dt}
+    line["autotune"] = tuned
     print(json.dumps(line))
 
 
@@ -322,6 +340,8 @@ def bench_learned(args):
                             circuits as C, default_context)
     from qaintensor import learned as Lrn
     ctx = default_context()
+    tuned = autotune(ctx, args, ((128, 128), (256, 256), (384, 384),
+                                 (512, 512)), (1, 4, 8, 16, 32))
     cutoff, chi, margin = 1e-8, 256, 0.02
     t0 = time.perf_counter()
     Xs, Ts = [], []
@@ -454,6 +474,7 @@ def bench_learned(args):
             "sample": "oracle, plain FP64, first %d layers (%d gates) of the "
                       "Haar circuit, %.1f s" % (li + 1, done, dt)}
     mlp.free()
+    line["autotune"] = tuned
     print(json.dumps(line))
 
 
@@ -471,6 +492,10 @@ def main():
                     help="after the run, copy a MIB-MiB device buffer 4 "
                          "times (a kernel of known traffic for calibrating "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the Jacobi tilings on synthetic problems "
+                         "before timing (the default tilings measured "
+                         "faster on the circuit workloads)")
     ap.add_argument("--only-batched", action="store_true",
                     help="time the headline form only (for profiler runs)")
     args = ap.parse_args()
@@ -494,6 +519,9 @@ def main():
                             circuits as C, default_context)
 
     ctx = default_context()
+    # per-gate plain (1 problem), chosen-form subsets and whole layers
+    tuned = autotune(ctx, args, ((128, 128), (256, 256), (384, 384),
+                                 (512, 512)), (1, 4, 8, 16))
     layers = C.brickwall_layers(N_QUBITS, DEPTH, SEED)
     n2q = C.count_2q(layers)
     pristine = DeviceMPS(N_QUBITS, CHIMAX, ctx)
@@ -618,6 +646,7 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         line["copy_probe_bytes"] = nb
+    line["autotune"] = tuned
     print(json.dumps(line))
 
 

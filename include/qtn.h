@@ -48,6 +48,11 @@ extern "C" {
 #define QTN_ERR_TRUNC      -5  /* every singular value is within the cutoff  */
 #define QTN_ERR_NOTFINITE  -6  /* NaN/Inf met in the input of an SVD         */
 
+/* Jacobi block-pair tilings (qtn_tune_jacobi, qtn_set_jacobi_tiling) */
+#define QTN_TILING_W32       0  /* 32x32 Gram blocks, three launches a round */
+#define QTN_TILING_W32_FUSED 1  /* 32x32 Gram blocks, one fused launch        */
+#define QTN_TILING_W64       2  /* 64x64 Gram blocks, three launches a round */
+
 /* precision forms of the two-site update */
 #define QTN_FORM_PLAIN      0  /* ComplexF64 end to end                       */
 #define QTN_FORM_CHOSEN     1  /* complex FP32 bulk, FP64 core (see docs)     */
@@ -318,6 +323,24 @@ int32_t qtn_mlp_train(qtn_mlp* mlp, int64_t nb, const float* X,
  * the one of src/mpo.jl:40-56 (MPO(m) of a two-site operator).            */
 int32_t qtn_gate_schmidt(qtn_ctx* ctx, int32_t ngates, const double* gates,
                          double* out);
+/* Autotuner of the Jacobi SVD engine.  Times each block-pair tiling on
+ * `batch` synthetic complex Gaussian rows x cols problems (FP64 sweeps, or
+ * FP32 sweeps when single != 0) with HIP events, best of two runs after a
+ * warm-up, and keeps the fastest for this context, column bucket (cols
+ * rounded up to a multiple of 64) and batch class (floor(log2(batch)),
+ * capped at 6): every later batched SVD of that bucket whose number of
+ * solved problems falls in the class uses it.  best receives the QTN_TILING_* code; us (3
+ * doubles, may be NULL) the microseconds per SVD of W32, W32_FUSED and
+ * W64 (-1 where a tiling does not apply).  No counterpart in the
+ * reference.                                                              */
+int32_t qtn_tune_jacobi(qtn_ctx* ctx, int64_t rows, int64_t cols,
+                        int32_t batch, int32_t single, int32_t* best,
+                        double* us);
+/* Sets the tiling of a bucket and batch class (e.g. from a saved table of
+ * qtn_tune_jacobi).                                                       */
+int32_t qtn_set_jacobi_tiling(qtn_ctx* ctx, int64_t cols, int32_t single,
+                              int32_t batch, int32_t tiling);
+
 /* The predictor's input for the gates on (sites[i], sites[i]+1) of a state:
  * out (72 x ngates floats, column-major, host) holds per gate the bond
  * spectrum before the gate (top 64 values, zero padded), the gate's four

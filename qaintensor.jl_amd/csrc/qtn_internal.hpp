@@ -4,7 +4,10 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <string>
+#include <tuple>
+#include <utility>
 #include <vector>
 
 #include "qtn.h"
@@ -102,6 +105,10 @@ struct qtn_ctx {
   int device = 0;
   uint64_t sketch_seed = 0x5eed;  // range-finder sketches (learned form)
   hipStream_t stream = nullptr;
+  // Jacobi block-pair tiling per (padded column count, FP32 sweeps, batch
+  // class of the active problems): set by qtn_tune_jacobi /
+  // qtn_set_jacobi_tiling (QTN_TILING_* codes).
+  std::map<std::tuple<int64_t, int, int>, int> jac_tiling;
   // Named scratch buffers grown on demand (never shrunk until destroy).
   std::vector<qtn::DevBuf> bufs;
   // work counters (see qtn_ctx_counters)

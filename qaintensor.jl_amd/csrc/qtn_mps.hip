@@ -863,7 +863,9 @@ int32_t rand_range_finder(qtn_ctx* ctx, int ng,
   QTN_CHECK_HIP(hipGetLastError());
   JacobiEngine eng;
   eng.slot_base = 160;
-  QTN_TRY(eng.reserve(ctx, ng, rowsB, colsB));
+  int nat = 0;
+  for (int g = 0; g < ng; ++g) nat += attempt[g] ? 1 : 0;
+  QTN_TRY(eng.reserve(ctx, ng, rowsB, colsB, false, nat));
   std::vector<int64_t> nsv(ng);
   for (int g = 0; g < ng; ++g) nsv[g] = attempt[g] ? lw[g] : 1;
   int sweeps = 0;
@@ -1157,7 +1159,9 @@ int32_t apply_gates_core(qtn_ctx* ctx, int ng, qtn_mps* const* ms,
                          dim3(256), 0, s, (const double2*)dX, (float2*)dX32,
                          (int64_t)ng * x_sz);
       QTN_CHECK_HIP(hipGetLastError());
-      QTN_TRY(eng32.reserve(ctx, ng, rows_p, cols_p, true));
+      int n32 = 0;
+      for (int g = 0; g < ng; ++g) n32 += cand[g];
+      QTN_TRY(eng32.reserve(ctx, ng, rows_p, cols_p, true, n32));
       eng32.chimax_per = chis;
       QTN_TRY(eng32.run32(ctx, (float2*)dX32, rows_p, x_sz, (float2*)dW32,
                           w_sz, any_w ? wantw.data() : nullptr, nsv.data(),
@@ -1216,7 +1220,9 @@ int32_t apply_gates_core(qtn_ctx* ctx, int ng, qtn_mps* const* ms,
       cols_eng = round_up64(c64);
     }
     eng.ldw = cols_p;  // W keeps the batch layout
-    QTN_TRY(eng.reserve(ctx, ng, rows_eng, cols_eng));
+    int n64 = 0;
+    for (int g = 0; g < ng; ++g) n64 += need64[g];
+    QTN_TRY(eng.reserve(ctx, ng, rows_eng, cols_eng, false, n64));
     eng.chimax_per = chis;
     QTN_TRY(eng.run(ctx, (double2*)dX, rows_p, x_sz, (double2*)dW, w_sz,
                     any_w ? wantw.data() : nullptr, nsv.data(), cutoff, 1,

@@ -1048,23 +1048,32 @@ __global__ __launch_bounds__(1024) void jac_sort_trunc(
 
 // ---------------------------------------------------------------------------
 int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
-                              int64_t cols_p_, bool single_) {
+                              int64_t cols_p_, bool single_, int n_active) {
   batch = batch_;
   single = single_;
   // The fused round (one workgroup per block pair) leaves each 64-row
   // chunk's load latency exposed with only 1-2 workgroups per CU; the
   // three-launch round overlaps them across all chunks and is faster at
   // chi = 256, so the fused round is opt-in (QTN_JACOBI_FUSED=1).
-  fused = false;
-  if (const char* e = getenv("QTN_JACOBI_FUSED")) fused = atoi(e) != 0;
   const size_t csz = single ? sizeof(float2) : sizeof(double2);
   rows_p = rows_p_;
   cols_p = cols_p_;
   // Block-pair width: one 64-column pair solved entirely in LDS for narrow
   // problems; 32-column pairs otherwise (the inner sweep's LDS traffic per
   // outer sweep grows linearly with W while the MFMA work does not change).
+  // The context's tuned tiling for this column bucket (qtn_tune_jacobi)
+  // replaces that default; QTN_JACOBI_W / QTN_JACOBI_FUSED override both.
+  fused = false;
   int wdef = 32;
+  const auto it = ctx->jac_tiling.find(
+      {cols_p, single ? 1 : 0,
+       batch_class(n_active >= 0 ? n_active : batch_)});
+  if (it != ctx->jac_tiling.end()) {
+    wdef = it->second == QTN_TILING_W64 ? 64 : 32;
+    fused = it->second == QTN_TILING_W32_FUSED;
+  }
   if (const char* e = getenv("QTN_JACOBI_W")) wdef = atoi(e) == 64 ? 64 : 32;
+  if (const char* e = getenv("QTN_JACOBI_FUSED")) fused = atoi(e) != 0;
   W = (cols_p <= 64) ? 64 : wdef;
   p = (int)(cols_p / W);
   const int64_t nchunk = std::max(rows_p, cols_p) / 64;
@@ -1449,4 +1458,161 @@ int32_t launch_identity(hipStream_t s, double2* W, int64_t n, int64_t sW,
   return launch_identity_t<double>(s, W, n, n, sW, batch);
 }
 
+namespace {
+
+// batch problems of rows_p x cols_p: complex Gaussian entries (counter
+// hash of the flat index, Box-Muller) in the leading rows x cols block,
+// zeros in the padding.
+template <typename R>
+__global__ void tune_fill(typename JT<R>::c_t* __restrict__ X, int64_t rows,
+                          int64_t cols, int64_t rows_p, int64_t cols_p,
+                          int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i % rows_p, c = (i / rows_p) % cols_p;
+    if (r >= rows || c >= cols) {
+      X[i] = JT<R>::mk((R)0, (R)0);
+      continue;
+    }
+    uint64_t z = 0x7a11ULL ^ ((uint64_t)i * 0x9e3779b97f4a7c15ULL);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    z ^= z >> 31;
+    const double u1 = ((double)(z >> 40) + 0.5) * (1.0 / 16777216.0);
+    const double u2 = (double)(z & 0xffffff) * (1.0 / 16777216.0);
+    const double rad = sqrt(-log(u1));
+    X[i] = JT<R>::mk((R)(rad * cos(6.283185307179586 * u2)),
+                     (R)(rad * sin(6.283185307179586 * u2)));
+  }
+}
+
+constexpr int kSlotTuneX = 180, kSlotTuneX0 = 181, kSlotTuneW = 182;
+
+// Time of one batched SVD (ms, best of `reps`) with tiling `code` forced
+// for the bucket; X is restored from X0 before every run.
+template <typename R>
+int32_t tune_time(qtn_ctx* ctx, int code, int batch, int64_t rows_p,
+                  int64_t cols_p, int64_t nsv, typename CType<R>::type* X,
+                  const typename CType<R>::type* X0,
+                  typename CType<R>::type* Wm, int reps, double* ms) {
+  const bool single = sizeof(R) == sizeof(float);
+  ctx->jac_tiling[{cols_p, single ? 1 : 0, batch_class(batch)}] = code;
+  hipStream_t s = ctx->stream;
+  const size_t xbytes =
+      (size_t)batch * rows_p * cols_p * sizeof(typename CType<R>::type);
+  std::vector<int64_t> nsvv(batch, nsv);
+  hipEvent_t e0, e1;
+  QTN_CHECK_HIP(hipEventCreate(&e0));
+  QTN_CHECK_HIP(hipEventCreate(&e1));
+  double best = -1.0;
+  for (int r = 0; r <= reps; ++r) {  // run 0 warms up (buffers, code)
+    QTN_CHECK_HIP(hipMemcpyAsync(X, X0, xbytes, hipMemcpyDeviceToDevice, s));
+    JacobiEngine eng;
+    eng.slot_base = 185;
+    QTN_TRY(eng.reserve(ctx, batch, rows_p, cols_p, single));
+    int sweeps = 0;
+    QTN_CHECK_HIP(hipEventRecord(e0, s));
+    const int32_t st = eng.run_t<R>(ctx, X, rows_p, rows_p * cols_p, Wm,
+                                    cols_p * cols_p, nullptr, nsvv.data(),
+                                    1e-8, 1, 0, 60, &sweeps, nullptr);
+    QTN_CHECK_HIP(hipEventRecord(e1, s));
+    QTN_CHECK_HIP(hipEventSynchronize(e1));
+    if (st != QTN_OK) {
+      hipEventDestroy(e0);
+      hipEventDestroy(e1);
+      return st;
+    }
+    float t = 0.f;
+    QTN_CHECK_HIP(hipEventElapsedTime(&t, e0, e1));
+    if (r > 0 && (best < 0 || t < best)) best = t;
+  }
+  QTN_CHECK_HIP(hipEventDestroy(e0));
+  QTN_CHECK_HIP(hipEventDestroy(e1));
+  *ms = best;
+  return QTN_OK;
+}
+
+template <typename R>
+int32_t tune_bucket(qtn_ctx* ctx, int64_t rows, int64_t cols, int batch,
+                    int* best_code, double* us) {
+  using C = typename CType<R>::type;
+  const bool single = sizeof(R) == sizeof(float);
+  const int64_t rows_p = round_up64(std::max(rows, cols));
+  const int64_t cols_p = round_up64(cols);
+  const int64_t nsv = std::min(rows, cols);
+  const size_t xn = (size_t)batch * rows_p * cols_p;
+  void *px, *px0, *pw;
+  QTN_TRY(ctx->buffer(kSlotTuneX, xn * sizeof(C), &px));
+  QTN_TRY(ctx->buffer(kSlotTuneX0, xn * sizeof(C), &px0));
+  QTN_TRY(ctx->buffer(kSlotTuneW, (size_t)batch * cols_p * cols_p * sizeof(C),
+                      &pw));
+  hipStream_t s = ctx->stream;
+  hipLaunchKernelGGL(tune_fill<R>,
+                     dim3((unsigned)std::min<int64_t>((xn + 255) / 256, 8192)),
+                     dim3(256), 0, s, (C*)px0, rows, cols, rows_p, cols_p,
+                     (int64_t)xn);
+  QTN_CHECK_HIP(hipGetLastError());
+  const int codes[3] = {QTN_TILING_W32, QTN_TILING_W32_FUSED,
+                        QTN_TILING_W64};
+  double best = -1.0;
+  int bc = QTN_TILING_W32;
+  for (int i = 0; i < 3; ++i) {
+    double ms = -1.0;
+    // one 64-column bucket has a single tiling (one block pair)
+    const bool applies = cols_p > 64 || codes[i] == QTN_TILING_W64;
+    if (applies)
+      QTN_TRY(tune_time<R>(ctx, codes[i], batch, rows_p, cols_p, nsv, (C*)px,
+                           (const C*)px0, (C*)pw, 2, &ms));
+    if (us) us[i] = ms >= 0 ? ms * 1e3 / batch : -1.0;
+    if (ms >= 0 && (best < 0 || ms < best)) {
+      best = ms;
+      bc = codes[i];
+    }
+  }
+  ctx->jac_tiling[{cols_p, single ? 1 : 0, batch_class(batch)}] = bc;
+  *best_code = bc;
+  return QTN_OK;
+}
+
+}  // namespace
+
 }  // namespace qtn
+
+using namespace qtn;
+
+extern "C" {
+
+int32_t qtn_tune_jacobi(qtn_ctx* ctx, int64_t rows, int64_t cols,
+                        int32_t batch, int32_t single, int32_t* best,
+                        double* us) {
+  clear_error();
+  QTN_ARG(ctx && best, "qtn_tune_jacobi: NULL argument");
+  QTN_ARG(rows >= 1 && cols >= 1 && rows <= 4096 && cols <= 4096,
+          "qtn_tune_jacobi: sizes must lie in [1, 4096]");
+  QTN_ARG(batch >= 1 && batch <= 1024, "qtn_tune_jacobi: batch in [1, 1024]");
+  QTN_CHECK_HIP(hipSetDevice(ctx->device));
+  int code = 0;
+  if (single)
+    QTN_TRY(tune_bucket<float>(ctx, rows, cols, batch, &code, us));
+  else
+    QTN_TRY(tune_bucket<double>(ctx, rows, cols, batch, &code, us));
+  *best = code;
+  return QTN_OK;
+}
+
+int32_t qtn_set_jacobi_tiling(qtn_ctx* ctx, int64_t cols, int32_t single,
+                              int32_t batch, int32_t tiling) {
+  clear_error();
+  QTN_ARG(ctx, "qtn_set_jacobi_tiling: NULL context");
+  QTN_ARG(cols >= 1 && cols <= 4096,
+          "qtn_set_jacobi_tiling: cols must lie in [1, 4096]");
+  QTN_ARG(tiling == QTN_TILING_W32 || tiling == QTN_TILING_W32_FUSED ||
+              tiling == QTN_TILING_W64,
+          "qtn_set_jacobi_tiling: unknown tiling");
+  QTN_ARG(batch >= 1, "qtn_set_jacobi_tiling: batch must be positive");
+  ctx->jac_tiling[{round_up64(cols), single ? 1 : 0, batch_class(batch)}] =
+      tiling;
+  return QTN_OK;
+}
+
+}  // extern "C"

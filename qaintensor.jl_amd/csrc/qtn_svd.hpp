@@ -57,8 +57,10 @@ struct JacobiEngine {
   int32_t account_sweep(int n_active, int qpairs, int nchunk);
   int* perm = nullptr;    // [batch][cols_p], column of X for each sigma
 
+  // n_active: problems that will actually be solved (the rest are masked;
+  // < 0: all of them); it selects the tuned tiling's batch class.
   int32_t reserve(qtn_ctx* ctx, int batch, int64_t rows_p, int64_t cols_p,
-                  bool single = false);
+                  bool single = false, int n_active = -1);
   // Orthogonalise the columns of every X (rows_p x cols_p, ld rows_p,
   // batch stride sX); if W is given (cols_p x cols_p, stride sW) it
   // accumulates the rotations starting from the identity, for the problems
@@ -100,6 +102,14 @@ int32_t launch_identity_t(hipStream_t s, typename CType<R>::type* W,
                           int64_t n, int64_t ld, int64_t sW, int batch);
 
 inline int64_t round_up64(int64_t x) { return ((x + 63) / 64) * 64; }
+
+// Batch class of the tuned tilings: floor(log2(active problems)), 0..6
+// (1, 2-3, 4-7, ..., 64 and more).
+inline int batch_class(int n) {
+  int c = 0;
+  while (c < 6 && (2 << c) <= n) ++c;
+  return c;
+}
 
 // SVD of a device matrix (defined in qtn_svd_api.hip; contract documented
 // there).  Uses context slots 12 and 13.

@@ -2,7 +2,7 @@
 API over libqtn (hand-written gfx950 kernels behind a C ABI)."""
 from ._lib import (Context, QtnError, QtnLibraryError, default_context,
                    load as load_library, lib_path)
-from . import circuits
+from . import autotune, circuits, learned
 from .device import (DeviceMPO, DeviceMPS, FORM_PLAIN, FORM_CHOSEN,
                      apply_1q_multi, apply_layer_multi, run_circuits)
 from .network import (ErrorException, GeneralTensorNetwork, Summation,

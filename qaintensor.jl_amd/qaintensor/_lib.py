@@ -38,7 +38,7 @@ EXPORTS = [
     "qtn_mps_apply_1q", "qtn_mps_apply_2q", "qtn_mps_apply_layer",
     "qtn_mps_apply_layer_multi", "qtn_mps_apply_1q_multi",
     "qtn_mps_apply_layer_learned", "qtn_mps_gate_features",
-    "qtn_gate_schmidt",
+    "qtn_gate_schmidt", "qtn_tune_jacobi", "qtn_set_jacobi_tiling",
     "qtn_mps_apply_layer_predicted",
     "qtn_mps_amplitudes", "qtn_mps_expect_z", "qtn_mps_overlap",
     "qtn_mps_norm2", "qtn_mpo_create", "qtn_mpo_destroy",
@@ -128,6 +128,8 @@ def _declare(lib):
                                                _i64, _pi64, _pi64, _pd,
                                                _pi32]),
         "qtn_gate_schmidt": (_i32, [_vp, _i32, _pd, _pd]),
+        "qtn_tune_jacobi": (_i32, [_vp, _i64, _i64, _i32, _i32, _pi32, _pd]),
+        "qtn_set_jacobi_tiling": (_i32, [_vp, _i64, _i32, _i32, _i32]),
         "qtn_mps_gate_features": (_i32, [_vp, _i32, _pi32, _pd, _i64,
                                          _pf]),
         "qtn_mps_apply_layer_predicted": (_i32, [_vp, _i32, _pi32, _pd, _pd,

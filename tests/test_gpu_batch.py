@@ -56,3 +56,37 @@ def test_multi_rejects_overlap(gpu_ctx):
     g = np.eye(4, dtype=complex)
     with pytest.raises(QtnError, match="overlap"):
         apply_layer_multi([d, d], [2, 3], [g, g], [8, 8], 1e-10)
+
+
+def test_jacobi_autotuner_table_and_results(gpu_ctx):
+    """qtn_tune_jacobi times the tilings of a bucket and keeps the fastest;
+    states run under each forced tiling agree with the default run to the
+    plain form's bar.  Private contexts keep the tuned tables away from the
+    other tests."""
+    from qaintensor import Context, autotune as A
+    tctx = Context(0)
+    table = A.tune_jacobi(tctx, buckets=((128, 128), (256, 192)),
+                          batches=(1, 5), precisions=("fp64", "fp32"))
+    assert set(table) == {"%s:%d:b%d" % (p, c, b) for p in ("fp64", "fp32")
+                          for c in (128, 192) for b in (1, 4)}
+    for entry in table.values():
+        us = entry["us_per_svd"]
+        assert us["w32"] > 0 and us["w64"] > 0
+        assert entry["best"] == min((v, k) for k, v in us.items()
+                                    if v is not None)[1]
+    n, chi, cutoff = 14, 64, 1e-10
+    layers = C.brickwall_layers(n, 10, 21)
+    for tiling in ("w32", "w32_fused", "w64"):
+        c = Context(0)
+        ref = DeviceMPS(n, chi, c)  # default tilings
+        ref.run_layers(layers, cutoff, chi, batched=True)
+        # every batch class of the 128-column bucket
+        A.apply_table({"fp64:128:b%d" % (1 << k): {"best": tiling}
+                       for k in range(7)}, c)
+        dev = DeviceMPS(n, chi, c)
+        dev.run_layers(layers, cutoff, chi, batched=True)
+        assert list(dev.bond_dims()) == list(ref.bond_dims())
+        ov = dev.overlap(ref)
+        assert abs(1 - abs(ov) ** 2 / (dev.norm2() * ref.norm2())) <= 1e-10
+        dev.free()
+        ref.free()
