@@ -154,11 +154,13 @@ def bench_tfim(args):
     """64-site transverse-field Ising MPO (D = 3, J = 1, h_i ~ U[0.5, 1.5])
     on a random complex Gaussian MPS with bonds min(1024, 2^j, 2^(64-j)):
     set_sites (canonical form on the GPU), <psi|H|psi> by the environment
-    sweep, and W|psi> by zip-up + recompression to chi 1024 (cutoff 1e-10),
-    whose <psi|W psi> is compared with the exact value.  The CPU leg is the
+    sweep in complex FP64 and with the bond-sized contractions as BF16x3
+    split-precision GEMMs (the chosen form), and W|psi> by zip-up +
+    recompression to chi 1024 (cutoff 1e-10), whose <psi|W psi> is compared
+    with the exact value.  The CPU leg is the
     oracle's environment sweep on the same (left-canonicalised) state."""
     import torch
-    from qaintensor import DeviceMPO, DeviceMPS, default_context
+    from qaintensor import DeviceMPO, DeviceMPS, FORM_CHOSEN, default_context
     from qaintensor import hamiltonians as Hm
     n, chi, J, seed, cutoff = 64, 1024, 1.0, 4, 1e-10
     ctx = default_context()
@@ -176,11 +178,19 @@ def bench_tfim(args):
     torch.cuda.synchronize()
     t_set = time.perf_counter() - t0
     print("set_sites %.2f s" % t_set, file=sys.stderr, flush=True)
+    e_gpu = psi.expect_mpo(mpo)  # warm-up (scratch buffers)
+    e_bf = psi.expect_mpo(mpo, FORM_CHOSEN)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     e_gpu = psi.expect_mpo(mpo)
     torch.cuda.synchronize()
     t_exp = time.perf_counter() - t0
-    print("expect_mpo %.3f s" % t_exp, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    e_bf = psi.expect_mpo(mpo, FORM_CHOSEN)
+    torch.cuda.synchronize()
+    t_bf = time.perf_counter() - t0
+    print("expect_mpo FP64 %.3f s, BF16x3 %.3f s" % (t_exp, t_bf),
+          file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     nrm = psi.apply_mpo(mpo, cutoff, chi, out)
     torch.cuda.synchronize()
@@ -198,8 +208,11 @@ def bench_tfim(args):
                                "recompression to chi 1024",
                    "n_sites": n, "chi": chi, "cutoff": cutoff},
         "expect_gpu": [e_gpu.real, e_gpu.imag],
+        "expect_bf16x3": [e_bf.real, e_bf.imag],
+        "bf16x3_rel_error_vs_fp64": abs(e_bf - e_gpu) / abs(e_gpu),
         "expect_zipup": [e_zip.real, e_zip.imag],
         "seconds": {"set_sites": t_set, "expect_mpo": t_exp,
+                    "expect_mpo_bf16x3": t_bf,
                     "apply_mpo_zipup": t_zip},
         "out_bond_max": int(max(out.bond_dims())),
     }
@@ -211,6 +224,7 @@ def bench_tfim(args):
         t_cpu = time.perf_counter() - t0
         line["expect_oracle"] = [e_ref.real, e_ref.imag]
         line["value"] = abs(e_gpu - e_ref) / abs(e_ref)
+        line["bf16x3_rel_error"] = abs(e_bf - e_ref) / abs(e_ref)
         line["zipup_rel_error"] = abs(e_zip - e_ref) / abs(e_ref)
         line["cpu_baseline"] = {
             "value": t_cpu, "unit": "s per <psi|H|psi>",

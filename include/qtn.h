@@ -284,6 +284,20 @@ int32_t qtn_mpo_destroy(qtn_mpo* mpo);
  * MFMA GEMMs; psi is normalised in the device gauge.                      */
 int32_t qtn_mps_expect_mpo(const qtn_mps* mps, const qtn_mpo* mpo,
                            double* out);
+/* The same sweep in a precision form: QTN_FORM_PLAIN (= qtn_mps_expect_mpo,
+ * complex FP64 throughout) or QTN_FORM_CHOSEN, where each site's two
+ * bond-sized contractions run as BF16x3 split-precision GEMMs (three bf16
+ * terms per operand, six products, FP32 accumulation on the bf16 matrix
+ * cores) once all of their dimensions reach 512; smaller ones stay FP64.  */
+int32_t qtn_mps_expect_mpo_form(const qtn_mps* mps, const qtn_mpo* mpo,
+                                int32_t form, double* out);
+/* C (m x n) = op(A) B in BF16x3 split precision (as above) on host
+ * complex FP64 arrays, column-major; opA 'N' (A is m x k) or 'C' (A is
+ * k x m, op = conjugate transpose).  No counterpart in the reference.     */
+int32_t qtn_zgemm_bf16x3(qtn_ctx* ctx, char opA, int64_t m, int64_t n,
+                         int64_t k, const double* A, int64_t lda,
+                         const double* B, int64_t ldb, double* C,
+                         int64_t ldc);
 /* out <- W|in> compressed: zip-up left to right (truncation per bond with
  * the relative cutoff and chimax), then a right-to-left SVD sweep that
  * recompresses with the same rule into out's right-canonical gauge.

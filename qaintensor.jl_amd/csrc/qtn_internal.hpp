@@ -92,6 +92,13 @@ int32_t launch_cgemm_grouped(hipStream_t s, int opA, int opB,
 int32_t launch_permute(hipStream_t s, const double2* in, int nd,
                        const int64_t* dims, const int* perm0, double2* out);
 
+// BF16x3 split-precision complex GEMM (qtn_bf16x3.hip): C = op(A) B, opA
+// 0 (N) or 2 (C), C overwritten; scratch in context slots 190-191.
+int32_t launch_zgemm_bf16x3(qtn_ctx* ctx, int opA, int64_t m, int64_t n,
+                            int64_t k, const double2* A, int64_t lda,
+                            const double2* B, int64_t ldb, double2* C,
+                            int64_t ldc);
+
 // Rank-predictor MLP (qtn_mlp.hip): prediction on device features, Y
 // points into the MLP's own scratch.
 int32_t mlp_predict_device(qtn_mlp* m, int64_t nb, const float* X,

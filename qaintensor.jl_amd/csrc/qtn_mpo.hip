@@ -242,10 +242,30 @@ int32_t qtn_mpo_destroy(qtn_mpo* m) {
 }
 
 int32_t qtn_mps_expect_mpo(const qtn_mps* m, const qtn_mpo* w, double* out) {
+  return qtn_mps_expect_mpo_form(m, w, QTN_FORM_PLAIN, out);
+}
+
+int32_t qtn_mps_expect_mpo_form(const qtn_mps* m, const qtn_mpo* w,
+                                int32_t form, double* out) {
   clear_error();
   QTN_ARG(m && w && out, "qtn_mps_expect_mpo: NULL argument");
   QTN_ARG(m->n == w->n, "qtn_mps_expect_mpo: MPS and MPO lengths differ");
   QTN_ARG(m->ctx == w->ctx, "qtn_mps_expect_mpo: different contexts");
+  QTN_ARG(form == QTN_FORM_PLAIN || form == QTN_FORM_CHOSEN,
+          "qtn_mps_expect_mpo: form must be QTN_FORM_PLAIN or "
+          "QTN_FORM_CHOSEN");
+  // chosen form: the two chi-sized contractions per site run as BF16x3
+  // split-precision GEMMs once every dimension reaches 512
+  const bool split = form == QTN_FORM_CHOSEN;
+  auto big_gemm = [&](hipStream_t st, int opA, int64_t mm, int64_t nn,
+                      int64_t kk, const double2* A, int64_t lda,
+                      const double2* B, int64_t ldb, double2* C,
+                      int64_t ldc) -> int32_t {
+    if (split && mm >= 512 && nn >= 512 && kk >= 512)
+      return launch_zgemm_bf16x3(m->ctx, opA, mm, nn, kk, A, lda, B, ldb, C,
+                                 ldc);
+    return gemm(st, opA, 0, mm, nn, kk, A, lda, B, ldb, C, ldc);
+  };
   qtn_ctx* ctx = m->ctx;
   hipStream_t s = ctx->stream;
   QTN_CHECK_HIP(hipSetDevice(ctx->device));
@@ -271,8 +291,8 @@ int32_t qtn_mps_expect_mpo(const qtn_mps* m, const qtn_mpo* w, double* out) {
     double2* T1 = (double2*)dT1;
     double2* T2 = (double2*)dT2;
     // T1[a, d, s, b'] = sum_a' L[a, d, a'] B[a', s, b']
-    QTN_TRY(gemm(s, 0, 0, cl * Dl, 2 * cr, cl, L, cl * Dl, m->site[j], cl,
-                 T1, cl * Dl));
+    QTN_TRY(big_gemm(s, 0, cl * Dl, 2 * cr, cl, L, cl * Dl, m->site[j], cl,
+                     T1, cl * Dl));
     // T2 = T1 as (a, b', d, s)
     QTN_TRY(permute(s, T1, {cl, Dl, 2, cr}, {0, 3, 1, 2}, T2));
     // Wp = W as (d, s, t, e)
@@ -284,8 +304,8 @@ int32_t qtn_mps_expect_mpo(const qtn_mps* m, const qtn_mpo* w, double* out) {
     // T2 = T1 as (a, t, e, b')
     QTN_TRY(permute(s, T1, {cl, cr, 2, Dr}, {0, 2, 3, 1}, T2));
     // L'[b, e, b'] = sum_{a,t} conj(B[a, t, b]) T2[a, t, e, b']
-    QTN_TRY(gemm(s, 2, 0, cr, Dr * cr, 2 * cl, m->site[j], 2 * cl, T2,
-                 2 * cl, L, cr));
+    QTN_TRY(big_gemm(s, 2, cr, Dr * cr, 2 * cl, m->site[j], 2 * cl, T2,
+                     2 * cl, L, cr));
   }
   QTN_CHECK_HIP(hipMemcpyAsync(out, dL, sizeof(double2),
                                hipMemcpyDeviceToHost, s));

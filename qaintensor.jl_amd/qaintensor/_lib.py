@@ -42,7 +42,8 @@ EXPORTS = [
     "qtn_mps_apply_layer_predicted",
     "qtn_mps_amplitudes", "qtn_mps_expect_z", "qtn_mps_overlap",
     "qtn_mps_norm2", "qtn_mpo_create", "qtn_mpo_destroy",
-    "qtn_mps_expect_mpo", "qtn_mps_apply_mpo",
+    "qtn_mps_expect_mpo", "qtn_mps_apply_mpo", "qtn_mps_expect_mpo_form",
+    "qtn_zgemm_bf16x3",
     "qtn_mlp_create", "qtn_mlp_destroy", "qtn_mlp_get_params",
     "qtn_mlp_forward", "qtn_mlp_loss_grad", "qtn_mlp_train",
 ]
@@ -143,6 +144,9 @@ def _declare(lib):
                                   ctypes.POINTER(_vp)]),
         "qtn_mpo_destroy": (_i32, [_vp]),
         "qtn_mps_expect_mpo": (_i32, [_vp, _vp, _pd]),
+        "qtn_mps_expect_mpo_form": (_i32, [_vp, _vp, _i32, _pd]),
+        "qtn_zgemm_bf16x3": (_i32, [_vp, ctypes.c_char, _i64, _i64, _i64, _pd,
+                                    _i64, _pd, _i64, _pd, _i64]),
         "qtn_mps_apply_mpo": (_i32, [_vp, _vp, _dbl, _i64, _vp, _pd]),
         "qtn_mlp_create": (_i32, [_vp, _i32, _i32, _pf, ctypes.POINTER(_vp)]),
         "qtn_mlp_destroy": (_i32, [_vp]),

@@ -191,11 +191,13 @@ def _set_sites(self, tensors):
     L.check(L.load().qtn_mps_set_sites(self.handle, L.i64ptr(dims), ptrs))
 
 
-def _expect_mpo(self, mpo):
-    """<psi|W|psi> (complex) on the GPU."""
+def _expect_mpo(self, mpo, form=FORM_PLAIN):
+    """<psi|W|psi> (complex) by the environment sweep; form FORM_CHOSEN runs
+    the bond-sized contractions (all dimensions >= 512) as BF16x3
+    split-precision GEMMs (qtn_mps_expect_mpo_form)."""
     out = np.zeros(2, dtype=np.float64)
-    L.check(L.load().qtn_mps_expect_mpo(self.handle, mpo.handle,
-                                        L.dptr(out)))
+    L.check(L.load().qtn_mps_expect_mpo_form(self.handle, mpo.handle,
+                                             int(form), L.dptr(out)))
     return complex(out[0], out[1])
 
 

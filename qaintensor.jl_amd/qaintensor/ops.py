@@ -103,6 +103,26 @@ def svd_values(A, ctx=None):
     return S
 
 
+def zgemm_bf16x3(A, B, opA="N", ctx=None):
+    """op(A) @ B in BF16x3 split precision on the GPU (FP32-level error;
+    qtn_zgemm_bf16x3); opA "N" or "C"."""
+    ctx = _ctx(ctx)
+    A = L.as_c128(A, "F")
+    B = L.as_c128(B, "F")
+    if opA not in ("N", "C"):
+        raise ValueError("opA must be 'N' or 'C'")
+    m = A.shape[0] if opA == "N" else A.shape[1]
+    k = A.shape[1] if opA == "N" else A.shape[0]
+    if B.shape[0] != k:
+        raise ValueError("inner dimensions differ")
+    n = B.shape[1]
+    C = np.zeros((m, n), dtype=np.complex128, order="F")
+    L.check(L.load().qtn_zgemm_bf16x3(
+        ctx.handle, opA.encode(), m, n, k, L.dptr(A), max(1, A.shape[0]),
+        L.dptr(B), max(1, k), L.dptr(C), max(1, m)))
+    return C
+
+
 def svd(A, absorb=False, ctx=None):
     """Thin SVD without truncation on the GPU (qtn_svd): (U, S, Vh) with all
     min(m, n) singular values, descending; absorb=True returns diag(S) @ Vh

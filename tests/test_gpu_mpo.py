@@ -1,12 +1,14 @@
 """MPO x MPS on the GPU (qtn_mpo_*, qtn_mps_set_sites, qtn_mps_expect_mpo,
 qtn_mps_apply_mpo) against the oracle (tfim_mpo, canonicalize, expect_mpo,
 zipup) and the dense Hamiltonian.  Complex FP64; tolerance 1e-10 relative
-on values, identical bond dimensions."""
+on values, identical bond dimensions.  The BF16x3 split-precision GEMM and
+the chosen-form expectation sweep are held to FP32-level error."""
 import numpy as np
 import pytest
 
 from oracle import qtn_oracle as O
-from qaintensor import DeviceMPO, DeviceMPS
+from qaintensor import DeviceMPO, DeviceMPS, FORM_CHOSEN
+from qaintensor import hamiltonians as Hm
 
 pytestmark = pytest.mark.gpu
 
@@ -97,3 +99,43 @@ def test_apply_mpo_truncated_matches_oracle_zipup(gpu_ctx, n, chi, cap):
     e = dev.expect_mpo(DeviceMPO(W, gpu_ctx))
     e_ref = O.expect_mpo(Bs, W)
     assert abs(e - e_ref) <= 1e-10 * abs(e_ref)
+
+
+@pytest.mark.parametrize("op,m,n,k", [("N", 200, 130, 300), ("C", 257, 64, 129),
+                                      ("N", 640, 512, 1024)])
+def test_zgemm_bf16x3_split_precision(gpu_ctx, op, m, n, k):
+    """BF16x3 (three bf16 terms, six products, FP32 accumulation): entries
+    within FP32-level error of the FP64 product, relative to the row and
+    column norms that bound them, across tile padding; not the FP64 result
+    itself (the split path ran)."""
+    from qaintensor import ops
+    rng = np.random.default_rng(m + n + k)
+    shape = (m, k) if op == "N" else (k, m)
+    A = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    B = rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))
+    C = ops.zgemm_bf16x3(A, B, op, ctx=gpu_ctx)
+    opA = A if op == "N" else A.conj().T
+    ref = opA @ B
+    bound = np.outer(np.linalg.norm(opA, axis=1), np.linalg.norm(B, axis=0))
+    rel = np.max(np.abs(C - ref) / bound)
+    assert rel < 2e-6
+    assert rel > 1e-12
+
+
+def test_expect_mpo_bf16x3_form(gpu_ctx):
+    """<psi|H|psi> of a chi = 512 state: the chosen form (BF16x3 on the
+    contractions with every dimension >= 512) within FP32-level error of
+    the FP64 sweep, which matches the oracle."""
+    n, chi = 22, 512
+    W = Hm.tfim_mpo(n, 1.0, Hm.tfim_fields(n, 9))
+    raw = Hm.random_mps_tensors(n, chi, 9)
+    psi = DeviceMPS(n, chi, gpu_ctx)
+    psi.set_sites(raw)
+    mpo = DeviceMPO(W, gpu_ctx)
+    e64 = psi.expect_mpo(mpo)
+    e16 = psi.expect_mpo(mpo, FORM_CHOSEN)
+    ref = O.expect_mpo_blas(O.left_canonical(raw), W)
+    assert abs(e64 - ref) <= 1e-10 * abs(ref)
+    rel = abs(e16 - e64) / abs(e64)
+    assert rel <= 1e-5
+    assert rel > 0.0
