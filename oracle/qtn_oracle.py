@@ -55,6 +55,10 @@ PINNING = {
     "expect_mpo_blas": "equals expect_mpo (einsum form) on random states",
     "zipup": "without truncation equals the dense H psi (n <= 10); the "
              "truncated result's lambda and norm follow the same rule",
+    "mlp_forward": "torch nn.Linear / exact-erf GELU layers built from the "
+                   "flat parameter layout, float64 (test_oracle.py)",
+    "mlp_loss_grad": "central finite differences of the loss over random "
+                     "parameters of every block",
 }
 
 

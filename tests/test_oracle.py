@@ -418,3 +418,35 @@ def test_mlp_oracle_gradient_matches_finite_differences():
     y = O.mlp_forward(p, X, nin, w)[0]
     l, _ = O.mlp_loss_grad(p, X, T, nin, w)
     assert abs(l - np.mean((y - T) ** 2)) < 1e-12
+
+
+def test_mlp_oracle_forward_matches_torch():
+    """The oracle's forward pass against torch's own layers (nn.Linear,
+    exact-erf GELU) in float64, built from the documented flat layout; the
+    first 64 inputs enter as log10(max(x, 1e-16)) / 8."""
+    import torch
+    nin, w, B = 72, 16, 9
+    rng = np.random.default_rng(3)
+    p = rng.standard_normal(O.mlp_nparams(nin, w)) * 0.3
+    X = np.abs(rng.standard_normal((nin, B)))
+    X[5, 0] = 0.0  # a zero spectrum entry takes the 1e-16 floor
+    o = 0
+    W1 = p[o:o + w * nin].reshape((w, nin), order="F"); o += w * nin
+    b1 = p[o:o + w]; o += w
+    W2 = p[o:o + w * w].reshape((w, w), order="F"); o += w * w
+    b2 = p[o:o + w]; o += w
+    W3 = p[o:o + w].reshape((1, w)); o += w
+    b3 = p[o:o + 1]
+    net = torch.nn.Sequential(torch.nn.Linear(nin, w), torch.nn.GELU(),
+                              torch.nn.Linear(w, w), torch.nn.GELU(),
+                              torch.nn.Linear(w, 1)).double()
+    with torch.no_grad():
+        for lin, (Wt, bt) in zip([net[0], net[2], net[4]],
+                                 [(W1, b1), (W2, b2), (W3, b3)]):
+            lin.weight.copy_(torch.from_numpy(np.ascontiguousarray(Wt)))
+            lin.bias.copy_(torch.from_numpy(np.ascontiguousarray(bt)))
+        inp = X.copy()
+        inp[:64] = np.log10(np.maximum(inp[:64], 1e-16)) / 8.0
+        ref = net(torch.from_numpy(inp.T.copy())).numpy().T
+    got = O.mlp_forward(p, X, nin, w)
+    assert np.max(np.abs(got - ref)) <= 1e-12
