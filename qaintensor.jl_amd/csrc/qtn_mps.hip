@@ -404,7 +404,7 @@ struct RandDesc {
   int64_t ldx;
   int trans;
   int64_t M, N, l;   // theta dims, sketch width
-  double2* Om;       // N x l sketch (ld N)
+  double2* Om;       // N x l sketch (ld: the batch's largest N)
   double2* Y;        // M x l (ld M)
   double2* Z;        // N x l (ld N)
   double2* Q;        // rows x l (ld: the batch's largest max(M, N))
@@ -417,9 +417,12 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
-// Om = complex Gaussian (N(0,1/2) + i N(0,1/2)) from a counter hash and the
-// Box-Muller transform; deterministic in (seed, gate, element).
-__global__ void gauss_fill(const RandDesc* __restrict__ rd, uint64_t seed) {
+// Om (N x l of the gate, ld ldo: the batch-wide leading dimension the
+// products read it with) = complex Gaussian (N(0,1/2) + i N(0,1/2)) from a
+// counter hash and the Box-Muller transform; deterministic in (seed, gate,
+// element).
+__global__ void gauss_fill(const RandDesc* __restrict__ rd, uint64_t seed,
+                           int64_t ldo) {
   const int bz = blockIdx.y;
   const RandDesc d = rd[bz];
   const int64_t total = d.N * d.l;
@@ -431,7 +434,7 @@ __global__ void gauss_fill(const RandDesc* __restrict__ rd, uint64_t seed) {
     const double u2 = (double)(h2 >> 11) * (1.0 / 9007199254740992.0);
     const double r = sqrt(-log(u1));  // sqrt(-2 ln u) / sqrt(2)
     const double t = 6.283185307179586 * u2;
-    d.Om[i] = make_double2(r * cos(t), r * sin(t));
+    d.Om[(i % d.N) + ldo * (i / d.N)] = make_double2(r * cos(t), r * sin(t));
   }
 }
 
@@ -731,7 +734,7 @@ int32_t rand_range_finder(qtn_ctx* ctx, int ng,
   QTN_CHECK_HIP(hipMemcpyAsync(d_rd, rd.data(), ng * sizeof(RandDesc),
                                hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(gauss_fill, dim3(grid_for(maxN * maxl), (unsigned)ng),
-                     dim3(256), 0, s, (const RandDesc*)d_rd, seed);
+                     dim3(256), 0, s, (const RandDesc*)d_rd, seed, maxN);
   QTN_CHECK_HIP(hipGetLastError());
   // descriptors of the four products, split by the orientation of X:
   //   Y = theta Om | theta Q   (M x l): trans X^H B, else X B
