@@ -193,7 +193,12 @@ __device__ __forceinline__ int inner_solve(
   const int tid = threadIdx.x;
   // Largest scaled off-diagonal |G_pq|^2 / (G_pp G_qq) of the block; nothing
   // to do when it meets the tolerance (the common case in late sweeps).
-  R omax = 0;
+  // Reduced per wave (shuffles), then across the 4 waves in LDS, so the
+  // problem-wide maximum takes one global atomic per workgroup.
+  __shared__ double wmax[4];
+  __shared__ int woff[4];
+  double omax = 0.0;
+  bool off = false;
   for (int idx = tid; idx < W * W; idx += 256) {
     const int r = idx % W, c = idx / W;
     if (r < c) {
@@ -201,15 +206,28 @@ __device__ __forceinline__ int inner_solve(
       if (a > zt && d > zt) {
         const c_t g = G[r][c];
         const R o2 = (g.x * g.x + g.y * g.y) / (a * d);
-        omax = fmax(omax, o2);
-        if (o2 > tol2) atomicOr(&any_off, 1);
+        omax = fmax(omax, (double)o2);
+        off = off || o2 > tol2;
       }
     }
   }
-  // positive doubles order like their bit patterns
-  if (omax > (R)0)
-    atomicMax((unsigned long long*)offmax_bz,
-              (unsigned long long)__double_as_longlong((double)omax));
+#pragma unroll
+  for (int sh = 32; sh > 0; sh >>= 1)
+    omax = fmax(omax, __shfl_xor(omax, sh, 64));
+  const bool woff_any = __any(off);
+  if ((tid & 63) == 0) {
+    wmax[tid >> 6] = omax;
+    woff[tid >> 6] = woff_any ? 1 : 0;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const double bm = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+    // positive doubles order like their bit patterns
+    if (bm > 0.0)
+      atomicMax((unsigned long long*)offmax_bz,
+                (unsigned long long)__double_as_longlong(bm));
+    any_off = woff[0] | woff[1] | woff[2] | woff[3];
+  }
   __syncthreads();
   if (!any_off) return 0;
   const int pr = tid / TP;  // rotation pair of this thread
