@@ -58,6 +58,9 @@ struct JT<double> {
   static __device__ __forceinline__ double rsq(double x) { return rsqrt(x); }
   static __device__ __forceinline__ double sq(double x) { return sqrt(x); }
   static __device__ __forceinline__ double ab(double x) { return fabs(x); }
+  static __device__ __forceinline__ double dv(double a, double b) {
+    return a / b;
+  }
 };
 template <>
 struct JT<float> {
@@ -72,9 +75,21 @@ struct JT<float> {
   static __device__ __forceinline__ c_t mk(float re, float im) {
     return make_float2(re, im);
   }
-  static __device__ __forceinline__ float rsq(float x) { return rsqrtf(x); }
-  static __device__ __forceinline__ float sq(float x) { return sqrtf(x); }
+  // The FP32 sweeps only produce candidates that are re-orthogonalised and
+  // re-measured in FP64, so the rotation parameters use the one-instruction
+  // hardware forms (v_rsq_f32, v_sqrt_f32, v_rcp_f32, about 1 ulp) instead
+  // of the correctly rounded library sequences: the inner sweep is bound by
+  // its per-round instruction count.
+  static __device__ __forceinline__ float rsq(float x) {
+    return __builtin_amdgcn_rsqf(x);
+  }
+  static __device__ __forceinline__ float sq(float x) {
+    return __builtin_amdgcn_sqrtf(x);
+  }
   static __device__ __forceinline__ float ab(float x) { return fabsf(x); }
+  static __device__ __forceinline__ float dv(float a, float b) {
+    return a * __builtin_amdgcn_rcpf(b);
+  }
 };
 
 // Round-robin (circle method) position -> block for 2p blocks.
@@ -247,8 +262,9 @@ __device__ __forceinline__ int inner_solve(
         const R inv = JT<R>::rsq(absc2);
         const R absc = absc2 * inv;
         const R zeta = (R)0.5 * (d - a) * inv;
-        const R tt = (zeta >= (R)0 ? (R)1 : (R)-1) /
-                     (JT<R>::ab(zeta) + JT<R>::sq((R)1 + zeta * zeta));
+        const R tt = JT<R>::dv(zeta >= (R)0 ? (R)1 : (R)-1,
+                               JT<R>::ab(zeta) +
+                                   JT<R>::sq((R)1 + zeta * zeta));
         cs = JT<R>::rsq((R)1 + tt * tt);
         sn = tt * cs;
         er = g.x * inv;  // e = e^{-i phi} = conj(g) / |g|
