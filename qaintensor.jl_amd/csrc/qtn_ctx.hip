@@ -58,6 +58,17 @@ int32_t qtn_ctx::pinned_buffer(size_t bytes, void** out) {
   return QTN_OK;
 }
 
+int32_t qtn_ctx::worker(int i, qtn_ctx** out) {
+  while ((int)workers.size() <= i) {
+    qtn_ctx* w = nullptr;
+    QTN_TRY(qtn_ctx_create(device, &w));
+    w->parent = this;
+    workers.push_back(w);
+  }
+  *out = workers[i];
+  return QTN_OK;
+}
+
 extern "C" {
 
 const char* qtn_last_error(void) { return qtn::g_last_error.c_str(); }
@@ -94,6 +105,8 @@ int32_t qtn_ctx_create(int32_t device, qtn_ctx** ctx) {
 int32_t qtn_ctx_destroy(qtn_ctx* ctx) {
   qtn::clear_error();
   if (!ctx) return QTN_OK;
+  for (qtn_ctx* w : ctx->workers) QTN_TRY(qtn_ctx_destroy(w));
+  ctx->workers.clear();
   QTN_CHECK_HIP(hipSetDevice(ctx->device));
   QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   for (auto& b : ctx->bufs)

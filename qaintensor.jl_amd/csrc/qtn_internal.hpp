@@ -136,6 +136,12 @@ struct qtn_ctx {
   // Returns a device buffer of at least `bytes` for slot `slot`.
   int32_t buffer(int slot, size_t bytes, void** out);
   int32_t pinned_buffer(size_t bytes, void** out);
+  // Worker contexts (same device, own stream and scratch) that run
+  // independent batches concurrently with this one; created on first use,
+  // destroyed with it.  parent is set on workers.
+  qtn_ctx* parent = nullptr;
+  std::vector<qtn_ctx*> workers;
+  int32_t worker(int i, qtn_ctx** out);
 };
 
 // Device-resident MPS (qtn_mps.hip): right-canonical site tensors with the

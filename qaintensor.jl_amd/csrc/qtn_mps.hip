@@ -17,6 +17,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "qtn_internal.hpp"
@@ -952,7 +954,8 @@ int32_t apply_gates_core(qtn_ctx* ctx, int ng, qtn_mps* const* ms,
     const qtn_mps* m = ms[g];
     const int i = sites[g];
     QTN_ARG(m != nullptr, "apply_layer: NULL state");
-    QTN_ARG(m->ctx == ctx, "apply_layer: states on different contexts");
+    QTN_ARG(m->ctx == ctx || ctx->parent == m->ctx,
+            "apply_layer: states on different contexts");
     QTN_ARG(i >= 1 && i < m->n, "apply_2q: site out of range");
     QTN_ARG(chis[g] >= 1 && chis[g] <= m->cap,
             "apply_layer: chimax above capacity");
@@ -1829,8 +1832,17 @@ int32_t qtn_mps_apply_layer_multi(int32_t ngates, qtn_mps* const* states,
     const int64_t M = 2 * m->dims[i - 1], N = 2 * m->dims[i + 1];
     bucket[round_up64(std::min(M, N))].push_back(g);
   }
-  for (auto& kv : bucket) {
-    const std::vector<int>& idx = kv.second;
+  std::vector<const std::vector<int>*> groups;
+  for (auto& kv : bucket) groups.push_back(&kv.second);
+  // One batched pass per group.  The groups are independent (their gates
+  // act on disjoint sites and read no bond another group writes), so they
+  // run concurrently: group i on executor i % nexec, executor 0 being this
+  // context and thread, executors 1..3 worker contexts (own stream and
+  // scratch) driven by their own host threads; four streams match the
+  // device's hardware queues.  Profiling runs them one after another.
+  const int nexec =
+      ctx->prof ? 1 : std::min<int>(4, (int)groups.size());
+  auto run_group = [&](qtn_ctx* c, const std::vector<int>& idx) -> int32_t {
     const int nb = (int)idx.size();
     std::vector<qtn_mps*> ms(nb);
     std::vector<int32_t> st(nb);
@@ -1845,14 +1857,51 @@ int32_t qtn_mps_apply_layer_multi(int32_t ngates, qtn_mps* const* states,
                   32 * sizeof(double));
       ch[b] = chimax[g] > 0 ? chimax[g] : states[g]->cap;
     }
-    QTN_TRY(apply_gates_core(ctx, nb, ms.data(), st.data(), gs.data(),
+    QTN_TRY(apply_gates_core(c, nb, ms.data(), st.data(), gs.data(),
                              cutoff, ch.data(), form, kp.data(), dw.data(),
                              nullptr, nullptr));
     for (int b = 0; b < nb; ++b) {
       if (kept) kept[idx[b]] = kp[b];
       if (disc_weight) disc_weight[idx[b]] = dw[b];
     }
+    return QTN_OK;
+  };
+  if (nexec <= 1) {
+    for (const auto* gi : groups) QTN_TRY(run_group(ctx, *gi));
+    return QTN_OK;
   }
+  std::vector<qtn_ctx*> exec(nexec, ctx);
+  for (int e = 1; e < nexec; ++e) {
+    QTN_TRY(ctx->worker(e - 1, &exec[e]));
+    exec[e]->jac_tiling = ctx->jac_tiling;  // the context's tuned tilings
+  }
+  // the workers' streams start after everything queued on this one
+  QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<int32_t> status(nexec, QTN_OK);
+  std::vector<std::string> message(nexec);
+  auto drive = [&](int e) {
+    if (e > 0 && hipSetDevice(exec[e]->device) != hipSuccess) {
+      status[e] = QTN_ERR_HIP;
+      message[e] = "hipSetDevice failed in a worker thread";
+      return;
+    }
+    for (size_t i = e; i < groups.size(); i += nexec) {
+      status[e] = run_group(exec[e], *groups[i]);
+      if (status[e] != QTN_OK) {
+        message[e] = qtn_last_error();  // thread-local: carried back below
+        return;
+      }
+    }
+  };
+  std::vector<std::thread> threads;
+  for (int e = 1; e < nexec; ++e) threads.emplace_back(drive, e);
+  drive(0);
+  for (auto& t : threads) t.join();
+  for (int e = 0; e < nexec; ++e)
+    if (status[e] != QTN_OK) {
+      set_error(message[e]);
+      return status[e];
+    }
   return QTN_OK;
 }
 
