@@ -1403,6 +1403,102 @@ int32_t apply_gates_core(qtn_ctx* ctx, int ng, qtn_mps* const* ms,
   return QTN_OK;
 }
 
+// A layer's gates (on one or many states sharing ctx, disjoint per state)
+// in one batched pass per padded column count of X (64, 128, ...), so that
+// gates of similar size share a Jacobi batch without padding small
+// problems to the largest.  chimax[i] <= 0: the state's capacity.
+int32_t apply_gates_grouped(qtn_ctx* ctx, int32_t ngates,
+                            qtn_mps* const* states, const int32_t* sites,
+                            const double* gates, const int64_t* chimax,
+                            double cutoff, int32_t form, int64_t* kept,
+                            double* disc_weight) {
+  std::map<int64_t, std::vector<int>> bucket;
+  std::map<const qtn_mps*, std::vector<char>> used;
+  for (int g = 0; g < ngates; ++g) {
+    const qtn_mps* m = states[g];
+    QTN_ARG(m && m->ctx == ctx, "apply_layer: state NULL or on another context");
+    const int i = sites[g];
+    QTN_ARG(i >= 1 && i < m->n, "apply_2q: site out of range");
+    std::vector<char>& u = used[m];
+    if (u.empty()) u.assign(m->n + 1, 0);
+    QTN_ARG(!u[i] && !u[i + 1], "apply_layer: gates overlap");
+    u[i] = u[i + 1] = 1;
+    const int64_t M = 2 * m->dims[i - 1], N = 2 * m->dims[i + 1];
+    bucket[round_up64(std::min(M, N))].push_back(g);
+  }
+  std::vector<const std::vector<int>*> groups;
+  for (auto& kv : bucket) groups.push_back(&kv.second);
+  // One batched pass per group.  The groups are independent (their gates
+  // act on disjoint sites and read no bond another group writes), so they
+  // run concurrently: group i on executor i % nexec, executor 0 being this
+  // context and thread, executors 1..3 worker contexts (own stream and
+  // scratch) driven by their own host threads; four streams match the
+  // device's hardware queues.  Profiling runs them one after another.
+  const int nexec =
+      ctx->prof ? 1 : std::min<int>(4, (int)groups.size());
+  auto run_group = [&](qtn_ctx* c, const std::vector<int>& idx) -> int32_t {
+    const int nb = (int)idx.size();
+    std::vector<qtn_mps*> ms(nb);
+    std::vector<int32_t> st(nb);
+    std::vector<double> gs((size_t)nb * 32);
+    std::vector<int64_t> ch(nb), kp(nb);
+    std::vector<double> dw(nb);
+    for (int b = 0; b < nb; ++b) {
+      const int g = idx[b];
+      ms[b] = states[g];
+      st[b] = sites[g];
+      std::memcpy(&gs[(size_t)b * 32], gates + (size_t)g * 32,
+                  32 * sizeof(double));
+      ch[b] = chimax[g] > 0 ? chimax[g] : states[g]->cap;
+    }
+    QTN_TRY(apply_gates_core(c, nb, ms.data(), st.data(), gs.data(),
+                             cutoff, ch.data(), form, kp.data(), dw.data(),
+                             nullptr, nullptr));
+    for (int b = 0; b < nb; ++b) {
+      if (kept) kept[idx[b]] = kp[b];
+      if (disc_weight) disc_weight[idx[b]] = dw[b];
+    }
+    return QTN_OK;
+  };
+  if (nexec <= 1) {
+    for (const auto* gi : groups) QTN_TRY(run_group(ctx, *gi));
+    return QTN_OK;
+  }
+  std::vector<qtn_ctx*> exec(nexec, ctx);
+  for (int e = 1; e < nexec; ++e) {
+    QTN_TRY(ctx->worker(e - 1, &exec[e]));
+    exec[e]->jac_tiling = ctx->jac_tiling;  // the context's tuned tilings
+  }
+  // the workers' streams start after everything queued on this one
+  QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<int32_t> status(nexec, QTN_OK);
+  std::vector<std::string> message(nexec);
+  auto drive = [&](int e) {
+    if (e > 0 && hipSetDevice(exec[e]->device) != hipSuccess) {
+      status[e] = QTN_ERR_HIP;
+      message[e] = "hipSetDevice failed in a worker thread";
+      return;
+    }
+    for (size_t i = e; i < groups.size(); i += nexec) {
+      status[e] = run_group(exec[e], *groups[i]);
+      if (status[e] != QTN_OK) {
+        message[e] = qtn_last_error();  // thread-local: carried back below
+        return;
+      }
+    }
+  };
+  std::vector<std::thread> threads;
+  for (int e = 1; e < nexec; ++e) threads.emplace_back(drive, e);
+  drive(0);
+  for (auto& t : threads) t.join();
+  for (int e = 0; e < nexec; ++e)
+    if (status[e] != QTN_OK) {
+      set_error(message[e]);
+      return status[e];
+    }
+  return QTN_OK;
+}
+
 // All gates on one state with one cap.
 int32_t mps_apply_gates(qtn_mps* m, int ng, const int32_t* sites,
                         const double* gates, double cutoff, int64_t chimax,
@@ -1629,8 +1725,10 @@ int32_t qtn_mps_apply_layer(qtn_mps* m, int32_t ngates, const int32_t* sites,
   if (ngates == 0) return QTN_OK;
   QTN_CHECK_HIP(hipSetDevice(m->ctx->device));
   const int64_t cm = (chimax > 0) ? chimax : m->cap;
-  return mps_apply_gates(m, ngates, sites, gates, cutoff, cm, form, kept,
-                         disc_weight, nullptr, nullptr);
+  std::vector<qtn_mps*> ms(ngates, m);
+  std::vector<int64_t> chis(ngates, cm);
+  return apply_gates_grouped(m->ctx, ngates, ms.data(), sites, gates,
+                             chis.data(), cutoff, form, kept, disc_weight);
 }
 
 int32_t qtn_mps_amplitudes(const qtn_mps* m, int64_t nb, const uint8_t* bits,
@@ -1815,94 +1913,8 @@ int32_t qtn_mps_apply_layer_multi(int32_t ngates, qtn_mps* const* states,
   qtn_ctx* ctx = states[0] ? states[0]->ctx : nullptr;
   QTN_ARG(ctx != nullptr, "qtn_mps_apply_layer_multi: NULL state");
   QTN_CHECK_HIP(hipSetDevice(ctx->device));
-  // one batched pass per padded column count of X (64, 128, ...): gates of
-  // similar size share a Jacobi batch without padding small problems to the
-  // largest
-  std::map<int64_t, std::vector<int>> bucket;
-  std::map<const qtn_mps*, std::vector<char>> used;
-  for (int g = 0; g < ngates; ++g) {
-    const qtn_mps* m = states[g];
-    QTN_ARG(m && m->ctx == ctx, "qtn_mps_apply_layer_multi: bad state");
-    const int i = sites[g];
-    QTN_ARG(i >= 1 && i < m->n, "apply_2q: site out of range");
-    std::vector<char>& u = used[m];
-    if (u.empty()) u.assign(m->n + 1, 0);
-    QTN_ARG(!u[i] && !u[i + 1], "apply_layer: gates overlap");
-    u[i] = u[i + 1] = 1;
-    const int64_t M = 2 * m->dims[i - 1], N = 2 * m->dims[i + 1];
-    bucket[round_up64(std::min(M, N))].push_back(g);
-  }
-  std::vector<const std::vector<int>*> groups;
-  for (auto& kv : bucket) groups.push_back(&kv.second);
-  // One batched pass per group.  The groups are independent (their gates
-  // act on disjoint sites and read no bond another group writes), so they
-  // run concurrently: group i on executor i % nexec, executor 0 being this
-  // context and thread, executors 1..3 worker contexts (own stream and
-  // scratch) driven by their own host threads; four streams match the
-  // device's hardware queues.  Profiling runs them one after another.
-  const int nexec =
-      ctx->prof ? 1 : std::min<int>(4, (int)groups.size());
-  auto run_group = [&](qtn_ctx* c, const std::vector<int>& idx) -> int32_t {
-    const int nb = (int)idx.size();
-    std::vector<qtn_mps*> ms(nb);
-    std::vector<int32_t> st(nb);
-    std::vector<double> gs((size_t)nb * 32);
-    std::vector<int64_t> ch(nb), kp(nb);
-    std::vector<double> dw(nb);
-    for (int b = 0; b < nb; ++b) {
-      const int g = idx[b];
-      ms[b] = states[g];
-      st[b] = sites[g];
-      std::memcpy(&gs[(size_t)b * 32], gates + (size_t)g * 32,
-                  32 * sizeof(double));
-      ch[b] = chimax[g] > 0 ? chimax[g] : states[g]->cap;
-    }
-    QTN_TRY(apply_gates_core(c, nb, ms.data(), st.data(), gs.data(),
-                             cutoff, ch.data(), form, kp.data(), dw.data(),
-                             nullptr, nullptr));
-    for (int b = 0; b < nb; ++b) {
-      if (kept) kept[idx[b]] = kp[b];
-      if (disc_weight) disc_weight[idx[b]] = dw[b];
-    }
-    return QTN_OK;
-  };
-  if (nexec <= 1) {
-    for (const auto* gi : groups) QTN_TRY(run_group(ctx, *gi));
-    return QTN_OK;
-  }
-  std::vector<qtn_ctx*> exec(nexec, ctx);
-  for (int e = 1; e < nexec; ++e) {
-    QTN_TRY(ctx->worker(e - 1, &exec[e]));
-    exec[e]->jac_tiling = ctx->jac_tiling;  // the context's tuned tilings
-  }
-  // the workers' streams start after everything queued on this one
-  QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-  std::vector<int32_t> status(nexec, QTN_OK);
-  std::vector<std::string> message(nexec);
-  auto drive = [&](int e) {
-    if (e > 0 && hipSetDevice(exec[e]->device) != hipSuccess) {
-      status[e] = QTN_ERR_HIP;
-      message[e] = "hipSetDevice failed in a worker thread";
-      return;
-    }
-    for (size_t i = e; i < groups.size(); i += nexec) {
-      status[e] = run_group(exec[e], *groups[i]);
-      if (status[e] != QTN_OK) {
-        message[e] = qtn_last_error();  // thread-local: carried back below
-        return;
-      }
-    }
-  };
-  std::vector<std::thread> threads;
-  for (int e = 1; e < nexec; ++e) threads.emplace_back(drive, e);
-  drive(0);
-  for (auto& t : threads) t.join();
-  for (int e = 0; e < nexec; ++e)
-    if (status[e] != QTN_OK) {
-      set_error(message[e]);
-      return status[e];
-    }
-  return QTN_OK;
+  return apply_gates_grouped(ctx, ngates, states, sites, gates, chimax,
+                             cutoff, form, kept, disc_weight);
 }
 
 }  // extern "C"
