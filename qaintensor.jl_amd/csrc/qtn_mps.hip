@@ -1426,16 +1426,58 @@ int32_t apply_gates_grouped(qtn_ctx* ctx, int32_t ngates,
     const int64_t M = 2 * m->dims[i - 1], N = 2 * m->dims[i + 1];
     bucket[round_up64(std::min(M, N))].push_back(g);
   }
-  std::vector<const std::vector<int>*> groups;
-  for (auto& kv : bucket) groups.push_back(&kv.second);
+  // Groups and their cost (gates x columns^2, the Jacobi work); the
+  // costliest is halved while it exceeds a quarter of the total and keeps at
+  // least kMinChunk gates per half, so one large size class does not set
+  // the critical path alone.
+  constexpr int kExec = 4, kMinChunk = 4;
+  struct Group {
+    std::vector<int> idx;
+    double cost;
+  };
+  std::vector<Group> groups;
+  double total = 0.0;
+  for (auto& kv : bucket) {
+    const double c = (double)kv.second.size() * (double)kv.first *
+                     (double)kv.first;
+    groups.push_back({kv.second, c});
+    total += c;
+  }
+  for (;;) {
+    int big = 0;
+    for (int i = 1; i < (int)groups.size(); ++i)
+      if (groups[i].cost > groups[big].cost) big = i;
+    Group& g = groups[big];
+    if (g.cost * kExec <= total || (int)g.idx.size() < 2 * kMinChunk) break;
+    const size_t half = g.idx.size() / 2;
+    Group h{std::vector<int>(g.idx.begin() + half, g.idx.end()),
+            g.cost * (double)(g.idx.size() - half) / (double)g.idx.size()};
+    g.cost -= h.cost;
+    g.idx.resize(half);
+    groups.push_back(std::move(h));
+  }
   // One batched pass per group.  The groups are independent (their gates
   // act on disjoint sites and read no bond another group writes), so they
-  // run concurrently: group i on executor i % nexec, executor 0 being this
-  // context and thread, executors 1..3 worker contexts (own stream and
-  // scratch) driven by their own host threads; four streams match the
-  // device's hardware queues.  Profiling runs them one after another.
-  const int nexec =
-      ctx->prof ? 1 : std::min<int>(4, (int)groups.size());
+  // run concurrently on up to four executors (four streams match the
+  // device's hardware queues): executor 0 is this context and thread,
+  // executors 1..3 worker contexts (own stream and scratch) driven by their
+  // own host threads.  Groups go costliest first to the least loaded
+  // executor.  Profiling runs them one after another.
+  const int nexec = ctx->prof ? 1 : std::min<int>(kExec, (int)groups.size());
+  std::vector<int> order(groups.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::sort(order.begin(), order.end(), [&](int a, int b) {
+    return groups[a].cost > groups[b].cost;
+  });
+  std::vector<std::vector<int>> plan(nexec);
+  std::vector<double> load(nexec, 0.0);
+  for (int gi : order) {
+    int e = 0;
+    for (int k = 1; k < nexec; ++k)
+      if (load[k] < load[e]) e = k;
+    plan[e].push_back(gi);
+    load[e] += groups[gi].cost;
+  }
   auto run_group = [&](qtn_ctx* c, const std::vector<int>& idx) -> int32_t {
     const int nb = (int)idx.size();
     std::vector<qtn_mps*> ms(nb);
@@ -1461,7 +1503,7 @@ int32_t apply_gates_grouped(qtn_ctx* ctx, int32_t ngates,
     return QTN_OK;
   };
   if (nexec <= 1) {
-    for (const auto* gi : groups) QTN_TRY(run_group(ctx, *gi));
+    for (const auto& g : groups) QTN_TRY(run_group(ctx, g.idx));
     return QTN_OK;
   }
   std::vector<qtn_ctx*> exec(nexec, ctx);
@@ -1479,8 +1521,8 @@ int32_t apply_gates_grouped(qtn_ctx* ctx, int32_t ngates,
       message[e] = "hipSetDevice failed in a worker thread";
       return;
     }
-    for (size_t i = e; i < groups.size(); i += nexec) {
-      status[e] = run_group(exec[e], *groups[i]);
+    for (int gi : plan[e]) {
+      status[e] = run_group(exec[e], groups[gi].idx);
       if (status[e] != QTN_OK) {
         message[e] = qtn_last_error();  // thread-local: carried back below
         return;
