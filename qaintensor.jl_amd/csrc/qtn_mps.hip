@@ -1407,11 +1407,15 @@ int32_t apply_gates_core(qtn_ctx* ctx, int ng, qtn_mps* const* ms,
 // in one batched pass per padded column count of X (64, 128, ...), so that
 // gates of similar size share a Jacobi batch without padding small
 // problems to the largest.  chimax[i] <= 0: the state's capacity.
+// hint / accepted (learned form, may be NULL): per-gate rank hints and the
+// range finder's outcome, as in apply_gates_core.
 int32_t apply_gates_grouped(qtn_ctx* ctx, int32_t ngates,
                             qtn_mps* const* states, const int32_t* sites,
                             const double* gates, const int64_t* chimax,
                             double cutoff, int32_t form, int64_t* kept,
-                            double* disc_weight) {
+                            double* disc_weight,
+                            const int64_t* hint = nullptr,
+                            int* accepted = nullptr) {
   std::map<int64_t, std::vector<int>> bucket;
   std::map<const qtn_mps*, std::vector<char>> used;
   for (int g = 0; g < ngates; ++g) {
@@ -1483,8 +1487,9 @@ int32_t apply_gates_grouped(qtn_ctx* ctx, int32_t ngates,
     std::vector<qtn_mps*> ms(nb);
     std::vector<int32_t> st(nb);
     std::vector<double> gs((size_t)nb * 32);
-    std::vector<int64_t> ch(nb), kp(nb);
+    std::vector<int64_t> ch(nb), kp(nb), hn(hint ? nb : 0);
     std::vector<double> dw(nb);
+    std::vector<int> ac(nb, 0);
     for (int b = 0; b < nb; ++b) {
       const int g = idx[b];
       ms[b] = states[g];
@@ -1492,13 +1497,16 @@ int32_t apply_gates_grouped(qtn_ctx* ctx, int32_t ngates,
       std::memcpy(&gs[(size_t)b * 32], gates + (size_t)g * 32,
                   32 * sizeof(double));
       ch[b] = chimax[g] > 0 ? chimax[g] : states[g]->cap;
+      if (hint) hn[b] = hint[g];
     }
     QTN_TRY(apply_gates_core(c, nb, ms.data(), st.data(), gs.data(),
                              cutoff, ch.data(), form, kp.data(), dw.data(),
-                             nullptr, nullptr));
+                             nullptr, nullptr, hint ? hn.data() : nullptr,
+                             ac.data()));
     for (int b = 0; b < nb; ++b) {
       if (kept) kept[idx[b]] = kp[b];
       if (disc_weight) disc_weight[idx[b]] = dw[b];
+      if (accepted) accepted[idx[b]] = ac[b];
     }
     return QTN_OK;
   };
@@ -2104,9 +2112,9 @@ int32_t qtn_mps_apply_layer_learned(qtn_mps* m, int32_t ngates,
   std::vector<qtn_mps*> ms(ngates, m);
   std::vector<int64_t> chis(ngates, cm);
   std::vector<int> acc(ngates, 0);
-  QTN_TRY(apply_gates_core(m->ctx, ngates, ms.data(), sites, gates, cutoff,
-                           chis.data(), QTN_FORM_LEARNED, kept, disc_weight,
-                           nullptr, nullptr, rank_hint, acc.data()));
+  QTN_TRY(apply_gates_grouped(m->ctx, ngates, ms.data(), sites, gates,
+                              chis.data(), cutoff, QTN_FORM_LEARNED, kept,
+                              disc_weight, rank_hint, acc.data()));
   if (accepted)
     for (int g = 0; g < ngates; ++g) accepted[g] = acc[g];
   return QTN_OK;
