@@ -1,4 +1,5 @@
 // Context lifetime, error text and device scratch management for libqtn.
+#include <algorithm>
 #include <cstring>
 #include <new>
 
@@ -63,6 +64,7 @@ int32_t qtn_ctx::worker(int i, qtn_ctx** out) {
     qtn_ctx* w = nullptr;
     QTN_TRY(qtn_ctx_create(device, &w));
     w->parent = this;
+    w->prof = prof;
     workers.push_back(w);
   }
   *out = workers[i];
@@ -131,10 +133,21 @@ int32_t qtn_ctx_counters(qtn_ctx* ctx, int64_t* out, int32_t n,
   QTN_ARG(ctx != nullptr && (n == 0 || out != nullptr),
           "qtn_ctx_counters: NULL argument");
   QTN_ARG(n >= 0 && n <= 4, "qtn_ctx_counters: n must be 0..4");
-  const int64_t v[4] = {ctx->n_svd, ctx->n_sweeps, ctx->max_sweeps,
-                        ctx->n_batches};
+  // the context's own work plus its workers' (concurrent layer groups)
+  int64_t v[4] = {ctx->n_svd, ctx->n_sweeps, ctx->max_sweeps,
+                  ctx->n_batches};
+  for (const qtn_ctx* w : ctx->workers) {
+    v[0] += w->n_svd;
+    v[1] += w->n_sweeps;
+    v[2] = std::max(v[2], w->max_sweeps);
+    v[3] += w->n_batches;
+  }
   for (int i = 0; i < n; ++i) out[i] = v[i];
-  if (reset) ctx->n_svd = ctx->n_sweeps = ctx->max_sweeps = ctx->n_batches = 0;
+  if (reset) {
+    ctx->n_svd = ctx->n_sweeps = ctx->max_sweeps = ctx->n_batches = 0;
+    for (qtn_ctx* w : ctx->workers)
+      w->n_svd = w->n_sweeps = w->max_sweeps = w->n_batches = 0;
+  }
   return QTN_OK;
 }
 
@@ -142,10 +155,14 @@ int32_t qtn_ctx_profile(qtn_ctx* ctx, int32_t enable) {
   qtn::clear_error();
   QTN_ARG(ctx != nullptr, "qtn_ctx_profile: ctx is NULL");
   QTN_CHECK_HIP(hipSetDevice(ctx->device));
-  ctx->prof = enable != 0;
-  for (int k = 0; k < qtn_ctx::kProfKernels; ++k) {
-    ctx->prof_count[k] = 0;
-    ctx->prof_ms[k] = ctx->prof_bytes[k] = ctx->prof_flops[k] = 0.0;
+  std::vector<qtn_ctx*> all(1, ctx);
+  all.insert(all.end(), ctx->workers.begin(), ctx->workers.end());
+  for (qtn_ctx* c : all) {
+    c->prof = enable != 0;
+    for (int k = 0; k < qtn_ctx::kProfKernels; ++k) {
+      c->prof_count[k] = 0;
+      c->prof_ms[k] = c->prof_bytes[k] = c->prof_flops[k] = 0.0;
+    }
   }
   return QTN_OK;
 }
@@ -156,10 +173,20 @@ int32_t qtn_ctx_profile_read(qtn_ctx* ctx, int32_t kernel, int64_t* count,
   QTN_ARG(ctx != nullptr, "qtn_ctx_profile_read: ctx is NULL");
   QTN_ARG(kernel >= 0 && kernel < qtn_ctx::kProfKernels,
           "qtn_ctx_profile_read: unknown kernel id");
-  if (count) *count = ctx->prof_count[kernel];
-  if (ms) *ms = ctx->prof_ms[kernel];
-  if (bytes) *bytes = ctx->prof_bytes[kernel];
-  if (flops) *flops = ctx->prof_flops[kernel];
+  // the context's own launches plus its workers'
+  int64_t c = ctx->prof_count[kernel];
+  double m = ctx->prof_ms[kernel], b = ctx->prof_bytes[kernel],
+         f = ctx->prof_flops[kernel];
+  for (const qtn_ctx* w : ctx->workers) {
+    c += w->prof_count[kernel];
+    m += w->prof_ms[kernel];
+    b += w->prof_bytes[kernel];
+    f += w->prof_flops[kernel];
+  }
+  if (count) *count = c;
+  if (ms) *ms = m;
+  if (bytes) *bytes = b;
+  if (flops) *flops = f;
   return QTN_OK;
 }
 

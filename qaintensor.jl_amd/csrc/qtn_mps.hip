@@ -1466,8 +1466,8 @@ int32_t apply_gates_grouped(qtn_ctx* ctx, int32_t ngates,
   // device's hardware queues): executor 0 is this context and thread,
   // executors 1..3 worker contexts (own stream and scratch) driven by their
   // own host threads.  Groups go costliest first to the least loaded
-  // executor.  Profiling runs them one after another.
-  const int nexec = ctx->prof ? 1 : std::min<int>(kExec, (int)groups.size());
+  // executor.  Per-kernel profiling, when on, covers the workers too.
+  const int nexec = std::min<int>(kExec, (int)groups.size());
   std::vector<int> order(groups.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
   std::sort(order.begin(), order.end(), [&](int a, int b) {
