@@ -156,8 +156,9 @@ def bench_tfim(args):
     set_sites (canonical form on the GPU), <psi|H|psi> by the environment
     sweep in complex FP64 and with the bond-sized contractions as BF16x3
     split-precision GEMMs (the chosen form), and W|psi> by zip-up +
-    recompression to chi 1024 (cutoff 1e-10), whose <psi|W psi> is compared
-    with the exact value.  The CPU leg is the
+    recompression to chi 1024 (cutoff 1e-10) in FP64 and in the chosen form
+    (FP32 sweeps on the cap-decided zip-up splits), whose <psi|W psi> is
+    compared with the exact value.  The CPU leg is the
     oracle's environment sweep on the same (left-canonicalised) state."""
     import torch
     from qaintensor import DeviceMPO, DeviceMPS, FORM_CHOSEN, default_context
@@ -197,6 +198,15 @@ def bench_tfim(args):
     t_zip = time.perf_counter() - t0
     e_zip = nrm * out.overlap(psi).conjugate()  # <psi|W psi>
     print("apply_mpo %.2f s" % t_zip, file=sys.stderr, flush=True)
+    out_c = DeviceMPS(n, chi, ctx)
+    t0 = time.perf_counter()
+    nrm_c = psi.apply_mpo(mpo, cutoff, chi, out_c, form=FORM_CHOSEN)
+    torch.cuda.synchronize()
+    t_zip_c = time.perf_counter() - t0
+    e_zip_c = nrm_c * out_c.overlap(psi).conjugate()
+    infid_c = 1.0 - abs(out_c.overlap(out))
+    print("apply_mpo chosen %.2f s (infidelity vs FP64 %.2e)"
+          % (t_zip_c, infid_c), file=sys.stderr, flush=True)
     line = {
         "metric": "<psi|H|psi> relative error, 64-site TFIM MPO x MPS, "
                   "chi 1024",
@@ -211,10 +221,15 @@ def bench_tfim(args):
         "expect_bf16x3": [e_bf.real, e_bf.imag],
         "bf16x3_rel_error_vs_fp64": abs(e_bf - e_gpu) / abs(e_gpu),
         "expect_zipup": [e_zip.real, e_zip.imag],
+        "expect_zipup_chosen": [e_zip_c.real, e_zip_c.imag],
+        "zipup_chosen_infidelity_vs_fp64": infid_c,
+        "zipup_chosen_norm_rel_diff": abs(nrm_c - nrm) / nrm,
         "seconds": {"set_sites": t_set, "expect_mpo": t_exp,
                     "expect_mpo_bf16x3": t_bf,
-                    "apply_mpo_zipup": t_zip},
+                    "apply_mpo_zipup": t_zip,
+                    "apply_mpo_zipup_chosen": t_zip_c},
         "out_bond_max": int(max(out.bond_dims())),
+        "out_bond_max_chosen": int(max(out_c.bond_dims())),
     }
     if not args.no_cpu:
         from oracle import qtn_oracle as O
@@ -226,6 +241,7 @@ def bench_tfim(args):
         line["value"] = abs(e_gpu - e_ref) / abs(e_ref)
         line["bf16x3_rel_error"] = abs(e_bf - e_ref) / abs(e_ref)
         line["zipup_rel_error"] = abs(e_zip - e_ref) / abs(e_ref)
+        line["zipup_chosen_rel_error"] = abs(e_zip_c - e_ref) / abs(e_ref)
         line["cpu_baseline"] = {
             "value": t_cpu, "unit": "s per <psi|H|psi>",
             "cores": int(os.environ.get("OPENBLAS_NUM_THREADS",

@@ -65,6 +65,8 @@ typedef struct qtn_mpo qtn_mpo;
 /* ---- context ----------------------------------------------------------- */
 /* Opens device `device`, creates one HIP stream and a workspace pool.      */
 int32_t qtn_ctx_create(int32_t device, qtn_ctx** ctx);
+/* Destroy every qtn_mps, qtn_mpo and qtn_mlp made on ctx first:
+ * their destroy calls use the context.                                    */
 int32_t qtn_ctx_destroy(qtn_ctx* ctx);
 /* Blocks until all work queued on the context's stream is done.            */
 int32_t qtn_ctx_sync(qtn_ctx* ctx);
@@ -307,6 +309,16 @@ int32_t qtn_zgemm_bf16x3(qtn_ctx* ctx, char opA, int64_t m, int64_t n,
 int32_t qtn_mps_apply_mpo(const qtn_mps* in, const qtn_mpo* mpo,
                           double cutoff, int64_t chimax, qtn_mps* out,
                           double* norm);
+/* qtn_mps_apply_mpo in a chosen form.  QTN_FORM_CHOSEN: each zip-up split
+ * whose rank the cap can decide (min(rows, cols) > its cap) runs its Jacobi
+ * sweeps in complex FP32; when FP32 keeps exactly the cap, the kept left
+ * vectors are re-orthonormalised in FP64 (three Newton-Schulz steps) and
+ * S V^H is recomputed as U^H A in FP64, else the split is redone in FP64.
+ * The right-to-left recompression stays FP64.  No counterpart in the
+ * reference (it has the plain form only).                                 */
+int32_t qtn_mps_apply_mpo_form(const qtn_mps* in, const qtn_mpo* mpo,
+                               double cutoff, int64_t chimax, int32_t form,
+                               qtn_mps* out, double* norm);
 
 /* ---- learned rank predictor (no counterpart in the reference) ---------- */
 /* A 3-layer FP32 MLP, y = W3 gelu(W2 gelu(W1 a + b1) + b2) + b3, whose

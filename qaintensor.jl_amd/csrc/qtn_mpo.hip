@@ -386,8 +386,17 @@ int32_t qtn_mps_set_sites(qtn_mps* m, const int64_t* dims,
 
 int32_t qtn_mps_apply_mpo(const qtn_mps* in, const qtn_mpo* w, double cutoff,
                           int64_t chimax, qtn_mps* out, double* norm) {
+  return qtn_mps_apply_mpo_form(in, w, cutoff, chimax, QTN_FORM_PLAIN, out,
+                                norm);
+}
+
+int32_t qtn_mps_apply_mpo_form(const qtn_mps* in, const qtn_mpo* w,
+                               double cutoff, int64_t chimax, int32_t form,
+                               qtn_mps* out, double* norm) {
   clear_error();
   QTN_ARG(in && w && out, "qtn_mps_apply_mpo: NULL argument");
+  QTN_ARG(form == QTN_FORM_PLAIN || form == QTN_FORM_CHOSEN,
+          "qtn_mps_apply_mpo: form must be QTN_FORM_PLAIN or QTN_FORM_CHOSEN");
   QTN_ARG(in->n == w->n && out->n == in->n,
           "qtn_mps_apply_mpo: lengths differ");
   QTN_ARG(in->ctx == w->ctx && out->ctx == in->ctx,
@@ -466,9 +475,14 @@ int32_t qtn_mps_apply_mpo(const qtn_mps* in, const qtn_mpo* w, double cutoff,
     }
     int64_t kn = 0;
     const int64_t kcap = std::min(cm, cap[j + 1]);
-    QTN_TRY(svd_device(ctx, T2, 2 * k, cr * Dr, 2 * k, true, cutoff, 1, kcap,
-                       1, (double2*)dU, (double*)dS, (double2*)dVh, &kn,
-                       nullptr));
+    if (form == QTN_FORM_CHOSEN)
+      QTN_TRY(svd_device_chosen(ctx, T2, 2 * k, cr * Dr, 2 * k, cutoff, 1,
+                                kcap, 1, (double2*)dU, (double*)dS,
+                                (double2*)dVh, &kn, nullptr));
+    else
+      QTN_TRY(svd_device(ctx, T2, 2 * k, cr * Dr, 2 * k, true, cutoff, 1,
+                         kcap, 1, (double2*)dU, (double*)dS, (double2*)dVh,
+                         &kn, nullptr));
     QTN_CHECK_HIP(hipMemcpyAsync(work[j], dU,
                                  (size_t)2 * k * kn * sizeof(double2),
                                  hipMemcpyDeviceToDevice, s));

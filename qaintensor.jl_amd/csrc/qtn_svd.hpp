@@ -117,5 +117,11 @@ int32_t svd_device(qtn_ctx* ctx, const double2* dA, int64_t m, int64_t n,
                    int64_t lda, bool truncate, double cutoff, int relative,
                    int64_t chimax, int absorb, double2* dU, double* dS,
                    double2* dVh, int64_t* k_out, double* disc_out);
+// The chosen form (FP32 sweeps where the cap decides the truncation, FP64
+// re-orthonormalisation and recomputation); contract in qtn_svd_api.hip.
+int32_t svd_device_chosen(qtn_ctx* ctx, const double2* dA, int64_t m,
+                          int64_t n, int64_t lda, double cutoff, int relative,
+                          int64_t chimax, int absorb, double2* dU, double* dS,
+                          double2* dVh, int64_t* k_out, double* disc_out);
 
 }  // namespace qtn

@@ -7,6 +7,7 @@ QtnLibraryError.
 import ctypes
 import os
 import threading
+import weakref
 
 import numpy as np
 
@@ -43,6 +44,7 @@ EXPORTS = [
     "qtn_mps_amplitudes", "qtn_mps_expect_z", "qtn_mps_overlap",
     "qtn_mps_norm2", "qtn_mpo_create", "qtn_mpo_destroy",
     "qtn_mps_expect_mpo", "qtn_mps_apply_mpo", "qtn_mps_expect_mpo_form",
+    "qtn_mps_apply_mpo_form",
     "qtn_zgemm_bf16x3",
     "qtn_mlp_create", "qtn_mlp_destroy", "qtn_mlp_get_params",
     "qtn_mlp_forward", "qtn_mlp_loss_grad", "qtn_mlp_train",
@@ -148,6 +150,8 @@ def _declare(lib):
         "qtn_zgemm_bf16x3": (_i32, [_vp, ctypes.c_char, _i64, _i64, _i64, _pd,
                                     _i64, _pd, _i64, _pd, _i64]),
         "qtn_mps_apply_mpo": (_i32, [_vp, _vp, _dbl, _i64, _vp, _pd]),
+        "qtn_mps_apply_mpo_form": (_i32, [_vp, _vp, _dbl, _i64, _i32, _vp,
+                                          _pd]),
         "qtn_mlp_create": (_i32, [_vp, _i32, _i32, _pf, ctypes.POINTER(_vp)]),
         "qtn_mlp_destroy": (_i32, [_vp]),
         "qtn_mlp_get_params": (_i32, [_vp, _pf]),
@@ -212,7 +216,10 @@ def as_c128(a, order="F"):
 
 
 class Context:
-    """Owns one qtn_ctx (device, stream, scratch)."""
+    """Owns one qtn_ctx (device, stream, scratch).  The states, operators
+    and networks made on it register themselves, and close() frees those
+    still alive before the context: the library's objects use their context
+    when they are destroyed."""
 
     def __init__(self, device=0):
         lib = load()
@@ -220,6 +227,11 @@ class Context:
         check(lib.qtn_ctx_create(int(device), ctypes.byref(h)))
         self._h = h
         self.device = device
+        self._children = weakref.WeakSet()
+
+    def adopt(self, obj):
+        """Register obj (with a free() method) to be freed by close()."""
+        self._children.add(obj)
 
     @property
     def handle(self):
@@ -267,6 +279,8 @@ class Context:
 
     def close(self):
         if self._h is not None:
+            for child in list(self._children):
+                child.free()
             check(load().qtn_ctx_destroy(self._h))
             self._h = None
 

@@ -27,6 +27,7 @@ class DeviceMPS:
         L.check(L.load().qtn_mps_create(self.ctx.handle, self.n, self.cap,
                                         ctypes.byref(h)))
         self._h = h
+        self.ctx.adopt(self)
 
     @property
     def handle(self):
@@ -201,13 +202,22 @@ def _expect_mpo(self, mpo, form=FORM_PLAIN):
     return complex(out[0], out[1])
 
 
-def _apply_mpo(self, mpo, cutoff, chimax, out):
+def _apply_mpo(self, mpo, cutoff, chimax, out, form=0):
     """out <- W|self> by zip-up and recompression; returns |W psi| of the
-    compressed state (out is normalised)."""
+    compressed state (out is normalised).  form FORM_CHOSEN runs the
+    cap-decided zip-up splits in FP32 with FP64 re-orthonormalisation
+    (qtn_mps_apply_mpo_form)."""
     nrm = ctypes.c_double(0.0)
-    L.check(L.load().qtn_mps_apply_mpo(self.handle, mpo.handle,
-                                       float(cutoff), int(chimax),
-                                       out.handle, ctypes.byref(nrm)))
+    lib = L.load()
+    if form:
+        L.check(lib.qtn_mps_apply_mpo_form(self.handle, mpo.handle,
+                                           float(cutoff), int(chimax),
+                                           int(form), out.handle,
+                                           ctypes.byref(nrm)))
+    else:
+        L.check(lib.qtn_mps_apply_mpo(self.handle, mpo.handle,
+                                      float(cutoff), int(chimax),
+                                      out.handle, ctypes.byref(nrm)))
     return nrm.value
 
 
@@ -237,6 +247,7 @@ class DeviceMPO:
         L.check(L.load().qtn_mpo_create(self.ctx.handle, self.n,
                                         L.i64ptr(D), ptrs, ctypes.byref(h)))
         self._h = h
+        self.ctx.adopt(self)
 
     @property
     def handle(self):

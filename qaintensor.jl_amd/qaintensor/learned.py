@@ -55,6 +55,7 @@ class RankMLP:
         L.check(L.load().qtn_mlp_create(self.ctx.handle, nin, width, _fp(p),
                                         ctypes.byref(h)))
         self._h = h
+        self.ctx.adopt(self)
 
     @property
     def handle(self):
