@@ -165,6 +165,9 @@ def bench_tfim(args):
     from qaintensor import hamiltonians as Hm
     n, chi, J, seed, cutoff = 64, 1024, 1.0, 4, 1e-10
     ctx = default_context()
+    # zip-up splits (3 chi x 2 chi as X = A^H) and recompression splits
+    # (2 chi x chi), one problem at a time
+    tuned = autotune(ctx, args, ((3 * chi, 2 * chi), (2 * chi, chi)), (1,))
     h = Hm.tfim_fields(n, seed)
     W = Hm.tfim_mpo(n, J, h)
     raw = Hm.random_mps_tensors(n, chi, seed)
@@ -230,6 +233,7 @@ def bench_tfim(args):
                     "apply_mpo_zipup_chosen": t_zip_c},
         "out_bond_max": int(max(out.bond_dims())),
         "out_bond_max_chosen": int(max(out_c.bond_dims())),
+        "autotune": tuned,
     }
     if not args.no_cpu:
         from oracle import qtn_oracle as O

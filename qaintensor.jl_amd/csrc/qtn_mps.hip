@@ -1122,8 +1122,9 @@ int32_t apply_gates_core(qtn_ctx* ctx, int ng, qtn_mps* const* ms,
   };
   // The learned form is the chosen form plus the range finder: gates whose
   // predicted rank stays below the cap try the sketch (sketch width
-  // hint + 16, well below the full rank), the other capped gates take the
-  // FP32 route, and the rest (and every rejected sketch) the FP64 engine.
+  // hint + 16, well below the full rank), the other capped gates (and
+  // rejected sketches where the cap binds) take the FP32 route, and the
+  // rest the FP64 engine.
   std::vector<int> attempt(ng, 0);
   std::vector<int64_t> lw(ng, 0);
   bool any_attempt = false;
@@ -1137,21 +1138,43 @@ int32_t apply_gates_core(qtn_ctx* ctx, int ng, qtn_mps* const* ms,
       }
     }
   }
+  // The sketches run first, so that a gate whose sketch is rejected can
+  // still take the FP32 route below.
+  std::vector<int> acc(ng, 0);
+  if (any_attempt) {
+    // each call draws a fresh sketch from the context's generator
+    ctx->sketch_seed =
+        ctx->sketch_seed * 6364136223846793005ULL + 1442695040888963407ULL;
+    const uint64_t seed = ctx->sketch_seed;
+    QTN_TRY(rand_range_finder(ctx, ng, gd, d_gd, rows_p, attempt, lw, chis,
+                              cutoff, seed, acc, kept, keptw, disc, total));
+    for (int g = 0; g < ng; ++g)
+      if (acc[g]) {
+        src[g] = 2;
+        need64[g] = 0;
+      }
+  }
   void *dX32 = nullptr, *dW32 = nullptr;
   if (chosen || learned) {
     // The FP32 route pays only where the cap binds.  Without a prediction
     // that is taken to be where the bond sat at the cap before the gate or
     // theta is at least twice the cap (a cutoff-bound spectrum below the
     // cap would send the FP32 result back to the FP64 engine); a predicted
-    // rank at the cap decides it in the learned form.
+    // rank at the cap decides it in the learned form, and a rejected sketch
+    // (the prediction was low) falls back to the structural test.
     std::vector<int> cand(ng, 0);
     bool any32 = false;
     for (int g = 0; g < ng; ++g) {
-      const bool capped =
-          (learned && hint && hint[g] > 0)
-              ? hint[g] >= chis[g]
-              : (ms[g]->dims[sites[g]] >= chis[g] || nsv[g] >= 2 * chis[g]);
-      cand[g] = nsv[g] > chis[g] && !attempt[g] && capped ? 1 : 0;
+      const bool structural =
+          ms[g]->dims[sites[g]] >= chis[g] || nsv[g] >= 2 * chis[g];
+      bool capped;
+      if (attempt[g])
+        capped = !acc[g] && structural;
+      else if (learned && hint && hint[g] > 0)
+        capped = hint[g] >= chis[g];
+      else
+        capped = structural;
+      cand[g] = nsv[g] > chis[g] && capped ? 1 : 0;
       any32 = any32 || cand[g];
     }
     if (any32) {
@@ -1186,23 +1209,6 @@ int32_t apply_gates_core(qtn_ctx* ctx, int ng, qtn_mps* const* ms,
           keptw[g] = kw32[g];
         }
       }
-    }
-  }
-  if (any_attempt) {
-    {
-      std::vector<int> acc;
-      // each call draws a fresh sketch from the context's generator
-      ctx->sketch_seed =
-          ctx->sketch_seed * 6364136223846793005ULL + 1442695040888963407ULL;
-      const uint64_t seed = ctx->sketch_seed;
-      QTN_TRY(rand_range_finder(ctx, ng, gd, d_gd, rows_p, attempt, lw, chis,
-                                cutoff, seed, acc, kept, keptw, disc,
-                                total));
-      for (int g = 0; g < ng; ++g)
-        if (acc[g]) {
-          src[g] = 2;
-          need64[g] = 0;
-        }
     }
   }
   if (rand_accepted)
