@@ -1,7 +1,8 @@
 """Edge cases of the device MPS path (qtn_mps_*): the smallest chain, the
 boundary sites, a bond cap of 1, an empty layer, and the error statuses
 for a zero gate, a non-finite gate, a site out of range and a cap above
-the state's capacity — each error leaves the state as it was."""
+the state's capacity — each error leaves the state as it was — and the
+teardown order of a context closed while its objects are alive."""
 import numpy as np
 import pytest
 
@@ -104,3 +105,31 @@ def test_cap_above_capacity_is_rejected(gpu_ctx):
     with pytest.raises(QtnError) as e:
         dev.apply_layer([1, 3], [np.eye(4), np.eye(4)], 1e-10, 8)
     assert e.value.status == QTN_ERR_ARG
+
+
+def test_context_close_frees_live_objects():
+    """Closing a context while states, an MPO and an MLP made on it are
+    alive frees them first (their destroy calls use the context); their
+    handles are then gone, and a new context works normally."""
+    from qaintensor import Context, DeviceMPO, learned
+    from qaintensor import hamiltonians as Hm
+    ctx = Context(0)
+    dev = DeviceMPS(6, 8, ctx)
+    dev.apply_2q(3, C.haar_unitary(np.random.default_rng(5), 4), 1e-12, 8)
+    mpo = DeviceMPO(Hm.tfim_mpo(6, 1.0, Hm.tfim_fields(6, 1)), ctx)
+    mlp = learned.RankMLP(learned.init_params(0), ctx=ctx)
+    ctx.close()
+    for obj in (dev, mpo):
+        with pytest.raises(QtnError) as e:
+            obj.handle
+        assert e.value.status == QTN_ERR_ARG
+    assert mlp.handle is None
+    dev.free()  # already freed: no-op
+    ctx2 = Context(0)
+    try:
+        fresh = DeviceMPS(4, 4, ctx2)
+        fresh.apply_2q(2, np.eye(4), 1e-12, 4)
+        assert list(fresh.bond_dims()) == [1, 1, 1, 1, 1]
+        fresh.free()
+    finally:
+        ctx2.close()
