@@ -311,11 +311,11 @@ int32_t qtn_mps_apply_mpo(const qtn_mps* in, const qtn_mpo* mpo,
                           double* norm);
 /* qtn_mps_apply_mpo in a chosen form.  QTN_FORM_CHOSEN: each zip-up split
  * whose rank the cap can decide (min(rows, cols) > its cap) runs its Jacobi
- * sweeps in complex FP32; when FP32 keeps exactly the cap, the kept left
- * vectors are re-orthonormalised in FP64 (three Newton-Schulz steps) and
- * S V^H is recomputed as U^H A in FP64, else the split is redone in FP64.
- * The right-to-left recompression stays FP64.  No counterpart in the
- * reference (it has the plain form only).                                 */
+ * sweeps in complex FP32; when FP32 keeps exactly the cap, the leading
+ * cap + 64 left vectors are re-orthonormalised in FP64 (three Newton-Schulz
+ * steps) and A's projection on them is split in FP64 (Rayleigh-Ritz),
+ * else the split is redone in FP64.  The right-to-left recompression stays
+ * FP64.  No counterpart in the reference (it has the plain form only).    */
 int32_t qtn_mps_apply_mpo_form(const qtn_mps* in, const qtn_mpo* mpo,
                                double cutoff, int64_t chimax, int32_t form,
                                qtn_mps* out, double* norm);

@@ -124,76 +124,6 @@ __global__ void ns_step(double2* __restrict__ Q, const double2* __restrict__ T,
   }
 }
 
-// S[j] = 2-norm of row j of B (k x n, ld k); one wave per row.
-__global__ __launch_bounds__(256) void row_norms(const double2* __restrict__ B,
-                                                 int64_t k, int64_t n,
-                                                 double* __restrict__ S) {
-  const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (j >= k) return;
-  double acc = 0.0;
-  for (int64_t c = lane; c < n; c += 64) {
-    const double2 v = B[j + c * k];
-    acc += v.x * v.x + v.y * v.y;
-  }
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-  if (lane == 0) S[j] = sqrt(acc);
-}
-
-// S[j] = 2-norm of column j of B (m x k, ld m); one wave per column.
-__global__ __launch_bounds__(256) void col_norms(const double2* __restrict__ B,
-                                                 int64_t m, int64_t k,
-                                                 double* __restrict__ S) {
-  const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (j >= k) return;
-  double acc = 0.0;
-  for (int64_t r = lane; r < m; r += 64) {
-    const double2 v = B[r + j * m];
-    acc += v.x * v.x + v.y * v.y;
-  }
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-  if (lane == 0) S[j] = sqrt(acc);
-}
-
-// U (m x k) = US / S column-wise; Vh (k x n, ld k) row j = f_j conj(V[:, j])^T
-// with f_j = S[j] (absorb) or 1.
-__global__ void finish_left(double2* __restrict__ US, int64_t m, int64_t k,
-                            const double* __restrict__ S) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m * k;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const double s = S[i / m];
-    const double inv = s > 0.0 ? 1.0 / s : 0.0;
-    US[i].x *= inv;
-    US[i].y *= inv;
-  }
-}
-
-__global__ void v_to_vh(const double2* __restrict__ V, int64_t n, int64_t k,
-                        const double* __restrict__ S, int absorb,
-                        double2* __restrict__ Vh) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * k;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = i % n, j = i / n;
-    const double f = absorb ? S[j] : 1.0;
-    const double2 v = V[r + j * n];
-    Vh[j + r * k] = make_double2(f * v.x, -f * v.y);
-  }
-}
-
-// dst = f * src row scaling undone: Vh (k x n) of U^H A is S V^H; rows are
-// divided by S when V^H itself is asked for.
-__global__ void unabsorb_rows(double2* __restrict__ B, int64_t k, int64_t n,
-                              const double* __restrict__ S) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k * n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const double s = S[i % k];
-    const double inv = s > 0.0 ? 1.0 / s : 0.0;
-    B[i].x *= inv;
-    B[i].y *= inv;
-  }
-}
-
 unsigned grid_for(int64_t n) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256,
                                                           8192));
@@ -294,12 +224,14 @@ int32_t svd_device(qtn_ctx* ctx, const double2* dA, int64_t m, int64_t n,
 // The chosen form of svd_device for a truncation the cap may decide
 // (truncate with chimax > 0 and min(m, n) > chimax): the Jacobi sweeps run
 // on a complex FP32 copy; the result is kept when the cap binds inside the
-// spectrum FP32 resolves (kept == chimax, clean status), and then the kept
-// singular vectors (U when m < n, V otherwise) are taken to FP64 and
-// re-orthonormalised by three Newton-Schulz steps, Q <- Q (3 I - Q^H Q) / 2,
-// after which the other factor and S come from A by FP64 GEMMs (S V^H =
-// U^H A, or U S = A V).  Otherwise the FP64 svd_device runs.  Outputs as
-// svd_device (dU and dVh required).  Uses slots 200-211.
+// spectrum FP32 resolves (kept == chimax, clean status).  Then the leading
+// chimax + 64 singular vectors (U when m < n, V otherwise) are taken to
+// FP64 and re-orthonormalised by three Newton-Schulz steps,
+// Q <- Q (3 I - Q^H Q) / 2, and A's projection on span(Q) is split by the
+// FP64 engine (Rayleigh-Ritz), whose factors give U, S and V^H.  If FP32
+// or the projection keeps another rank, the FP64 svd_device runs on A.
+// Outputs as svd_device (dU and dVh required).  Uses slots 200-211 and
+// svd_device's.
 int32_t svd_device_chosen(qtn_ctx* ctx, const double2* dA, int64_t m,
                           int64_t n, int64_t lda, double cutoff, int relative,
                           int64_t chimax, int absorb, double2* dU, double* dS,
@@ -342,61 +274,64 @@ int32_t svd_device_chosen(qtn_ctx* ctx, const double2* dA, int64_t m,
   if (st != 0 || k != chimax)
     return svd_device(ctx, dA, m, n, lda, true, cutoff, relative, chimax,
                       absorb, dU, dS, dVh, k_out, disc_out);
-  // Q: the kept vectors from W (U when trans, V otherwise), FP64
+  // Q: the leading kext = k + 64 vectors from W (U when trans, V
+  // otherwise), FP64, re-orthonormalised by Newton-Schulz steps.
+  const int64_t kext = std::min(nsv, k + 64);
   const int64_t r = trans ? m : n;
-  void *dQ, *dG, *dT, *dSv;
-  QTN_TRY(ctx->buffer(207, (size_t)r * k * sizeof(double2), &dQ));
-  QTN_TRY(ctx->buffer(208, (size_t)k * k * sizeof(double2), &dG));
-  QTN_TRY(ctx->buffer(209, (size_t)std::max(r, m) * k * sizeof(double2),
-                      &dT));
-  QTN_TRY(ctx->buffer(210, (size_t)k * sizeof(double), &dSv));
+  void *dQ, *dG, *dT, *dB, *dF;
+  QTN_TRY(ctx->buffer(207, (size_t)r * kext * sizeof(double2), &dQ));
+  QTN_TRY(ctx->buffer(208, (size_t)kext * kext * sizeof(double2), &dG));
+  QTN_TRY(ctx->buffer(209, (size_t)r * kext * sizeof(double2), &dT));
+  QTN_TRY(ctx->buffer(210, (size_t)(trans ? n : m) * kext * sizeof(double2),
+                      &dB));
+  QTN_TRY(ctx->buffer(211, (size_t)kext * k * sizeof(double2), &dF));
   double2* Q = (double2*)dQ;
   double2* G = (double2*)dG;
   double2* T = (double2*)dT;
-  double* Sv = (double*)dSv;
-  hipLaunchKernelGGL(gather_cols32, dim3(grid_for(r * k)), dim3(256), 0, s,
-                     (const float2*)dW, cols_p, r, eng.perm, k, Q);
+  double2* B = (double2*)dB;
+  double2* F = (double2*)dF;
+  hipLaunchKernelGGL(gather_cols32, dim3(grid_for(r * kext)), dim3(256), 0,
+                     s, (const float2*)dW, cols_p, r, eng.perm, kext, Q);
   QTN_CHECK_HIP(hipGetLastError());
   const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
   for (int it = 0; it < 3; ++it) {
-    QTN_TRY(launch_zgemm(s, 2, 0, k, k, r, one, Q, r, 0, Q, r, 0, zero, G, k,
-                         0, 1));
-    QTN_TRY(launch_zgemm(s, 0, 0, r, k, k, one, Q, r, 0, G, k, 0, zero, T, r,
-                         0, 1));
-    hipLaunchKernelGGL(ns_step, dim3(grid_for(r * k)), dim3(256), 0, s, Q, T,
-                       r * k);
+    QTN_TRY(launch_zgemm(s, 2, 0, kext, kext, r, one, Q, r, 0, Q, r, 0, zero,
+                         G, kext, 0, 1));
+    QTN_TRY(launch_zgemm(s, 0, 0, r, kext, kext, one, Q, r, 0, G, kext, 0,
+                         zero, T, r, 0, 1));
+    hipLaunchKernelGGL(ns_step, dim3(grid_for(r * kext)), dim3(256), 0, s, Q,
+                       T, r * kext);
     QTN_CHECK_HIP(hipGetLastError());
   }
+  // Rayleigh-Ritz in FP64: the projection of A on span(Q) is split by the
+  // FP64 engine (its columns are orthogonal to FP32 accuracy, so it takes
+  // few sweeps), which settles the order of the values near the cap that
+  // FP32 cannot resolve.  The split must keep exactly the cap again.
+  int64_t kb = 0;
+  double db = 0.0;
   if (trans) {
-    // U = Q; S V^H = U^H A (k x n, ld k); S = its row norms
-    QTN_CHECK_HIP(hipMemcpyAsync(dU, Q, (size_t)m * k * sizeof(double2),
-                                 hipMemcpyDeviceToDevice, s));
-    QTN_TRY(launch_zgemm(s, 2, 0, k, n, m, one, Q, m, 0, dA, lda, 0, zero,
-                         dVh, k, 0, 1));
-    hipLaunchKernelGGL(row_norms, dim3((unsigned)((k + 3) / 4)), dim3(256), 0,
-                       s, (const double2*)dVh, k, n, Sv);
-    if (!absorb)
-      hipLaunchKernelGGL(unabsorb_rows, dim3(grid_for(k * n)), dim3(256), 0,
-                         s, dVh, k, n, (const double*)Sv);
+    // B = Q^H A (kext x n) = Ub S Vh: U = Q Ub
+    QTN_TRY(launch_zgemm(s, 2, 0, kext, n, m, one, Q, m, 0, dA, lda, 0, zero,
+                         B, kext, 0, 1));
+    QTN_TRY(svd_device(ctx, B, kext, n, kext, true, cutoff, relative, chimax,
+                       absorb, F, dS, dVh, &kb, &db));
   } else {
-    // V = Q; U S = A V (m x k); S = its column norms; U = (A V) / S
-    QTN_TRY(launch_zgemm(s, 0, 0, m, k, n, one, dA, lda, 0, Q, n, 0, zero, T,
-                         m, 0, 1));
-    hipLaunchKernelGGL(col_norms, dim3((unsigned)((k + 3) / 4)), dim3(256), 0,
-                       s, (const double2*)T, m, k, Sv);
-    hipLaunchKernelGGL(finish_left, dim3(grid_for(m * k)), dim3(256), 0, s, T,
-                       m, k, (const double*)Sv);
-    QTN_CHECK_HIP(hipMemcpyAsync(dU, T, (size_t)m * k * sizeof(double2),
-                                 hipMemcpyDeviceToDevice, s));
-    hipLaunchKernelGGL(v_to_vh, dim3(grid_for(n * k)), dim3(256), 0, s,
-                       (const double2*)Q, n, k, (const double*)Sv, absorb,
-                       dVh);
+    // B = A Q (m x kext) = U S Vb: V^H = Vb Q^H
+    QTN_TRY(launch_zgemm(s, 0, 0, m, kext, n, one, dA, lda, 0, Q, n, 0, zero,
+                         B, m, 0, 1));
+    QTN_TRY(svd_device(ctx, B, m, kext, m, true, cutoff, relative, chimax,
+                       absorb, dU, dS, F, &kb, &db));
   }
-  QTN_CHECK_HIP(hipGetLastError());
-  if (dS)
-    QTN_CHECK_HIP(hipMemcpyAsync(dS, Sv, k * sizeof(double),
-                                 hipMemcpyDeviceToDevice, s));
-  if (k_out) *k_out = k;
+  if (kb != chimax)
+    return svd_device(ctx, dA, m, n, lda, true, cutoff, relative, chimax,
+                      absorb, dU, dS, dVh, k_out, disc_out);
+  if (trans)
+    QTN_TRY(launch_zgemm(s, 0, 0, m, kb, kext, one, Q, m, 0, F, kext, 0, zero,
+                         dU, m, 0, 1));
+  else
+    QTN_TRY(launch_zgemm(s, 0, 2, kb, n, kext, one, F, kb, 0, Q, n, 0, zero,
+                         dVh, kb, 0, 1));
+  if (k_out) *k_out = kb;
   if (disc_out) *disc_out = disc;
   return QTN_OK;
 }
