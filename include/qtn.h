@@ -11,7 +11,8 @@
  *   - Sizes and dims are int64_t (Julia Int).  Leg numbers are 1-based.
  *   - Every function returns int32_t status: QTN_OK (0) or a negative code;
  *     qtn_last_error() then returns a message (thread-local).  Nothing throws
- *     or aborts across the ABI.
+ *     or aborts across the ABI: a host allocation failure inside the library
+ *     returns QTN_ERR_OOM, any other internal exception QTN_ERR_HIP.
  *   - Host pointers are only read during the call; the library keeps none.
  *     Entry points whose name ends in _dev take device pointers instead.
  *   - Calls are synchronous with respect to the host unless stated.
@@ -43,7 +44,7 @@ extern "C" {
 #define QTN_OK              0
 #define QTN_ERR_ARG        -1  /* invalid argument (dims, legs, cutoff, ...) */
 #define QTN_ERR_HIP        -2  /* HIP runtime error                          */
-#define QTN_ERR_OOM        -3  /* device allocation failed                   */
+#define QTN_ERR_OOM        -3  /* device or host allocation failed           */
 #define QTN_ERR_NOCONV     -4  /* SVD sweep cap reached before convergence   */
 #define QTN_ERR_TRUNC      -5  /* every singular value is within the cutoff  */
 #define QTN_ERR_NOTFINITE  -6  /* NaN/Inf met in the input of an SVD         */

@@ -241,7 +241,7 @@ extern "C" {
 int32_t qtn_zgemm_bf16x3(qtn_ctx* ctx, char opA, int64_t m, int64_t n,
                          int64_t k, const double* A, int64_t lda,
                          const double* B, int64_t ldb, double* C,
-                         int64_t ldc) {
+                         int64_t ldc) try {
   clear_error();
   QTN_ARG(ctx && A && B && C, "qtn_zgemm_bf16x3: NULL argument");
   QTN_ARG(m >= 0 && n >= 0 && k >= 0, "qtn_zgemm_bf16x3: negative size");
@@ -275,6 +275,8 @@ int32_t qtn_zgemm_bf16x3(qtn_ctx* ctx, char opA, int64_t m, int64_t n,
                                hipMemcpyDeviceToHost, s));
   QTN_CHECK_HIP(hipStreamSynchronize(s));
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 }  // extern "C"

@@ -340,7 +340,7 @@ int32_t qtn_contract_pair_dev(qtn_ctx* ctx, const double* A, int32_t ndA,
                               const int64_t* dimsA, const double* B,
                               int32_t ndB, const int64_t* dimsB, int32_t nc,
                               const int32_t* legsA, const int32_t* legsB,
-                              const int32_t* outperm, double* C) {
+                              const int32_t* outperm, double* C) try {
   qtn::clear_error();
   QTN_ARG(ctx != nullptr, "qtn_contract_pair_dev: ctx is NULL");
   QTN_ARG((ndA == 0 || dimsA) && (ndB == 0 || dimsB) &&
@@ -352,13 +352,15 @@ int32_t qtn_contract_pair_dev(qtn_ctx* ctx, const double* A, int32_t ndA,
   QTN_CHECK_HIP(hipSetDevice(ctx->device));
   return qtn::run_plan(ctx, pl, (const double2*)A, ndA, dimsA,
                        (const double2*)B, ndB, dimsB, (double2*)C);
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_contract_pair(qtn_ctx* ctx, const double* A, int32_t ndA,
                           const int64_t* dimsA, const double* B, int32_t ndB,
                           const int64_t* dimsB, int32_t nc,
                           const int32_t* legsA, const int32_t* legsB,
-                          const int32_t* outperm, double* C) {
+                          const int32_t* outperm, double* C) try {
   qtn::clear_error();
   QTN_ARG(ctx != nullptr, "qtn_contract_pair: ctx is NULL");
   QTN_ARG((ndA == 0 || dimsA) && (ndB == 0 || dimsB) &&
@@ -387,11 +389,13 @@ int32_t qtn_contract_pair(qtn_ctx* ctx, const double* A, int32_t ndA,
                                hipMemcpyDeviceToHost, ctx->stream));
   QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_permute_dev(qtn_ctx* ctx, const double* in, int32_t nd,
                         const int64_t* dims, const int32_t* perm,
-                        double* out) {
+                        double* out) try {
   qtn::clear_error();
   QTN_ARG(ctx != nullptr, "qtn_permute_dev: ctx is NULL");
   QTN_ARG(nd >= 0 && nd <= 16, "qtn_permute_dev: rank must be 0..16");
@@ -413,10 +417,12 @@ int32_t qtn_permute_dev(qtn_ctx* ctx, const double* in, int32_t nd,
   }
   return qtn::launch_permute(ctx->stream, (const double2*)in, nd, dims,
                              p0.data(), (double2*)out);
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_permute(qtn_ctx* ctx, const double* in, int32_t nd,
-                    const int64_t* dims, const int32_t* perm, double* out) {
+                    const int64_t* dims, const int32_t* perm, double* out) try {
   qtn::clear_error();
   QTN_ARG(ctx != nullptr, "qtn_permute: ctx is NULL");
   QTN_ARG(nd >= 0 && nd <= 16, "qtn_permute: rank must be 0..16");
@@ -439,6 +445,8 @@ int32_t qtn_permute(qtn_ctx* ctx, const double* in, int32_t nd,
                                hipMemcpyDeviceToHost, ctx->stream));
   QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 }  // extern "C"

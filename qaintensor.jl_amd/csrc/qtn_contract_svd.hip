@@ -35,7 +35,7 @@ extern "C" int32_t qtn_contract_svd(qtn_ctx* ctx, const double* A,
                                     int32_t ndA, const int64_t* dimsA,
                                     const double* B, int32_t ndB,
                                     const int64_t* dimsB, int32_t legA,
-                                    int32_t legB, double er, double* C) {
+                                    int32_t legB, double er, double* C) try {
   qtn::clear_error();
   QTN_ARG(ctx != nullptr && A != nullptr && B != nullptr && C != nullptr &&
               dimsA != nullptr && dimsB != nullptr,
@@ -104,4 +104,6 @@ extern "C" int32_t qtn_contract_svd(qtn_ctx* ctx, const double* A,
                                hipMemcpyDeviceToHost, s));
   QTN_CHECK_HIP(hipStreamSynchronize(s));
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }

@@ -12,6 +12,21 @@ static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 void clear_error() { g_last_error.clear(); }
 
+int32_t abi_exception() noexcept {
+  try {
+    throw;
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed");
+    return QTN_ERR_OOM;
+  } catch (const std::exception& e) {
+    set_error(std::string("internal error: ") + e.what());
+    return QTN_ERR_HIP;
+  } catch (...) {
+    set_error("internal error: unknown exception");
+    return QTN_ERR_HIP;
+  }
+}
+
 }  // namespace qtn
 
 int32_t qtn_ctx::buffer(int slot, size_t bytes, void** out) {
@@ -77,7 +92,7 @@ const char* qtn_last_error(void) { return qtn::g_last_error.c_str(); }
 
 const char* qtn_version(void) { return "libqtn 0.1.0 (gfx950)"; }
 
-int32_t qtn_ctx_create(int32_t device, qtn_ctx** ctx) {
+int32_t qtn_ctx_create(int32_t device, qtn_ctx** ctx) try {
   qtn::clear_error();
   QTN_ARG(ctx != nullptr, "qtn_ctx_create: ctx is NULL");
   *ctx = nullptr;
@@ -102,9 +117,11 @@ int32_t qtn_ctx_create(int32_t device, qtn_ctx** ctx) {
   }
   *ctx = c;
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
-int32_t qtn_ctx_destroy(qtn_ctx* ctx) {
+int32_t qtn_ctx_destroy(qtn_ctx* ctx) try {
   qtn::clear_error();
   if (!ctx) return QTN_OK;
   for (qtn_ctx* w : ctx->workers) QTN_TRY(qtn_ctx_destroy(w));
@@ -118,17 +135,21 @@ int32_t qtn_ctx_destroy(qtn_ctx* ctx) {
   QTN_CHECK_HIP(hipStreamDestroy(ctx->stream));
   delete ctx;
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
-int32_t qtn_ctx_sync(qtn_ctx* ctx) {
+int32_t qtn_ctx_sync(qtn_ctx* ctx) try {
   qtn::clear_error();
   QTN_ARG(ctx != nullptr, "qtn_ctx_sync: ctx is NULL");
   QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_ctx_counters(qtn_ctx* ctx, int64_t* out, int32_t n,
-                         int32_t reset) {
+                         int32_t reset) try {
   qtn::clear_error();
   QTN_ARG(ctx != nullptr && (n == 0 || out != nullptr),
           "qtn_ctx_counters: NULL argument");
@@ -149,9 +170,11 @@ int32_t qtn_ctx_counters(qtn_ctx* ctx, int64_t* out, int32_t n,
       w->n_svd = w->n_sweeps = w->max_sweeps = w->n_batches = 0;
   }
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
-int32_t qtn_ctx_profile(qtn_ctx* ctx, int32_t enable) {
+int32_t qtn_ctx_profile(qtn_ctx* ctx, int32_t enable) try {
   qtn::clear_error();
   QTN_ARG(ctx != nullptr, "qtn_ctx_profile: ctx is NULL");
   QTN_CHECK_HIP(hipSetDevice(ctx->device));
@@ -165,10 +188,12 @@ int32_t qtn_ctx_profile(qtn_ctx* ctx, int32_t enable) {
     }
   }
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_ctx_profile_read(qtn_ctx* ctx, int32_t kernel, int64_t* count,
-                             double* ms, double* bytes, double* flops) {
+                             double* ms, double* bytes, double* flops) try {
   qtn::clear_error();
   QTN_ARG(ctx != nullptr, "qtn_ctx_profile_read: ctx is NULL");
   QTN_ARG(kernel >= 0 && kernel < qtn_ctx::kProfKernels,
@@ -188,14 +213,18 @@ int32_t qtn_ctx_profile_read(qtn_ctx* ctx, int32_t kernel, int64_t* count,
   if (bytes) *bytes = b;
   if (flops) *flops = f;
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
-int32_t qtn_ctx_stream(qtn_ctx* ctx, void** stream) {
+int32_t qtn_ctx_stream(qtn_ctx* ctx, void** stream) try {
   qtn::clear_error();
   QTN_ARG(ctx != nullptr && stream != nullptr,
           "qtn_ctx_stream: NULL argument");
   *stream = (void*)ctx->stream;
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 }  // extern "C"

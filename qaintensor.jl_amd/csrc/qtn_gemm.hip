@@ -400,7 +400,7 @@ int32_t qtn_zgemm_dev(qtn_ctx* ctx, char opA, char opB, int64_t m, int64_t n,
                       int64_t k, const double* alpha, const double* A,
                       int64_t lda, int64_t strideA, const double* B,
                       int64_t ldb, int64_t strideB, const double* beta,
-                      double* C, int64_t ldc, int64_t strideC, int64_t batch) {
+                      double* C, int64_t ldc, int64_t strideC, int64_t batch) try {
   qtn::clear_error();
   QTN_ARG(ctx && alpha && beta, "qtn_zgemm_dev: NULL argument");
   const int oa = qtn::op_code(opA), ob = qtn::op_code(opB);
@@ -416,13 +416,15 @@ int32_t qtn_zgemm_dev(qtn_ctx* ctx, char opA, char opB, int64_t m, int64_t n,
                            (const double2*)B, ldb, strideB,
                            make_double2(beta[0], beta[1]), (double2*)C, ldc,
                            strideC, batch);
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_cgemm_dev(qtn_ctx* ctx, char opA, char opB, int64_t m, int64_t n,
                       int64_t k, const float* alpha, const float* A,
                       int64_t lda, int64_t strideA, const float* B,
                       int64_t ldb, int64_t strideB, const float* beta,
-                      float* C, int64_t ldc, int64_t strideC, int64_t batch) {
+                      float* C, int64_t ldc, int64_t strideC, int64_t batch) try {
   qtn::clear_error();
   QTN_ARG(ctx && alpha && beta, "qtn_cgemm_dev: NULL argument");
   const int oa = qtn::op_code(opA), ob = qtn::op_code(opB);
@@ -437,6 +439,8 @@ int32_t qtn_cgemm_dev(qtn_ctx* ctx, char opA, char opB, int64_t m, int64_t n,
                            lda, strideA, (const float2*)B, ldb, strideB,
                            make_float2(beta[0], beta[1]), (float2*)C, ldc,
                            strideC, batch);
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 }  // extern "C"

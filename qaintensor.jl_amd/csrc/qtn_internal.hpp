@@ -16,6 +16,10 @@ namespace qtn {
 
 void set_error(const std::string& msg);
 void clear_error();
+// Status for an exception caught at the C ABI (or in a worker thread):
+// QTN_ERR_OOM for a failed host allocation, QTN_ERR_HIP otherwise, with
+// the exception's text as the error message.  Call inside a catch block.
+int32_t abi_exception() noexcept;
 
 #define QTN_CHECK_HIP(call)                                                  \
   do {                                                                       \

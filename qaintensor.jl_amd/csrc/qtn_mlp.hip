@@ -326,7 +326,7 @@ using namespace qtn;
 extern "C" {
 
 int32_t qtn_mlp_create(qtn_ctx* ctx, int32_t nin, int32_t width,
-                       const float* params, qtn_mlp** out) {
+                       const float* params, qtn_mlp** out) try {
   clear_error();
   QTN_ARG(ctx && params && out, "qtn_mlp_create: NULL argument");
   QTN_ARG(nin >= 1 && nin <= 4096 && width >= 1 && width <= 4096,
@@ -355,9 +355,11 @@ int32_t qtn_mlp_create(qtn_ctx* ctx, int32_t nin, int32_t width,
   QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   *out = m;
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
-int32_t qtn_mlp_destroy(qtn_mlp* m) {
+int32_t qtn_mlp_destroy(qtn_mlp* m) try {
   clear_error();
   if (!m) return QTN_OK;
   QTN_CHECK_HIP(hipSetDevice(m->ctx->device));
@@ -365,9 +367,11 @@ int32_t qtn_mlp_destroy(qtn_mlp* m) {
   if (m->arena) QTN_CHECK_HIP(hipFree(m->arena));
   delete m;
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
-int32_t qtn_mlp_get_params(const qtn_mlp* m, float* params) {
+int32_t qtn_mlp_get_params(const qtn_mlp* m, float* params) try {
   clear_error();
   QTN_ARG(m && params, "qtn_mlp_get_params: NULL argument");
   QTN_CHECK_HIP(hipSetDevice(m->ctx->device));
@@ -375,9 +379,11 @@ int32_t qtn_mlp_get_params(const qtn_mlp* m, float* params) {
                                hipMemcpyDeviceToHost, m->ctx->stream));
   QTN_CHECK_HIP(hipStreamSynchronize(m->ctx->stream));
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
-int32_t qtn_mlp_forward(qtn_mlp* m, int64_t nb, const float* X, float* Y) {
+int32_t qtn_mlp_forward(qtn_mlp* m, int64_t nb, const float* X, float* Y) try {
   clear_error();
   QTN_ARG(m && X && Y, "qtn_mlp_forward: NULL argument");
   QTN_ARG(nb >= 1, "qtn_mlp_forward: empty batch");
@@ -391,10 +397,12 @@ int32_t qtn_mlp_forward(qtn_mlp* m, int64_t nb, const float* X, float* Y) {
                                hipMemcpyDeviceToHost, ctx->stream));
   QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_mlp_loss_grad(qtn_mlp* m, int64_t nb, const float* X,
-                          const float* T, float* loss, float* grad) {
+                          const float* T, float* loss, float* grad) try {
   clear_error();
   QTN_ARG(m && X && T && loss && grad, "qtn_mlp_loss_grad: NULL argument");
   QTN_ARG(nb >= 1, "qtn_mlp_loss_grad: empty batch");
@@ -416,11 +424,13 @@ int32_t qtn_mlp_loss_grad(qtn_mlp* m, int64_t nb, const float* X,
                                hipMemcpyDeviceToHost, ctx->stream));
   QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_mlp_train(qtn_mlp* m, int64_t nb, const float* X, const float* T,
                       int32_t steps, int64_t minibatch, float lr,
-                      float* loss_hist) {
+                      float* loss_hist) try {
   clear_error();
   QTN_ARG(m && X && T, "qtn_mlp_train: NULL argument");
   QTN_ARG(nb >= 1 && steps >= 0, "qtn_mlp_train: bad sizes");
@@ -459,6 +469,8 @@ int32_t qtn_mlp_train(qtn_mlp* m, int64_t nb, const float* X, const float* T,
                                  hipMemcpyDeviceToHost, s));
   QTN_CHECK_HIP(hipStreamSynchronize(s));
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 }  // extern "C"

@@ -183,7 +183,7 @@ using namespace qtn;
 extern "C" {
 
 int32_t qtn_mpo_create(qtn_ctx* ctx, int32_t nsites, const int64_t* D,
-                       const double* const* sites, qtn_mpo** out) {
+                       const double* const* sites, qtn_mpo** out) try {
   clear_error();
   QTN_ARG(ctx && D && sites && out, "qtn_mpo_create: NULL argument");
   QTN_ARG(nsites >= 1, "qtn_mpo_create: need at least one site");
@@ -229,9 +229,11 @@ int32_t qtn_mpo_create(qtn_ctx* ctx, int32_t nsites, const int64_t* D,
   QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   *out = m;
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
-int32_t qtn_mpo_destroy(qtn_mpo* m) {
+int32_t qtn_mpo_destroy(qtn_mpo* m) try {
   clear_error();
   if (!m) return QTN_OK;
   QTN_CHECK_HIP(hipSetDevice(m->ctx->device));
@@ -239,14 +241,18 @@ int32_t qtn_mpo_destroy(qtn_mpo* m) {
   if (m->arena) QTN_CHECK_HIP(hipFree(m->arena));
   delete m;
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
-int32_t qtn_mps_expect_mpo(const qtn_mps* m, const qtn_mpo* w, double* out) {
+int32_t qtn_mps_expect_mpo(const qtn_mps* m, const qtn_mpo* w, double* out) try {
   return qtn_mps_expect_mpo_form(m, w, QTN_FORM_PLAIN, out);
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_mps_expect_mpo_form(const qtn_mps* m, const qtn_mpo* w,
-                                int32_t form, double* out) {
+                                int32_t form, double* out) try {
   clear_error();
   QTN_ARG(m && w && out, "qtn_mps_expect_mpo: NULL argument");
   QTN_ARG(m->n == w->n, "qtn_mps_expect_mpo: MPS and MPO lengths differ");
@@ -311,10 +317,12 @@ int32_t qtn_mps_expect_mpo_form(const qtn_mps* m, const qtn_mpo* w,
                                hipMemcpyDeviceToHost, s));
   QTN_CHECK_HIP(hipStreamSynchronize(s));
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_mps_set_sites(qtn_mps* m, const int64_t* dims,
-                          const double* const* sites) {
+                          const double* const* sites) try {
   clear_error();
   QTN_ARG(m && dims && sites, "qtn_mps_set_sites: NULL argument");
   const int n = m->n;
@@ -382,17 +390,21 @@ int32_t qtn_mps_set_sites(qtn_mps* m, const int64_t* dims,
     wd[j + 1] = k;
   }
   return right_sweep(m, work, wd, 0.0, m->cap, nullptr);
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_mps_apply_mpo(const qtn_mps* in, const qtn_mpo* w, double cutoff,
-                          int64_t chimax, qtn_mps* out, double* norm) {
+                          int64_t chimax, qtn_mps* out, double* norm) try {
   return qtn_mps_apply_mpo_form(in, w, cutoff, chimax, QTN_FORM_PLAIN, out,
                                 norm);
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_mps_apply_mpo_form(const qtn_mps* in, const qtn_mpo* w,
                                double cutoff, int64_t chimax, int32_t form,
-                               qtn_mps* out, double* norm) {
+                               qtn_mps* out, double* norm) try {
   clear_error();
   QTN_ARG(in && w && out, "qtn_mps_apply_mpo: NULL argument");
   QTN_ARG(form == QTN_FORM_PLAIN || form == QTN_FORM_CHOSEN,
@@ -492,6 +504,8 @@ int32_t qtn_mps_apply_mpo_form(const qtn_mps* in, const qtn_mpo* w,
     k = kn;
   }
   return right_sweep(out, work, wd, cutoff, cm, norm);
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 }  // extern "C"

@@ -1530,22 +1530,36 @@ int32_t apply_gates_grouped(qtn_ctx* ctx, int32_t ngates,
   std::vector<int32_t> status(nexec, QTN_OK);
   std::vector<std::string> message(nexec);
   auto drive = [&](int e) {
-    if (e > 0 && hipSetDevice(exec[e]->device) != hipSuccess) {
-      status[e] = QTN_ERR_HIP;
-      message[e] = "hipSetDevice failed in a worker thread";
-      return;
-    }
-    for (int gi : plan[e]) {
-      status[e] = run_group(exec[e], groups[gi].idx);
-      if (status[e] != QTN_OK) {
-        message[e] = qtn_last_error();  // thread-local: carried back below
+    // an exception must not leave a worker thread (std::terminate)
+    try {
+      if (e > 0 && hipSetDevice(exec[e]->device) != hipSuccess) {
+        status[e] = QTN_ERR_HIP;
+        message[e] = "hipSetDevice failed in a worker thread";
         return;
       }
+      for (int gi : plan[e]) {
+        status[e] = run_group(exec[e], groups[gi].idx);
+        if (status[e] != QTN_OK) {
+          message[e] = qtn_last_error();  // thread-local: carried back below
+          return;
+        }
+      }
+    } catch (...) {
+      status[e] = abi_exception();
+      message[e] = qtn_last_error();
     }
   };
   std::vector<std::thread> threads;
-  for (int e = 1; e < nexec; ++e) threads.emplace_back(drive, e);
+  std::vector<int> here;  // executors whose thread could not be started
+  for (int e = 1; e < nexec; ++e) {
+    try {
+      threads.emplace_back(drive, e);
+    } catch (...) {
+      here.push_back(e);
+    }
+  }
   drive(0);
+  for (int e : here) drive(e);
   for (auto& t : threads) t.join();
   for (int e = 0; e < nexec; ++e)
     if (status[e] != QTN_OK) {
@@ -1609,7 +1623,7 @@ using namespace qtn;
 extern "C" {
 
 int32_t qtn_mps_create(qtn_ctx* ctx, int32_t nsites, int64_t chimax_cap,
-                       qtn_mps** out) {
+                       qtn_mps** out) try {
   clear_error();
   QTN_ARG(ctx && out, "qtn_mps_create: NULL argument");
   QTN_ARG(nsites >= 2, "qtn_mps_create: need at least 2 sites");
@@ -1660,9 +1674,11 @@ int32_t qtn_mps_create(qtn_ctx* ctx, int32_t nsites, int64_t chimax_cap,
   QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   *out = m;
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
-int32_t qtn_mps_destroy(qtn_mps* m) {
+int32_t qtn_mps_destroy(qtn_mps* m) try {
   clear_error();
   if (!m) return QTN_OK;
   QTN_CHECK_HIP(hipSetDevice(m->ctx->device));
@@ -1670,9 +1686,11 @@ int32_t qtn_mps_destroy(qtn_mps* m) {
   if (m->arena) QTN_CHECK_HIP(hipFree(m->arena));
   delete m;
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
-int32_t qtn_mps_copy(const qtn_mps* a, qtn_mps* b) {
+int32_t qtn_mps_copy(const qtn_mps* a, qtn_mps* b) try {
   clear_error();
   QTN_ARG(a && b, "qtn_mps_copy: NULL argument");
   QTN_ARG(a->n == b->n && a->cap == b->cap,
@@ -1692,17 +1710,21 @@ int32_t qtn_mps_copy(const qtn_mps* a, qtn_mps* b) {
   QTN_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
   b->dims = a->dims;
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
-int32_t qtn_mps_bond_dims(const qtn_mps* m, int64_t* dims) {
+int32_t qtn_mps_bond_dims(const qtn_mps* m, int64_t* dims) try {
   clear_error();
   QTN_ARG(m && dims, "qtn_mps_bond_dims: NULL argument");
   std::copy(m->dims.begin(), m->dims.end(), dims);
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_mps_bond_spectrum(const qtn_mps* m, int32_t bond, double* S,
-                              int64_t* k) {
+                              int64_t* k) try {
   clear_error();
   QTN_ARG(m && S && k, "qtn_mps_bond_spectrum: NULL argument");
   QTN_ARG(bond >= 0 && bond <= m->n, "qtn_mps_bond_spectrum: bad bond");
@@ -1712,10 +1734,12 @@ int32_t qtn_mps_bond_spectrum(const qtn_mps* m, int32_t bond, double* S,
   QTN_CHECK_HIP(hipMemcpy(S, m->lam[bond], *k * sizeof(double),
                           hipMemcpyDeviceToHost));
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_mps_get_site(const qtn_mps* m, int32_t site, double* out,
-                         int64_t* dims) {
+                         int64_t* dims) try {
   clear_error();
   QTN_ARG(m && dims, "qtn_mps_get_site: NULL argument");
   QTN_ARG(site >= 1 && site <= m->n, "qtn_mps_get_site: bad site");
@@ -1730,9 +1754,11 @@ int32_t qtn_mps_get_site(const qtn_mps* m, int32_t site, double* out,
                             hipMemcpyDeviceToHost));
   }
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
-int32_t qtn_mps_apply_1q(qtn_mps* m, int32_t site, const double* u) {
+int32_t qtn_mps_apply_1q(qtn_mps* m, int32_t site, const double* u) try {
   clear_error();
   QTN_ARG(m && u, "qtn_mps_apply_1q: NULL argument");
   QTN_ARG(site >= 1 && site <= m->n, "qtn_mps_apply_1q: site out of range");
@@ -1749,12 +1775,14 @@ int32_t qtn_mps_apply_1q(qtn_mps* m, int32_t site, const double* u) {
   QTN_CHECK_HIP(hipGetLastError());
   QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_mps_apply_2q(qtn_mps* m, int32_t site, const double* g,
                          double cutoff, int64_t chimax, int32_t form,
                          int64_t* kept, double* disc_weight, double* spectrum,
-                         int64_t* spectrum_len) {
+                         int64_t* spectrum_len) try {
   clear_error();
   QTN_ARG(m && g, "qtn_mps_apply_2q: NULL argument");
   QTN_ARG(cutoff >= 0.0, "Error must be positive");
@@ -1765,12 +1793,14 @@ int32_t qtn_mps_apply_2q(qtn_mps* m, int32_t site, const double* g,
   const int64_t cm = (chimax > 0) ? chimax : m->cap;
   return mps_apply_gates(m, 1, &site, g, cutoff, cm, form, kept,
                          disc_weight, spectrum, spectrum_len);
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_mps_apply_layer(qtn_mps* m, int32_t ngates, const int32_t* sites,
                             const double* gates, double cutoff,
                             int64_t chimax, int32_t form, int64_t* kept,
-                            double* disc_weight) {
+                            double* disc_weight) try {
   clear_error();
   QTN_ARG(m && (ngates == 0 || (sites && gates)),
           "qtn_mps_apply_layer: NULL argument");
@@ -1785,10 +1815,12 @@ int32_t qtn_mps_apply_layer(qtn_mps* m, int32_t ngates, const int32_t* sites,
   std::vector<int64_t> chis(ngates, cm);
   return apply_gates_grouped(m->ctx, ngates, ms.data(), sites, gates,
                              chis.data(), cutoff, form, kept, disc_weight);
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_mps_amplitudes(const qtn_mps* m, int64_t nb, const uint8_t* bits,
-                           double* out) {
+                           double* out) try {
   clear_error();
   QTN_ARG(m && bits && out, "qtn_mps_amplitudes: NULL argument");
   QTN_ARG(nb >= 1, "qtn_mps_amplitudes: nb must be >= 1");
@@ -1823,9 +1855,11 @@ int32_t qtn_mps_amplitudes(const qtn_mps* m, int64_t nb, const uint8_t* bits,
                                hipMemcpyDeviceToHost, s));
   QTN_CHECK_HIP(hipStreamSynchronize(s));
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
-int32_t qtn_mps_expect_z(const qtn_mps* m, double* out) {
+int32_t qtn_mps_expect_z(const qtn_mps* m, double* out) try {
   clear_error();
   QTN_ARG(m && out, "qtn_mps_expect_z: NULL argument");
   qtn_ctx* ctx = m->ctx;
@@ -1871,9 +1905,11 @@ int32_t qtn_mps_expect_z(const qtn_mps* m, double* out) {
   QTN_CHECK_HIP(hipStreamSynchronize(s));
   for (int j = 0; j < m->n; ++j) out[j] = h[4 * j + 2] / h[4 * j + 0];
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
-int32_t qtn_mps_overlap(const qtn_mps* a, const qtn_mps* b, double* out) {
+int32_t qtn_mps_overlap(const qtn_mps* a, const qtn_mps* b, double* out) try {
   clear_error();
   QTN_ARG(a && b && out, "qtn_mps_overlap: NULL argument");
   QTN_ARG(a->n == b->n, "qtn_mps_overlap: different number of sites");
@@ -1909,17 +1945,21 @@ int32_t qtn_mps_overlap(const qtn_mps* a, const qtn_mps* b, double* out) {
                                hipMemcpyDeviceToHost, s));
   QTN_CHECK_HIP(hipStreamSynchronize(s));
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
-int32_t qtn_mps_norm2(const qtn_mps* m, double* out) {
+int32_t qtn_mps_norm2(const qtn_mps* m, double* out) try {
   double v[2];
   QTN_TRY(qtn_mps_overlap(m, m, v));
   *out = v[0];
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_mps_apply_1q_multi(int32_t ngates, qtn_mps* const* states,
-                               const int32_t* sites, const double* us) {
+                               const int32_t* sites, const double* us) try {
   clear_error();
   QTN_ARG(ngates >= 0, "qtn_mps_apply_1q_multi: negative count");
   if (ngates == 0) return QTN_OK;
@@ -1951,13 +1991,15 @@ int32_t qtn_mps_apply_1q_multi(int32_t ngates, qtn_mps* const* states,
   QTN_CHECK_HIP(hipGetLastError());
   QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_mps_apply_layer_multi(int32_t ngates, qtn_mps* const* states,
                                   const int32_t* sites, const double* gates,
                                   const int64_t* chimax, double cutoff,
                                   int32_t form, int64_t* kept,
-                                  double* disc_weight) {
+                                  double* disc_weight) try {
   clear_error();
   QTN_ARG(ngates >= 0, "qtn_mps_apply_layer_multi: negative count");
   if (ngates == 0) return QTN_OK;
@@ -1971,6 +2013,8 @@ int32_t qtn_mps_apply_layer_multi(int32_t ngates, qtn_mps* const* states,
   QTN_CHECK_HIP(hipSetDevice(ctx->device));
   return apply_gates_grouped(ctx, ngates, states, sites, gates, chimax,
                              cutoff, form, kept, disc_weight);
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 }  // extern "C"
@@ -2017,7 +2061,7 @@ int32_t gather_features(qtn_mps* m, int ng, const int32_t* sites,
 extern "C" {
 
 int32_t qtn_gate_schmidt(qtn_ctx* ctx, int32_t ngates, const double* gates,
-                         double* out) {
+                         double* out) try {
   clear_error();
   QTN_ARG(ctx && (ngates == 0 || (gates && out)),
           "qtn_gate_schmidt: NULL argument");
@@ -2039,11 +2083,13 @@ int32_t qtn_gate_schmidt(qtn_ctx* ctx, int32_t ngates, const double* gates,
                                hipMemcpyDeviceToHost, s));
   QTN_CHECK_HIP(hipStreamSynchronize(s));
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_mps_gate_features(qtn_mps* m, int32_t ngates,
                               const int32_t* sites, const double* schmidt,
-                              int64_t chimax, float* out) {
+                              int64_t chimax, float* out) try {
   clear_error();
   QTN_ARG(m && (ngates == 0 || (sites && schmidt && out)),
           "qtn_mps_gate_features: NULL argument");
@@ -2058,6 +2104,8 @@ int32_t qtn_mps_gate_features(qtn_mps* m, int32_t ngates,
                                hipMemcpyDeviceToHost, m->ctx->stream));
   QTN_CHECK_HIP(hipStreamSynchronize(m->ctx->stream));
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_mps_apply_layer_predicted(qtn_mps* m, int32_t ngates,
@@ -2067,7 +2115,7 @@ int32_t qtn_mps_apply_layer_predicted(qtn_mps* m, int32_t ngates,
                                       int64_t chimax, qtn_mlp* mlp,
                                       double margin, int64_t* kept,
                                       double* disc_weight, int32_t* accepted,
-                                      int64_t* hints) {
+                                      int64_t* hints) try {
   clear_error();
   QTN_ARG(m && mlp && (ngates == 0 || (sites && gates && schmidt)),
           "qtn_mps_apply_layer_predicted: NULL argument");
@@ -2099,13 +2147,15 @@ int32_t qtn_mps_apply_layer_predicted(qtn_mps* m, int32_t ngates,
   }
   return qtn_mps_apply_layer_learned(m, ngates, sites, gates, cutoff, chimax,
                                      h.data(), kept, disc_weight, accepted);
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_mps_apply_layer_learned(qtn_mps* m, int32_t ngates,
                                     const int32_t* sites, const double* gates,
                                     double cutoff, int64_t chimax,
                                     const int64_t* rank_hint, int64_t* kept,
-                                    double* disc_weight, int32_t* accepted) {
+                                    double* disc_weight, int32_t* accepted) try {
   clear_error();
   QTN_ARG(m && (ngates == 0 || (sites && gates && rank_hint)),
           "qtn_mps_apply_layer_learned: NULL argument");
@@ -2124,6 +2174,8 @@ int32_t qtn_mps_apply_layer_learned(qtn_mps* m, int32_t ngates,
   if (accepted)
     for (int g = 0; g < ngates; ++g) accepted[g] = acc[g];
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 }  // extern "C"

@@ -1618,7 +1618,7 @@ extern "C" {
 
 int32_t qtn_tune_jacobi(qtn_ctx* ctx, int64_t rows, int64_t cols,
                         int32_t batch, int32_t single, int32_t* best,
-                        double* us) {
+                        double* us) try {
   clear_error();
   QTN_ARG(ctx && best, "qtn_tune_jacobi: NULL argument");
   QTN_ARG(rows >= 1 && cols >= 1 && rows <= 4096 && cols <= 4096,
@@ -1632,10 +1632,12 @@ int32_t qtn_tune_jacobi(qtn_ctx* ctx, int64_t rows, int64_t cols,
     QTN_TRY(tune_bucket<double>(ctx, rows, cols, batch, &code, us));
   *best = code;
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_set_jacobi_tiling(qtn_ctx* ctx, int64_t cols, int32_t single,
-                              int32_t batch, int32_t tiling) {
+                              int32_t batch, int32_t tiling) try {
   clear_error();
   QTN_ARG(ctx, "qtn_set_jacobi_tiling: NULL context");
   QTN_ARG(cols >= 1 && cols <= 4096,
@@ -1647,6 +1649,8 @@ int32_t qtn_set_jacobi_tiling(qtn_ctx* ctx, int64_t cols, int32_t single,
   ctx->jac_tiling[{round_up64(cols), single ? 1 : 0, batch_class(batch)}] =
       tiling;
   return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 }  // extern "C"

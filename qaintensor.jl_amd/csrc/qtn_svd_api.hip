@@ -381,7 +381,7 @@ extern "C" {
 int32_t qtn_svd_trunc(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
                       int64_t lda, double cutoff, int32_t relative,
                       int64_t chimax, int64_t* k, double* U, double* S,
-                      double* Vh, double* disc_weight) {
+                      double* Vh, double* disc_weight) try {
   qtn::clear_error();
   QTN_ARG(ctx != nullptr && A != nullptr, "qtn_svd_trunc: NULL argument");
   QTN_ARG(m >= 1 && n >= 1, "qtn_svd_trunc: empty matrix");
@@ -390,11 +390,13 @@ int32_t qtn_svd_trunc(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
   QTN_CHECK_HIP(hipSetDevice(ctx->device));
   return qtn::svd_host(ctx, m, n, A, lda, true, cutoff, relative, chimax, 0,
                        k, U, S, Vh, disc_weight);
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_svd(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
                 int64_t lda, int32_t absorb, double* U, double* S,
-                double* Vh) {
+                double* Vh) try {
   qtn::clear_error();
   QTN_ARG(ctx != nullptr && A != nullptr, "qtn_svd: NULL argument");
   QTN_ARG(m >= 1 && n >= 1, "qtn_svd: empty matrix");
@@ -403,10 +405,12 @@ int32_t qtn_svd(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
   QTN_CHECK_HIP(hipSetDevice(ctx->device));
   return qtn::svd_host(ctx, m, n, A, lda, false, 0.0, 0, 0, absorb, nullptr,
                        U, S, Vh, nullptr);
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 int32_t qtn_svd_values(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
-                       int64_t lda, double* S) {
+                       int64_t lda, double* S) try {
   qtn::clear_error();
   QTN_ARG(ctx != nullptr && A != nullptr && S != nullptr,
           "qtn_svd_values: NULL argument");
@@ -415,6 +419,8 @@ int32_t qtn_svd_values(qtn_ctx* ctx, int64_t m, int64_t n, const double* A,
   QTN_CHECK_HIP(hipSetDevice(ctx->device));
   return qtn::svd_host(ctx, m, n, A, lda, false, 0.0, 0, 0, 0, nullptr,
                        nullptr, S, nullptr, nullptr);
+} catch (...) {
+  return qtn::abi_exception();
 }
 
 }  // extern "C"
