@@ -141,6 +141,59 @@ def test_apply_mpo_chosen_form_zipup(n, chi, cap):
         ctx.close()
 
 
+def test_apply_mpo_chosen_form_falls_back_below_the_cap():
+    """|a> + |b> (two random product states: rank 2 at every bond) under
+    W = 2 I written with bond dimension 2 (two identical paths), cap 3: the
+    zip-up splits are 4 x 4 of rank 2, so each is decided by the cutoff
+    below the cap, FP32 keeps 2 != 3 and the split is redone in FP64; the
+    chosen zip-up equals the FP64 one bit for bit while the FP32 kernels
+    ran."""
+    from qaintensor import Context
+    from qaintensor._lib import QTN_ERR_ARG, QtnError
+    n = 10
+    rng = np.random.default_rng(12)
+    va = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
+    vb = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
+    Bs = []
+    for j in range(n):
+        dl = 1 if j == 0 else 2
+        dr = 1 if j == n - 1 else 2
+        B = np.zeros((dl, 2, dr), dtype=np.complex128)
+        for a, v in enumerate((va, vb)):
+            B[0 if dl == 1 else a, :, 0 if dr == 1 else a] = v[j]
+        Bs.append(B)
+    eye = np.eye(2, dtype=np.complex128)
+    W = []
+    for j in range(n):
+        dl = 1 if j == 0 else 2
+        dr = 1 if j == n - 1 else 2
+        Wj = np.zeros((dl, 2, 2, dr), dtype=np.complex128)
+        for d in range(2):
+            Wj[0 if dl == 1 else d, :, :, 0 if dr == 1 else d] = eye
+        W.append(Wj)
+    ctx = Context(0)
+    try:
+        dev = upload(Bs, 2, ctx)
+        mpo = DeviceMPO(W, ctx)
+        out64 = DeviceMPS(n, 3, ctx)
+        nrm64 = dev.apply_mpo(mpo, 1e-10, 3, out64)
+        ctx.profile(True)
+        out32 = DeviceMPS(n, 3, ctx)
+        nrm32 = dev.apply_mpo(mpo, 1e-10, 3, out32, form=FORM_CHOSEN)
+        assert ctx.profile_read()["jac_inner_fp32"]["count"] > 0
+        ctx.profile(False)
+        assert nrm32 == nrm64
+        assert list(out32.bond_dims()) == list(out64.bond_dims())
+        assert max(out64.bond_dims()) == 2
+        for j in range(1, n + 1):
+            assert np.array_equal(out32.site(j), out64.site(j))
+        with pytest.raises(QtnError) as e:
+            dev.apply_mpo(mpo, 1e-10, 3, out32, form=7)
+        assert e.value.status == QTN_ERR_ARG
+    finally:
+        ctx.close()
+
+
 @pytest.mark.parametrize("op,m,n,k", [("N", 200, 130, 300), ("C", 257, 64, 129),
                                       ("N", 640, 512, 1024)])
 def test_zgemm_bf16x3_split_precision(gpu_ctx, op, m, n, k):
