@@ -231,8 +231,9 @@ int32_t qtn_mps_apply_layer(qtn_mps* mps, int32_t ngates,
                             int64_t* kept, double* disc_weight);
 
 /* A layer as qtn_mps_apply_layer in the learned form: gate i whose
- * predicted rank rank_hint[i] lies in (0, chimax) and whose sketch
- * (rank_hint[i] + 16 columns) is at most 3/4 of the full rank first takes the
+ * predicted rank rank_hint[i] is positive and whose sketch
+ * (rank_hint[i] + 16 columns) is narrower than chimax and at most 3/4 of
+ * the full rank first takes the
  * randomized range finder (2 power iterations, complex FP64); its exact
  * discarded weight, |theta - Q Q^H theta|_F^2 summed entry by entry plus
  * the dropped sketch values^2, must meet the cutoff, else the gate falls

@@ -1121,17 +1121,17 @@ int32_t apply_gates_core(qtn_ctx* ctx, int ng, qtn_mps* const* ms,
     return QTN_OK;
   };
   // The learned form is the chosen form plus the range finder: gates whose
-  // predicted rank stays below the cap try the sketch (sketch width
-  // hint + 16, well below the full rank), the other capped gates (and
-  // rejected sketches where the cap binds) take the FP32 route, and the
-  // rest the FP64 engine.
+  // sketch (width hint + 16) stays below the cap and well below the full
+  // rank try it; the other capped gates (and rejected sketches where the
+  // cap binds) take the FP32 route, and the rest the FP64 engine.  A
+  // sketch as wide as the cap would only repeat what the cap decides.
   std::vector<int> attempt(ng, 0);
   std::vector<int64_t> lw(ng, 0);
   bool any_attempt = false;
   if (learned && hint) {
     for (int g = 0; g < ng; ++g) {
       const int64_t l = std::min<int64_t>(hint[g] + 16, nsv[g]);
-      if (hint[g] > 0 && hint[g] < chis[g] && 4 * l <= 3 * nsv[g]) {
+      if (hint[g] > 0 && l < chis[g] && 4 * l <= 3 * nsv[g]) {
         attempt[g] = 1;
         lw[g] = l;
         any_attempt = true;
@@ -1159,9 +1159,10 @@ int32_t apply_gates_core(qtn_ctx* ctx, int ng, qtn_mps* const* ms,
     // The FP32 route pays only where the cap binds.  Without a prediction
     // that is taken to be where the bond sat at the cap before the gate or
     // theta is at least twice the cap (a cutoff-bound spectrum below the
-    // cap would send the FP32 result back to the FP64 engine); a predicted
-    // rank at the cap decides it in the learned form, and a rejected sketch
-    // (the prediction was low) falls back to the structural test.
+    // cap would send the FP32 result back to the FP64 engine).  In the
+    // learned form a predicted rank at the cap decides it, a prediction
+    // within the sketch margin (16) below the cap is too close to call and
+    // falls back to the structural test, as does a rejected sketch.
     std::vector<int> cand(ng, 0);
     bool any32 = false;
     for (int g = 0; g < ng; ++g) {
@@ -1171,7 +1172,7 @@ int32_t apply_gates_core(qtn_ctx* ctx, int ng, qtn_mps* const* ms,
       if (attempt[g])
         capped = !acc[g] && structural;
       else if (learned && hint && hint[g] > 0)
-        capped = hint[g] >= chis[g];
+        capped = hint[g] >= chis[g] || (hint[g] + 16 >= chis[g] && structural);
       else
         capped = structural;
       cand[g] = nsv[g] > chis[g] && capped ? 1 : 0;

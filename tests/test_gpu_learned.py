@@ -50,9 +50,11 @@ def test_range_finder_exact_check(gpu_ctx, hint, expect_accept):
     k_ref, _ = ref.apply_layer(sites, gates, cutoff, 256)
     hints = np.full(len(pairs), hint)
     k, _, acc = dev.apply_layer_learned(sites, gates, cutoff, 256, hints)
-    # gates whose sketch (hint + 16) is at most 3/4 of min(M, N)
+    # gates whose sketch (hint + 16) is narrower than the cap and at most
+    # 3/4 of min(M, N)
     big = [i for i, (w, _) in enumerate(pairs)
-           if 4 * (hint + 16) <= 3 * min(2 * d0[w - 1], 2 * d0[w + 1])]
+           if hint + 16 < 256
+           and 4 * (hint + 16) <= 3 * min(2 * d0[w - 1], 2 * d0[w + 1])]
     if expect_accept:
         assert len(big) > 0 and all(acc[i] == 1 for i in big)
         # exact low rank: the same kept ranks as the full SVD
