@@ -285,8 +285,12 @@ class Context:
             self._h = None
 
     def __del__(self):
+        # at interpreter exit contexts may be finalised before their
+        # objects: free those first, as close() does
         try:
             if self._h is not None and _lib is not None:
+                for child in list(getattr(self, "_children", ())):
+                    child.free()
                 _lib.qtn_ctx_destroy(self._h)
                 self._h = None
         except Exception:
