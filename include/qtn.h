@@ -76,8 +76,11 @@ int32_t qtn_ctx_sync(qtn_ctx* ctx);
 int32_t qtn_ctx_stream(qtn_ctx* ctx, void** stream);
 /* Work counters of the context since creation (or the last reset):
  * out[0] SVD problems solved, out[1] Jacobi sweeps summed over problems,
- * out[2] largest sweep count of one problem, out[3] batched SVD launches.
- * n = number of slots to write (<= 4).  reset != 0 zeroes them after.     */
+ * out[2] largest sweep count of one problem, out[3] batched SVD launches,
+ * out[4] chosen-form single splits whose FP32-seeded Rayleigh-Ritz result
+ * passed its FP64 check, out[5] chosen-form single splits redone by the
+ * FP64 engine.  n = number of slots to write (<= 6).  reset != 0 zeroes
+ * them after.                                                             */
 int32_t qtn_ctx_counters(qtn_ctx* ctx, int64_t* out, int32_t n,
                          int32_t reset);
 /* Per-kernel timing with HIP events on the context stream (off by default).
@@ -224,7 +227,14 @@ int32_t qtn_mps_apply_2q(qtn_mps* mps, int32_t site, const double* g,
                          int64_t* kept, double* disc_weight,
                          double* spectrum, int64_t* spectrum_len);
 /* A layer of gates on disjoint neighbouring pairs, batched in one pass:
- * sites[i], gates[16*i..] as in apply_2q.  kept/disc per gate.            */
+ * sites[i], gates[16*i..] as in apply_2q.  kept/disc per gate.
+ * Failure: arguments and gate entries (NaN / Inf) are checked before any
+ * gate is applied, so those errors leave the state untouched.  The gates
+ * run in independent size groups; an error only the sweeps can detect
+ * (QTN_ERR_NOCONV, QTN_ERR_TRUNC for a zero two-site tensor) is returned
+ * after the other groups may have applied their gates, so after such an
+ * error the state must be treated as undefined.  The same holds for
+ * qtn_mps_apply_layer_multi and the learned / predicted layer calls.     */
 int32_t qtn_mps_apply_layer(qtn_mps* mps, int32_t ngates,
                             const int32_t* sites, const double* gates,
                             double cutoff, int64_t chimax, int32_t form,

@@ -255,24 +255,30 @@ int32_t qtn_zgemm_bf16x3(qtn_ctx* ctx, char opA, int64_t m, int64_t n,
   if (m == 0 || n == 0) return QTN_OK;
   QTN_CHECK_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
+  // The host arrays are read and written only inside their rows x cols
+  // extents (a BLAS caller need not allocate past ld*(cols-1)+rows): they
+  // are staged by pitched copies into packed device buffers, and only the
+  // m rows of each column of C are copied back.
   const int64_t rowsA = op == 0 ? m : k, colsA = op == 0 ? k : m;
   void *dA, *dB, *dC;
-  QTN_TRY(ctx->buffer(192, (size_t)lda * colsA * sizeof(double2), &dA));
-  QTN_TRY(ctx->buffer(193, (size_t)ldb * n * sizeof(double2), &dB));
-  QTN_TRY(ctx->buffer(194, (size_t)ldc * n * sizeof(double2), &dC));
-  (void)rowsA;
-  QTN_CHECK_HIP(hipMemcpyAsync(dA, A, (size_t)lda * colsA * sizeof(double2),
-                               hipMemcpyHostToDevice, s));
-  QTN_CHECK_HIP(hipMemcpyAsync(dB, B, (size_t)ldb * n * sizeof(double2),
-                               hipMemcpyHostToDevice, s));
+  QTN_TRY(ctx->buffer(192, (size_t)rowsA * colsA * sizeof(double2), &dA));
+  QTN_TRY(ctx->buffer(193, (size_t)k * n * sizeof(double2), &dB));
+  QTN_TRY(ctx->buffer(194, (size_t)m * n * sizeof(double2), &dC));
   if (k == 0) {
-    QTN_CHECK_HIP(hipMemsetAsync(dC, 0, (size_t)ldc * n * sizeof(double2), s));
+    QTN_CHECK_HIP(hipMemsetAsync(dC, 0, (size_t)m * n * sizeof(double2), s));
   } else {
-    QTN_TRY(launch_zgemm_bf16x3(ctx, op, m, n, k, (const double2*)dA, lda,
-                                (const double2*)dB, ldb, (double2*)dC, ldc));
+    QTN_CHECK_HIP(hipMemcpy2DAsync(
+        dA, rowsA * sizeof(double2), A, lda * sizeof(double2),
+        rowsA * sizeof(double2), colsA, hipMemcpyHostToDevice, s));
+    QTN_CHECK_HIP(hipMemcpy2DAsync(dB, k * sizeof(double2), B,
+                                   ldb * sizeof(double2), k * sizeof(double2),
+                                   n, hipMemcpyHostToDevice, s));
+    QTN_TRY(launch_zgemm_bf16x3(ctx, op, m, n, k, (const double2*)dA, rowsA,
+                                (const double2*)dB, k, (double2*)dC, m));
   }
-  QTN_CHECK_HIP(hipMemcpyAsync(C, dC, (size_t)ldc * n * sizeof(double2),
-                               hipMemcpyDeviceToHost, s));
+  QTN_CHECK_HIP(hipMemcpy2DAsync(C, ldc * sizeof(double2), dC,
+                                 m * sizeof(double2), m * sizeof(double2), n,
+                                 hipMemcpyDeviceToHost, s));
   QTN_CHECK_HIP(hipStreamSynchronize(s));
   return QTN_OK;
 } catch (...) {

@@ -153,21 +153,26 @@ int32_t qtn_ctx_counters(qtn_ctx* ctx, int64_t* out, int32_t n,
   qtn::clear_error();
   QTN_ARG(ctx != nullptr && (n == 0 || out != nullptr),
           "qtn_ctx_counters: NULL argument");
-  QTN_ARG(n >= 0 && n <= 4, "qtn_ctx_counters: n must be 0..4");
+  QTN_ARG(n >= 0 && n <= 6, "qtn_ctx_counters: n must be 0..6");
   // the context's own work plus its workers' (concurrent layer groups)
-  int64_t v[4] = {ctx->n_svd, ctx->n_sweeps, ctx->max_sweeps,
-                  ctx->n_batches};
+  int64_t v[6] = {ctx->n_svd, ctx->n_sweeps, ctx->max_sweeps,
+                  ctx->n_batches, ctx->n_rr_accepted, ctx->n_rr_redone};
   for (const qtn_ctx* w : ctx->workers) {
     v[0] += w->n_svd;
     v[1] += w->n_sweeps;
     v[2] = std::max(v[2], w->max_sweeps);
     v[3] += w->n_batches;
+    v[4] += w->n_rr_accepted;
+    v[5] += w->n_rr_redone;
   }
   for (int i = 0; i < n; ++i) out[i] = v[i];
   if (reset) {
     ctx->n_svd = ctx->n_sweeps = ctx->max_sweeps = ctx->n_batches = 0;
-    for (qtn_ctx* w : ctx->workers)
+    ctx->n_rr_accepted = ctx->n_rr_redone = 0;
+    for (qtn_ctx* w : ctx->workers) {
       w->n_svd = w->n_sweeps = w->max_sweeps = w->n_batches = 0;
+      w->n_rr_accepted = w->n_rr_redone = 0;
+    }
   }
   return QTN_OK;
 } catch (...) {

@@ -12,6 +12,7 @@
 // MFMA operand fetch (lane l reads row l%16 of k-slice l/16) is a unit-stride
 // LDS read.  op = N / T / C is resolved while staging, so transposed and
 // conjugated operands cost nothing extra.
+#include <algorithm>
 #include "qtn_internal.hpp"
 
 namespace qtn {
@@ -407,7 +408,10 @@ int32_t qtn_zgemm_dev(qtn_ctx* ctx, char opA, char opB, int64_t m, int64_t n,
   QTN_ARG(oa >= 0 && ob >= 0, "qtn_zgemm_dev: op must be 'N', 'T' or 'C'");
   QTN_ARG(m >= 0 && n >= 0 && k >= 0 && batch >= 0,
           "qtn_zgemm_dev: negative dimension");
-  QTN_ARG(lda >= 1 && ldb >= 1 && ldc >= 1 && ldc >= m,
+  // as BLAS: A is stored m x k (op 'N') or k x m, B k x n or n x k
+  QTN_ARG(lda >= std::max<int64_t>(1, oa == 0 ? m : k) &&
+              ldb >= std::max<int64_t>(1, ob == 0 ? k : n) &&
+              ldc >= std::max<int64_t>(1, m),
           "qtn_zgemm_dev: bad leading dimension");
   QTN_CHECK_HIP(hipSetDevice(ctx->device));
   return qtn::launch_zgemm(ctx->stream, oa, ob, m, n, k,
@@ -431,7 +435,10 @@ int32_t qtn_cgemm_dev(qtn_ctx* ctx, char opA, char opB, int64_t m, int64_t n,
   QTN_ARG(oa >= 0 && ob >= 0, "qtn_cgemm_dev: op must be 'N', 'T' or 'C'");
   QTN_ARG(m >= 0 && n >= 0 && k >= 0 && batch >= 0,
           "qtn_cgemm_dev: negative dimension");
-  QTN_ARG(lda >= 1 && ldb >= 1 && ldc >= 1 && ldc >= m,
+  // as BLAS: A is stored m x k (op 'N') or k x m, B k x n or n x k
+  QTN_ARG(lda >= std::max<int64_t>(1, oa == 0 ? m : k) &&
+              ldb >= std::max<int64_t>(1, ob == 0 ? k : n) &&
+              ldc >= std::max<int64_t>(1, m),
           "qtn_cgemm_dev: bad leading dimension");
   QTN_CHECK_HIP(hipSetDevice(ctx->device));
   return qtn::launch_cgemm(ctx->stream, oa, ob, m, n, k,

@@ -124,6 +124,9 @@ struct qtn_ctx {
   std::vector<qtn::DevBuf> bufs;
   // work counters (see qtn_ctx_counters)
   int64_t n_svd = 0, n_sweeps = 0, max_sweeps = 0, n_batches = 0;
+  // chosen-form single splits (svd_device_chosen): Rayleigh-Ritz splits
+  // accepted, and splits redone by the FP64 engine
+  int64_t n_rr_accepted = 0, n_rr_redone = 0;
   // per-kernel event timing (qtn_ctx_profile)
   // ids: 0 jac_inner, 1 jac_gram, 2 jac_update, 3 jac_round (FP64 sweeps),
   // 4 jac_inner, 5 jac_gram, 6 jac_update (FP32 sweeps of the chosen form)

@@ -1095,6 +1095,9 @@ int32_t apply_gates_core(qtn_ctx* ctx, int ng, qtn_mps* const* ms,
   // (min(M, N) > chimax) and keeps that result when the cap binds inside
   // the spectrum FP32 resolves (kept == chimax with every kept value above
   // the FP32 noise floor); every other gate goes to the FP64 engine.
+  // (FP32 on every gate measured no gain: on Haar brick-walls the uncapped
+  // two-site spectra of the middle layers carry values between the cutoff
+  // and the FP32 floor, about 2e-5 |X|_F, which FP32 cannot decide.)
   JacobiEngine eng, eng32;
   eng32.slot_base = kSlotEng32;
   std::vector<int> wantw(ng);
@@ -1437,6 +1440,15 @@ int32_t apply_gates_grouped(qtn_ctx* ctx, int32_t ngates,
     const int64_t M = 2 * m->dims[i - 1], N = 2 * m->dims[i + 1];
     bucket[round_up64(std::min(M, N))].push_back(g);
   }
+  // A non-finite gate is refused before any group runs, so such a layer
+  // leaves every state untouched.  (Errors that only the sweeps can find --
+  // no convergence, a zero two-site tensor -- surface after the other
+  // groups have written their sites; the header documents this.)
+  for (int64_t e = 0; e < (int64_t)ngates * 32; ++e)
+    if (!std::isfinite(gates[e])) {
+      set_error("apply_layer: gate contains NaN or Inf");
+      return QTN_ERR_NOTFINITE;
+    }
   // Groups and their cost (gates x columns^2, the Jacobi work); the
   // costliest is halved while it exceeds a quarter of the total and keeps at
   // least kMinChunk gates per half, so one large size class does not set

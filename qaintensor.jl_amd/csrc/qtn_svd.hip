@@ -192,19 +192,135 @@ __global__ __launch_bounds__(256) void jac_gram(
     }
 }
 
-// Check and n_inner cyclic sweeps on the Hermitian Gram block G (LDS),
-// accumulating the rotations in Q (the identity on entry).  Returns the
-// number of rotations (0 when the block already meets the tolerance).
-// Called by all 256 threads; any_off / nrot are zeroed __shared__ ints
-// (visible after a barrier) and offmax_bz the problem's sweep maximum.
+// Rotation of the pair (p, q) from the Gram entries a = G_pp, d = G_qq and
+// g = G_pq (Golub & Van Loan 8.5, complex Hermitian form):
+//   J = [[cs, sn], [-sn e, cs e]], e = e^{-i phi} = conj(g) / |g|,
+// J^H [[a, g], [conj g, d]] J = diag(ap, dq).  With zeta = (d - a) / (2|g|)
+// and tau = |zeta| + sqrt(1 + zeta^2) (so t = tan = sign(zeta) / tau):
+// cs = tau / sqrt(1 + tau^2), sn = sign(zeta) / sqrt(1 + tau^2).  Computed
+// without branches; a pair that already meets the tolerance (or has a
+// column below the zero threshold) gets cs = 1, sn = 0, e = 1, which
+// leaves every entry it touches unchanged bit for bit.
+template <typename R>
+struct Rot {
+  R cs, sn, er, ei, ap, dq;
+  bool on;
+};
+
+template <typename R>
+__device__ __forceinline__ Rot<R> make_rot(R a, R d,
+                                           typename JT<R>::c_t g, R zt,
+                                           R tol2) {
+  Rot<R> o;
+  const R absc2 = g.x * g.x + g.y * g.y;
+  o.on = (a > zt && d > zt && absc2 > tol2 * a * d);
+  const R s2 = o.on ? absc2 : (R)1;
+  const R inv = JT<R>::rsq(s2);
+  const R absc = s2 * inv;
+  const R zeta = (R)0.5 * (d - a) * inv;
+  const R tau = JT<R>::ab(zeta) + JT<R>::sq((R)1 + zeta * zeta);
+  const R rr = JT<R>::rsq((R)1 + tau * tau);
+  const R sgn = zeta >= (R)0 ? (R)1 : (R)-1;
+  const R tt = JT<R>::dv(sgn, tau);
+  o.cs = o.on ? tau * rr : (R)1;
+  o.sn = o.on ? sgn * rr : (R)0;
+  o.er = o.on ? g.x * inv : (R)1;
+  o.ei = o.on ? -g.y * inv : (R)0;
+  o.ap = o.on ? a - tt * absc : a;
+  o.dq = o.on ? d + tt * absc : d;
+  return o;
+}
+
+// [u v] <- [u v] J (a row of two entries in the columns p, q of the pair)
+template <typename R>
+__device__ __forceinline__ void rot_right(const Rot<R>& j,
+                                          typename JT<R>::c_t& u,
+                                          typename JT<R>::c_t& v) {
+  const R yr = j.er * v.x - j.ei * v.y;
+  const R yi = j.er * v.y + j.ei * v.x;
+  const typename JT<R>::c_t nu =
+      JT<R>::mk(j.cs * u.x - j.sn * yr, j.cs * u.y - j.sn * yi);
+  v = JT<R>::mk(j.sn * u.x + j.cs * yr, j.sn * u.y + j.cs * yi);
+  u = nu;
+}
+
+// [u; v] <- J^H [u; v], J^H = [[cs, -sn conj(e)], [sn, cs conj(e)]]
+template <typename R>
+__device__ __forceinline__ void rot_left(const Rot<R>& j,
+                                         typename JT<R>::c_t& u,
+                                         typename JT<R>::c_t& v) {
+  const R yr = j.er * v.x + j.ei * v.y;
+  const R yi = j.er * v.y - j.ei * v.x;
+  const typename JT<R>::c_t nu =
+      JT<R>::mk(j.cs * u.x - j.sn * yr, j.cs * u.y - j.sn * yi);
+  v = JT<R>::mk(j.sn * u.x + j.cs * yr, j.sn * u.y + j.cs * yi);
+  u = nu;
+}
+
+// Round-robin slots of the inner sweep.  The W indices of the block pair
+// sit in NP = W/2 pairs of slots (pair P: side 0 and side 1); a round
+// rotates every pair at once, then every index moves one step along the
+// ring side0(1) -> side0(2) -> ... -> side0(NP-1) -> side1(NP-1) -> ... ->
+// side1(0) -> side0(1) while side0(0) stays (the circle method: all
+// W(W-1)/2 index pairs meet once in W-1 rounds, after which every index is
+// back in its first slot).  Round 0 holds index P at side0(P) and W-1-P at
+// side1(P).
+template <int NP>
+__device__ __forceinline__ int slot_index0(int P, int side) {
+  return side ? 2 * NP - 1 - P : P;
+}
+// slot (P, side) -> slot of the next round, as P' * 2 + side'
+template <int NP>
+__device__ __forceinline__ int slot_next(int P, int side) {
+  if (side == 0) {
+    if (P == 0) return 0;
+    if (P == NP - 1) return 2 * (NP - 1) + 1;
+    return 2 * (P + 1);
+  }
+  if (P == 0) return 2 * 1;
+  return 2 * (P - 1) + 1;
+}
+
+// Check and n_inner cyclic sweeps on the Hermitian Gram block G (LDS,
+// row-major), accumulating the rotations in Q (the identity on entry,
+// row-major on return).  Returns the number of rotations (0 when the block
+// already meets the tolerance).  Called by all 256 threads; any_off / nrot
+// are zeroed __shared__ ints (visible after a barrier) and offmax_bz the
+// problem's sweep maximum.
+//
+// A round rotates the NP = W/2 slot pairs at once, i.e. G <- J^H G J with J
+// block diagonal over the pairs, so the (pair P, pair P') 2x2 block of G
+// becomes J_P^H G_PP' J_P': it depends on that block and the two pairs'
+// rotations only.  Thread t owns the blocks (P, P') with P' = t mod NP and
+// P = t div NP + k 256/NP, and rows 2P, 2P+1 of Q in the columns of pair P'.
+// The sweep keeps G and Q as four planes (side x side) in block order, so a
+// thread's own block is read at consecutive addresses across its wave and
+// the diagonal block of pair P' at stride NP + 1 (no LDS bank conflicts);
+// after the update each entry is written straight to the slot its index
+// moves to, i.e. where its owner of the next round reads it.  Per round a
+// thread reads its blocks, the diagonal block of its column pair (and of
+// its row pair in FP32), derives J_P' -- J_P comes from the lane of its wave
+// that derived it as its own J_P' (FP64, a shuffle) or is derived too
+// (FP32, a few one-instruction transcendentals) -- transforms in registers
+// and writes.  Gb / Qb (four planes each, 4 NP^2 entries) double-buffer the
+// planes: the round reads one buffer and writes the other, one barrier per
+// round; Gb == Gp (no second buffer): a barrier separates the reads from
+// the writes.  The planes of the first buffer live in G's and Q's own
+// storage (W (W+1) >= 4 NP^2 entries).  The diagonal block of a rotated
+// pair is set to diag(ap, dq) exactly.
 template <typename R, int W>
 __device__ __forceinline__ int inner_solve(
-    typename JT<R>::c_t (*G)[W + 1], typename JT<R>::c_t (*Q)[W + 1], R zt,
-    R tol2, int n_inner, double* offmax_bz, int& any_off, int& nrot) {
+    typename JT<R>::c_t (*G)[W + 1], typename JT<R>::c_t (*Q)[W + 1],
+    typename JT<R>::c_t* Gb, typename JT<R>::c_t* Qb, R zt, R tol2,
+    int n_inner, double* offmax_bz, int& any_off, int& nrot) {
   using c_t = typename JT<R>::c_t;
-  constexpr int NP = W / 2;       // rotation pairs per round
-  constexpr int TP = 256 / NP;    // threads per pair
-  constexpr int RPT = W / TP;     // rows (columns) per thread
+  constexpr int NP = W / 2;             // rotation pairs per round
+  constexpr int NBLK = NP * NP;         // 2x2 blocks
+  constexpr int NBT = NBLK / 256;       // blocks per thread
+  constexpr int RS = 256 / NP;          // row-pair stride between them
+  constexpr bool kShare = sizeof(R) == sizeof(double);
+  static_assert(NBT >= 1 && NBT * 256 == NBLK, "W must be 32 or 64");
+  static_assert(W * (W + 1) >= 4 * NBLK, "planes must fit in G's storage");
   const int tid = threadIdx.x;
   // Largest scaled off-diagonal |G_pq|^2 / (G_pp G_qq) of the block; nothing
   // to do when it meets the tolerance (the common case in late sweeps).
@@ -245,95 +361,125 @@ __device__ __forceinline__ int inner_solve(
   }
   __syncthreads();
   if (!any_off) return 0;
-  const int pr = tid / TP;  // rotation pair of this thread
-  const int t = tid % TP;
+
+  const int lane = tid & 63;
+  const int pc_i = tid % NP;
+  const int pr0 = tid / NP;
+  c_t* Gp = &G[0][0];  // planes, first buffer
+  c_t* Qp = &Q[0][0];
+  const bool dbuf = Gb != Gp;
+  // row-major G -> planes (round-0 slots); Q planes = the identity
+  {
+    c_t v[NBT][4];
+#pragma unroll
+    for (int k = 0; k < NBT; ++k) {
+      const int P = pr0 + RS * k;
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        v[k][s] = G[slot_index0<NP>(P, s >> 1)][slot_index0<NP>(pc_i, s & 1)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NBT; ++k) {
+      const int P = pr0 + RS * k;
+      const int blk = P * NP + pc_i;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        Gp[s * NBLK + blk] = v[k][s];
+        const bool one = 2 * P + (s >> 1) == slot_index0<NP>(pc_i, s & 1);
+        Qp[s * NBLK + blk] = JT<R>::mk(one ? (R)1 : (R)0, (R)0);
+      }
+    }
+    __syncthreads();
+  }
+  // next-round slots of this thread's column sides and row sides (fixed)
+  int dc[2], dr[NBT][2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    dc[s] = slot_next<NP>(pc_i, s);
+#pragma unroll
+    for (int k = 0; k < NBT; ++k) dr[k][s] = slot_next<NP>(pr0 + RS * k, s);
+  }
+  c_t* Gc = Gp;
+  c_t* Gn = Gb;
+  c_t* Qc = Qp;
+  c_t* Qn = Qb;
   for (int sweep = 0; sweep < n_inner; ++sweep) {
     int sweep_rot = 0;
     for (int round = 0; round < W - 1; ++round) {
-      const int c1 = rr_block(pr, round, W);
-      const int c2 = rr_block(W - 1 - pr, round, W);
-      const int pp = min(c1, c2), qq = max(c1, c2);
-      const R a = G[pp][pp].x, d = G[qq][qq].x;
-      const c_t g = G[pp][qq];
-      const R absc2 = g.x * g.x + g.y * g.y;
-      const bool on = (a > zt && d > zt && absc2 > tol2 * a * d);
-      R cs = 1, sn = 0, er = 1, ei = 0, ap = a, dq = d;
-      if (on) {
-        const R inv = JT<R>::rsq(absc2);
-        const R absc = absc2 * inv;
-        const R zeta = (R)0.5 * (d - a) * inv;
-        const R tt = JT<R>::dv(zeta >= (R)0 ? (R)1 : (R)-1,
-                               JT<R>::ab(zeta) +
-                                   JT<R>::sq((R)1 + zeta * zeta));
-        cs = JT<R>::rsq((R)1 + tt * tt);
-        sn = tt * cs;
-        er = g.x * inv;  // e = e^{-i phi} = conj(g) / |g|
-        ei = -g.y * inv;
-        ap = a - tt * absc;
-        dq = d + tt * absc;
-        sweep_rot += (t == 0);
-        // column update: [x_p x_q] <- [x_p x_q] J,
-        // J = [[cs, sn], [-sn e, cs e]]
-        c_t gp[RPT], gq[RPT], qp[RPT], qv[RPT];
+      // --- every LDS read of the round
+      const int dcol = pc_i * (NP + 1);
+      const R ga = Gc[0 * NBLK + dcol].x, gd = Gc[3 * NBLK + dcol].x;
+      const c_t gg = Gc[1 * NBLK + dcol];
+      c_t b[NBT][4], q[NBT][4];
+      R ra[NBT], rd[NBT];
+      c_t rg[NBT];
 #pragma unroll
-        for (int j = 0; j < RPT; ++j) {
-          const int r = t + TP * j;
-          gp[j] = G[r][pp];
-          gq[j] = G[r][qq];
-          qp[j] = Q[r][pp];
-          qv[j] = Q[r][qq];
+      for (int k = 0; k < NBT; ++k) {
+        const int blk = (pr0 + RS * k) * NP + pc_i;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          b[k][s] = Gc[s * NBLK + blk];
+          q[k][s] = Qc[s * NBLK + blk];
         }
+        if (!kShare) {
+          const int drow = (pr0 + RS * k) * (NP + 1);
+          ra[k] = Gc[0 * NBLK + drow].x;
+          rd[k] = Gc[3 * NBLK + drow].x;
+          rg[k] = Gc[1 * NBLK + drow];
+        }
+      }
+      if (!dbuf) __syncthreads();  // every read precedes every write
+      // --- rotations
+      const Rot<R> jc = make_rot<R>(ga, gd, gg, zt, tol2);
+      Rot<R> jr[NBT];
 #pragma unroll
-        for (int j = 0; j < RPT; ++j) {
-          const int r = t + TP * j;
-          {
-            const R yr = er * gq[j].x - ei * gq[j].y;
-            const R yi = er * gq[j].y + ei * gq[j].x;
-            G[r][pp] = JT<R>::mk(cs * gp[j].x - sn * yr,
-                                 cs * gp[j].y - sn * yi);
-            G[r][qq] = JT<R>::mk(sn * gp[j].x + cs * yr,
-                                 sn * gp[j].y + cs * yi);
-          }
-          {
-            const R yr = er * qv[j].x - ei * qv[j].y;
-            const R yi = er * qv[j].y + ei * qv[j].x;
-            Q[r][pp] = JT<R>::mk(cs * qp[j].x - sn * yr,
-                                 cs * qp[j].y - sn * yi);
-            Q[r][qq] = JT<R>::mk(sn * qp[j].x + cs * yr,
-                                 sn * qp[j].y + cs * yi);
-          }
+      for (int k = 0; k < NBT; ++k) {
+        if (kShare) {
+          // the lane of this wave whose column pair is this row pair
+          const int src = (lane & ~(NP - 1)) | (pr0 + RS * k);
+          jr[k].cs = __shfl(jc.cs, src, 64);
+          jr[k].sn = __shfl(jc.sn, src, 64);
+          jr[k].er = __shfl(jc.er, src, 64);
+          jr[k].ei = __shfl(jc.ei, src, 64);
+        } else {
+          jr[k] = make_rot<R>(ra[k], rd[k], rg[k], zt, tol2);
+        }
+      }
+      // --- transform, then write each entry to its next-round slot
+#pragma unroll
+      for (int k = 0; k < NBT; ++k) {
+        // b[s]: s = 2 row side + column side
+        rot_right<R>(jc, b[k][0], b[k][1]);
+        rot_right<R>(jc, b[k][2], b[k][3]);
+        rot_left<R>(jr[k], b[k][0], b[k][2]);
+        rot_left<R>(jr[k], b[k][1], b[k][3]);
+        if (pr0 + RS * k == pc_i && jc.on) {
+          b[k][0] = JT<R>::mk(jc.ap, (R)0);
+          b[k][1] = JT<R>::mk((R)0, (R)0);
+          b[k][2] = JT<R>::mk((R)0, (R)0);
+          b[k][3] = JT<R>::mk(jc.dq, (R)0);
+          ++sweep_rot;
+        }
+        rot_right<R>(jc, q[k][0], q[k][1]);
+        rot_right<R>(jc, q[k][2], q[k][3]);
+        const int P = pr0 + RS * k;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int rs = dr[k][s >> 1], cs = dc[s & 1];
+          Gn[((rs & 1) * 2 + (cs & 1)) * NBLK + (rs >> 1) * NP + (cs >> 1)] =
+              b[k][s];
+          // Q rows stay with the thread; only the column moves
+          Qn[((s >> 1) * 2 + (cs & 1)) * NBLK + P * NP + (cs >> 1)] = q[k][s];
         }
       }
       __syncthreads();
-      if (on) {
-        // row update: [r_p; r_q] <- J^H [r_p; r_q],
-        // J^H = [[cs, -sn conj(e)], [sn, cs conj(e)]]
-        c_t xp[RPT], xq[RPT];
-#pragma unroll
-        for (int j = 0; j < RPT; ++j) {
-          xp[j] = G[pp][t + TP * j];
-          xq[j] = G[qq][t + TP * j];
-        }
-#pragma unroll
-        for (int j = 0; j < RPT; ++j) {
-          const int c = t + TP * j;
-          const R yr = er * xq[j].x + ei * xq[j].y;
-          const R yi = er * xq[j].y - ei * xq[j].x;
-          c_t np = JT<R>::mk(cs * xp[j].x - sn * yr, cs * xp[j].y - sn * yi);
-          c_t nq = JT<R>::mk(sn * xp[j].x + cs * yr, sn * xp[j].y + cs * yi);
-          // the 2x2 block becomes diag(ap, dq) exactly
-          if (c == pp) {
-            np = JT<R>::mk(ap, (R)0);
-            nq = JT<R>::mk((R)0, (R)0);
-          } else if (c == qq) {
-            np = JT<R>::mk((R)0, (R)0);
-            nq = JT<R>::mk(dq, (R)0);
-          }
-          G[pp][c] = np;
-          G[qq][c] = nq;
-        }
-      }
-      __syncthreads();
+      c_t* t = Gc;
+      Gc = Gn;
+      Gn = t;
+      t = Qc;
+      Qc = Qn;
+      Qn = t;
     }
     // rotations of this sweep (one count per pair per round)
     if (sweep_rot) atomicAdd(&nrot, sweep_rot);
@@ -345,6 +491,24 @@ __device__ __forceinline__ int inner_solve(
     __syncthreads();
     if (tid == 0) any_off += total;  // running total (started at 1)
     __syncthreads();
+  }
+  // Q planes (every index back in its round-0 slot) -> row-major Q
+  {
+    c_t v[NBT][4];
+#pragma unroll
+    for (int k = 0; k < NBT; ++k) {
+      const int blk = (pr0 + RS * k) * NP + pc_i;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) v[k][s] = Qc[s * NBLK + blk];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NBT; ++k) {
+      const int P = pr0 + RS * k;
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        Q[2 * P + (s >> 1)][slot_index0<NP>(pc_i, s & 1)] = v[k][s];
+    }
   }
   __syncthreads();
   return any_off - 1;
@@ -372,8 +536,15 @@ __global__ __launch_bounds__(256) void jac_inner(
   const int bz = blockIdx.y;
   const int pi = blockIdx.x;
   if (!active[bz]) return;
+  // The inner sweeps double-buffer their G and Q planes where the second
+  // buffers (2 x W^2 entries) keep the kernel within 72 KiB of LDS (W =
+  // 32; not W = 64)
+  constexpr bool kDB =
+      (2 * W * (W + 1) + 2 * W * W) * sizeof(c_t) <= 72 * 1024;
   __shared__ c_t G[W][W + 1];  // G[row][col]
   __shared__ c_t Q[W][W + 1];  // Q[row][col]
+  __shared__ c_t G2[kDB ? W * W : 1];
+  __shared__ c_t Q2[kDB ? W * W : 1];
   __shared__ int any_off;
   __shared__ int nrot;
   const int tid = threadIdx.x;
@@ -433,8 +604,9 @@ __global__ __launch_bounds__(256) void jac_inner(
     G[r][c] = JT<R>::mk(u.x, -u.y);
   }
   __syncthreads();
-  const int total = inner_solve<R, W>(G, Q, zt, tol2, n_inner, &offmax[bz],
-                                      any_off, nrot);
+  const int total = inner_solve<R, W>(
+      G, Q, kDB ? G2 : &G[0][0], kDB ? Q2 : &Q[0][0], zt, tol2, n_inner,
+      &offmax[bz], any_off, nrot);
   if (total == 0) {
     if (tid == 0) flags[bz * p + pi] = 0;
     return;
@@ -764,8 +936,8 @@ __global__ __launch_bounds__(256) void jac_round(
   // 2. inner sweeps
   const R zt = (R)zthr[bz];
   const R tol2 = (R)(tol * tol);
-  const int total = inner_solve<R, W>(G, Q, zt, tol2, n_inner, &offmax[bz],
-                                      any_off, nrot);
+  const int total = inner_solve<R, W>(G, Q, &G[0][0], &Q[0][0], zt, tol2,
+                                      n_inner, &offmax[bz], any_off, nrot);
   if (total == 0) return;
   if (tid == 0) {
     atomicAdd(&counters[bz], total);

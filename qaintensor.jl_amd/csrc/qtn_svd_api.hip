@@ -124,6 +124,25 @@ __global__ void ns_step(double2* __restrict__ Q, const double2* __restrict__ T,
   }
 }
 
+// Partial sums of |x|^2 over a column-major rows x cols block (ld ld):
+// block b writes part[b]; the host adds the parts in order.
+constexpr int kSumsqBlocks = 512;
+__global__ __launch_bounds__(256) void sumsq_partial(
+    const double2* __restrict__ X, int64_t rows, int64_t cols, int64_t ld,
+    double* __restrict__ part) {
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < rows * cols;
+       i += (int64_t)gridDim.x * 256) {
+    const double2 v = X[(i % rows) + (i / rows) * ld];
+    acc += v.x * v.x + v.y * v.y;
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  __shared__ double w[4];
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (w[0] + w[1]) + (w[2] + w[3]);
+}
+
 unsigned grid_for(int64_t n) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256,
                                                           8192));
@@ -228,9 +247,17 @@ int32_t svd_device(qtn_ctx* ctx, const double2* dA, int64_t m, int64_t n,
 // chimax + 64 singular vectors (U when m < n, V otherwise) are taken to
 // FP64 and re-orthonormalised by three Newton-Schulz steps,
 // Q <- Q (3 I - Q^H Q) / 2, and A's projection on span(Q) is split by the
-// FP64 engine (Rayleigh-Ritz), whose factors give U, S and V^H.  If FP32
-// or the projection keeps another rank, the FP64 svd_device runs on A.
-// Outputs as svd_device (dU and dVh required).  Uses slots 200-211 and
+// FP64 engine (Rayleigh-Ritz), whose factors give U, S and V^H.
+// A-posteriori check, in FP64: the weight of A outside span(Q),
+// w_out = |A - Q Q^H A|_F^2 (the residual formed explicitly and summed
+// entry by entry), must not exceed the weight db the split discards inside
+// span(Q) (the 64 guard directions past the cap).  When it does, the
+// spectrum does not decay past the cap (a flat, Marchenko-Pastur-like
+// spectrum): the kept subspace is then not resolved by the guard
+// directions, and the split is redone by the FP64 svd_device.  The same
+// happens if FP32 or the projection keeps another rank.  disc_out is the
+// FP64 discarded weight of the returned split, w_out + db.
+// Outputs as svd_device (dU and dVh required).  Uses slots 200-213 and
 // svd_device's.
 int32_t svd_device_chosen(qtn_ctx* ctx, const double2* dA, int64_t m,
                           int64_t n, int64_t lda, double cutoff, int relative,
@@ -271,9 +298,11 @@ int32_t svd_device_chosen(qtn_ctx* ctx, const double2* dA, int64_t m,
   QTN_CHECK_HIP(hipMemcpyAsync(&disc, eng.disc, sizeof(double),
                                hipMemcpyDeviceToHost, s));
   QTN_CHECK_HIP(hipStreamSynchronize(s));
-  if (st != 0 || k != chimax)
+  if (st != 0 || k != chimax) {
+    ctx->n_rr_redone += 1;
     return svd_device(ctx, dA, m, n, lda, true, cutoff, relative, chimax,
                       absorb, dU, dS, dVh, k_out, disc_out);
+  }
   // Q: the leading kext = k + 64 vectors from W (U when trans, V
   // otherwise), FP64, re-orthonormalised by Newton-Schulz steps.
   const int64_t kext = std::min(nsv, k + 64);
@@ -322,9 +351,41 @@ int32_t svd_device_chosen(qtn_ctx* ctx, const double2* dA, int64_t m,
     QTN_TRY(svd_device(ctx, B, m, kext, m, true, cutoff, relative, chimax,
                        absorb, dU, dS, F, &kb, &db));
   }
-  if (kb != chimax)
+  if (kb != chimax) {
+    ctx->n_rr_redone += 1;
     return svd_device(ctx, dA, m, n, lda, true, cutoff, relative, chimax,
                       absorb, dU, dS, dVh, k_out, disc_out);
+  }
+  // residual R = A - Q B (trans) or A - B Q^H, and its squared norm
+  void *dR, *dP;
+  QTN_TRY(ctx->buffer(212, (size_t)m * n * sizeof(double2), &dR));
+  QTN_TRY(ctx->buffer(213, kSumsqBlocks * sizeof(double), &dP));
+  double2* Rm = (double2*)dR;
+  QTN_CHECK_HIP(hipMemcpy2DAsync(Rm, m * sizeof(double2), dA,
+                                 lda * sizeof(double2), m * sizeof(double2),
+                                 n, hipMemcpyDeviceToDevice, s));
+  const double2 mone = make_double2(-1.0, 0.0);
+  if (trans)
+    QTN_TRY(launch_zgemm(s, 0, 0, m, n, kext, mone, Q, m, 0, B, kext, 0, one,
+                         Rm, m, 0, 1));
+  else
+    QTN_TRY(launch_zgemm(s, 0, 2, m, n, kext, mone, B, m, 0, Q, n, 0, one,
+                         Rm, m, 0, 1));
+  hipLaunchKernelGGL(sumsq_partial, dim3(kSumsqBlocks), dim3(256), 0, s,
+                     (const double2*)Rm, m, n, m, (double*)dP);
+  QTN_CHECK_HIP(hipGetLastError());
+  std::vector<double> part(kSumsqBlocks);
+  QTN_CHECK_HIP(hipMemcpyAsync(part.data(), dP, kSumsqBlocks * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+  QTN_CHECK_HIP(hipStreamSynchronize(s));
+  double w_out = 0.0;
+  for (double v : part) w_out += v;
+  if (!(w_out <= db)) {
+    ctx->n_rr_redone += 1;
+    return svd_device(ctx, dA, m, n, lda, true, cutoff, relative, chimax,
+                      absorb, dU, dS, dVh, k_out, disc_out);
+  }
+  ctx->n_rr_accepted += 1;
   if (trans)
     QTN_TRY(launch_zgemm(s, 0, 0, m, kb, kext, one, Q, m, 0, F, kext, 0, zero,
                          dU, m, 0, 1));
@@ -332,7 +393,8 @@ int32_t svd_device_chosen(qtn_ctx* ctx, const double2* dA, int64_t m,
     QTN_TRY(launch_zgemm(s, 0, 2, kb, n, kext, one, F, kb, 0, Q, n, 0, zero,
                          dVh, kb, 0, 1));
   if (k_out) *k_out = kb;
-  if (disc_out) *disc_out = disc;
+  if (disc_out) *disc_out = w_out + db;
+  (void)disc;
   return QTN_OK;
 }
 

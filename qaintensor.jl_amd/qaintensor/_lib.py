@@ -249,11 +249,12 @@ class Context:
 
     def counters(self, reset=False):
         """dict of the context's work counters (qtn_ctx_counters)."""
-        out = np.zeros(4, dtype=np.int64)
-        check(load().qtn_ctx_counters(self.handle, i64ptr(out), 4,
+        out = np.zeros(6, dtype=np.int64)
+        check(load().qtn_ctx_counters(self.handle, i64ptr(out), 6,
                                       int(bool(reset))))
         return dict(svd=int(out[0]), sweeps=int(out[1]),
-                    max_sweeps=int(out[2]), batches=int(out[3]))
+                    max_sweeps=int(out[2]), batches=int(out[3]),
+                    rr_accepted=int(out[4]), rr_redone=int(out[5]))
 
     PROFILED_KERNELS = ("jac_inner", "jac_gram", "jac_update", "jac_round",
                         "jac_inner_fp32", "jac_gram_fp32",
@@ -298,13 +299,16 @@ class Context:
 
 
 _default_ctx = None
+# held while the default context is created (Context() takes _lock itself
+# through load(), so this is a separate lock)
+_default_lock = threading.Lock()
 
 
 def default_context():
-    """Process-wide context on device 0 (created on first use)."""
+    """Process-wide context on device 0 (created on first use; two threads
+    asking at once get the same one)."""
     global _default_ctx
-    with _lock:
-        pass
-    if _default_ctx is None:
-        _default_ctx = Context(0)
-    return _default_ctx
+    with _default_lock:
+        if _default_ctx is None:
+            _default_ctx = Context(0)
+        return _default_ctx

@@ -98,6 +98,32 @@ def test_gemm_dev_rejects_unknown_op(gpu_ctx):
     assert st == L.QTN_ERR_ARG
 
 
+@pytest.mark.parametrize("oa,ob,lda,ldb,ldc", [
+    ("N", "N", 69, 33, 70),   # lda < m for op(A) = A (m x k stored)
+    ("T", "N", 32, 33, 70),   # lda < k for op(A) = A^T (k x m stored)
+    ("N", "N", 70, 32, 70),   # ldb < k for op(B) = B (k x n stored)
+    ("N", "C", 70, 44, 70),   # ldb < n for op(B) = B^H (n x k stored)
+    ("N", "N", 70, 33, 69),   # ldc < m
+])
+def test_gemm_dev_rejects_short_leading_dimension(gpu_ctx, oa, ob, lda, ldb,
+                                                  ldc):
+    """BLAS's leading-dimension rule (m = 70, n = 45, k = 33): a stored
+    operand shorter than its rows is an argument error, not a read out of
+    bounds; nothing is launched (the device pointers are never touched)."""
+    one = np.array([1.0, 0.0])
+    for fn in ("qtn_zgemm_dev", "qtn_cgemm_dev"):
+        f = getattr(L.load(), fn)
+        if fn == "qtn_cgemm_dev":
+            o = np.array([1.0, 0.0], dtype=np.float32)
+            al = o.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        else:
+            al = L.dptr(one)
+        st = f(gpu_ctx.handle, oa.encode(), ob.encode(), 70, 45, 33, al,
+               None, lda, 0, None, ldb, 0, al, None, ldc, 0, 1)
+        assert st == L.QTN_ERR_ARG
+        assert "leading dimension" in L.load().qtn_last_error().decode()
+
+
 def test_permute_dev(gpu_ctx):
     rng = np.random.default_rng(3)
     dims = (3, 4, 5, 2)

@@ -92,6 +92,32 @@ def test_non_finite_gate_is_rejected(gpu_ctx):
     assert np.array_equal(dev.amplitudes(all_bits(4)), before)
 
 
+@pytest.mark.parametrize("form", [0, FORM_CHOSEN])
+def test_non_finite_gate_in_a_mixed_layer_leaves_state(gpu_ctx, form):
+    """A layer whose gates fall in two size groups (two-site matrices of
+    at most 64 and of 128 columns), one gate non-finite: refused with
+    QTN_ERR_NOTFINITE before any group runs, so no gate of the layer is
+    applied."""
+    rng = np.random.default_rng(6)
+    n = 14
+    dev = DeviceMPS(n, 64, gpu_ctx)
+    for layer in range(8):
+        ws = list(range(1 + layer % 2, n, 2))
+        dev.apply_layer(ws, [C.haar_unitary(rng, 4) for _ in ws], 1e-12, 64)
+    dims = list(dev.bond_dims())
+    assert dims[1] == 2 and dims[7] == 64
+    before = dev.amplitudes(all_bits(n))
+    ws = list(range(1, n, 2))
+    gates = [C.haar_unitary(rng, 4) for _ in ws]
+    gates[3] = gates[3].copy()
+    gates[3][0, 1] = np.inf
+    with pytest.raises(QtnError) as e:
+        dev.apply_layer(ws, gates, 1e-12, 64, form=form)
+    assert e.value.status == QTN_ERR_NOTFINITE
+    assert list(dev.bond_dims()) == dims
+    assert np.array_equal(dev.amplitudes(all_bits(n)), before)
+
+
 @pytest.mark.parametrize("site", [0, 4, -1])
 def test_site_out_of_range(gpu_ctx, site):
     dev = DeviceMPS(4, 4, gpu_ctx)

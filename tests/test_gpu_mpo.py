@@ -141,6 +141,79 @@ def test_apply_mpo_chosen_form_zipup(n, chi, cap):
         ctx.close()
 
 
+def _chosen_vs_fp64(ctx, dev, mpo, n, cap):
+    """FP64 and chosen-form zip-ups of dev; returns (out64, out32, counters
+    of the chosen run) after checking the chosen form's bound: same bond
+    dimensions, norm difference and infidelity <= 1e-6."""
+    out64 = DeviceMPS(n, cap, ctx)
+    nrm64 = dev.apply_mpo(mpo, 1e-8, cap, out64)
+    ctx.counters(reset=True)
+    out32 = DeviceMPS(n, cap, ctx)
+    nrm32 = dev.apply_mpo(mpo, 1e-8, cap, out32, form=FORM_CHOSEN)
+    cnt = ctx.counters(reset=True)
+    print("%d splits accepted, %d redone in FP64"
+          % (cnt["rr_accepted"], cnt["rr_redone"]))
+    assert list(out32.bond_dims()) == list(out64.bond_dims())
+    assert abs(nrm32 - nrm64) <= 1e-6 * nrm64
+    assert abs(1 - abs(out32.overlap(out64))) <= 1e-6
+    return out64, out32, cnt
+
+
+def test_apply_mpo_chosen_form_split_passes_its_check(gpu_ctx):
+    """The chosen zip-up's FP64 check of an FP32-seeded Rayleigh-Ritz split
+    (weight outside span(Q) of the cap + 64 vectors <= weight of the 64
+    guard vectors) passes where the spectrum falls off past the cap: TFIM
+    on a random 16-site state of bond 128, cap 192 (splits up to
+    384 x 384, square, so the smallest third of the values carries less
+    weight than the 64 past the cap)."""
+    n, chi, cap = 16, 128, 192
+    W = Hm.tfim_mpo(n, 1.0, Hm.tfim_fields(n, 4))
+    dev = upload(O.random_mps(n, chi, 31), chi, gpu_ctx)
+    out64, _, cnt = _chosen_vs_fp64(gpu_ctx, dev, DeviceMPO(W, gpu_ctx), n,
+                                    cap)
+    assert cnt["rr_accepted"] > 0 and cnt["rr_redone"] == 0
+    assert max(out64.bond_dims()) == cap
+
+
+def test_apply_mpo_chosen_form_flat_spectrum_is_redone_in_fp64(gpu_ctx):
+    """A split whose spectrum is flat past the cap fails the check and is
+    redone by the FP64 engine.  18 qubits, psi = sum_a L_a R_a / sqrt(384)
+    with orthonormal L_a = l_(a div 2) (x) |a mod 2> on qubits 1-9 (l: a
+    random 256 x 192 isometry, so bond 8 has rank 192) and R_a a random
+    512 x 384 isometry on qubits 10-18: 384 equal Schmidt values at bond
+    9.  The identity MPO with cap 192 splits that bond as a 384 x 384
+    matrix: 128 values outside span(Q) against 64 in the guard."""
+    rng = np.random.default_rng(17)
+
+    def isometry(r, c):
+        q, _ = np.linalg.qr(rng.standard_normal((r, c)) +
+                            1j * rng.standard_normal((r, c)))
+        return q
+
+    n, cap = 18, 192
+    lq, R = isometry(256, 192), isometry(512, 384)
+    Lm = np.zeros((512, 384), dtype=np.complex128)
+    for a in range(384):
+        Lm[256 * (a % 2):256 * (a % 2) + 256, a] = lq[:, a // 2]
+    psi = (Lm @ R.T).ravel(order="F") / np.sqrt(384.0)
+    # site tensors by an SVD sweep that drops the exactly-zero values
+    T, m, lb = [], psi.reshape((1, -1)), 1
+    for _ in range(n - 1):
+        m = m.reshape((2 * lb, -1), order="F")
+        U, S, Vh = np.linalg.svd(m, full_matrices=False)
+        r = int(np.sum(S > 1e-12 * S[0]))
+        T.append(U[:, :r].reshape((lb, 2, r), order="F"))
+        m, lb = S[:r, None] * Vh[:r], r
+    T.append(m.reshape((lb, 2, 1), order="F"))
+    assert [t.shape[2] for t in T][7:9] == [192, 384]
+    dev = upload(T, 384, gpu_ctx)
+    eye = [np.eye(2, dtype=np.complex128).reshape((1, 2, 2, 1))] * n
+    out64, _, cnt = _chosen_vs_fp64(gpu_ctx, dev, DeviceMPO(eye, gpu_ctx), n,
+                                    cap)
+    assert cnt["rr_redone"] >= 1
+    assert out64.bond_dims()[9] == cap
+
+
 def test_apply_mpo_chosen_form_falls_back_below_the_cap():
     """|a> + |b> (two random product states: rank 2 at every bond) under
     W = 2 I written with bond dimension 2 (two identical paths), cap 3: the
@@ -213,6 +286,40 @@ def test_zgemm_bf16x3_split_precision(gpu_ctx, op, m, n, k):
     rel = np.max(np.abs(C - ref) / bound)
     assert rel < 2e-6
     assert rel > 1e-12
+
+
+def test_zgemm_bf16x3_leading_dimensions(gpu_ctx):
+    """Leading dimensions larger than the rows: only the rows x cols
+    extents are read, and the padding rows of C keep the caller's values
+    (the entry point's host arrays are BLAS-style column-major views)."""
+    from qaintensor import _lib as L
+    rng = np.random.default_rng(5)
+    m, n, k = 70, 33, 90
+    lda, ldb, ldc = m + 7, k + 5, m + 11
+    Ab = rng.standard_normal((lda, k)) + 1j * rng.standard_normal((lda, k))
+    Bb = rng.standard_normal((ldb, n)) + 1j * rng.standard_normal((ldb, n))
+    Ab, Bb = np.asfortranarray(Ab), np.asfortranarray(Bb)
+    Cb = np.full((ldc, n), 3.0 - 2.0j, dtype=np.complex128, order="F")
+    L.check(L.load().qtn_zgemm_bf16x3(
+        gpu_ctx.handle, b"N", m, n, k, L.dptr(Ab), lda, L.dptr(Bb), ldb,
+        L.dptr(Cb), ldc))
+    ref = Ab[:m] @ Bb[:k]
+    bound = np.outer(np.linalg.norm(Ab[:m], axis=1),
+                     np.linalg.norm(Bb[:k], axis=0))
+    assert np.max(np.abs(Cb[:m] - ref) / bound) < 2e-6
+    assert np.all(Cb[m:] == 3.0 - 2.0j)
+    # op = 'C' with a padded A (k x m stored, lda > k)
+    Ac = np.asfortranarray(rng.standard_normal((k + 3, m)) +
+                           1j * rng.standard_normal((k + 3, m)))
+    Cc = np.full((ldc, n), -1.0, dtype=np.complex128, order="F")
+    L.check(L.load().qtn_zgemm_bf16x3(
+        gpu_ctx.handle, b"C", m, n, k, L.dptr(Ac), k + 3, L.dptr(Bb), ldb,
+        L.dptr(Cc), ldc))
+    refc = Ac[:k].conj().T @ Bb[:k]
+    boundc = np.outer(np.linalg.norm(Ac[:k], axis=0),
+                      np.linalg.norm(Bb[:k], axis=0))
+    assert np.max(np.abs(Cc[:m] - refc) / boundc) < 2e-6
+    assert np.all(Cc[m:] == -1.0)
 
 
 def test_expect_mpo_bf16x3_form(gpu_ctx):
