@@ -25,6 +25,9 @@ struct JacobiEngine {
   bool single = false; // FP32 sweeps (complex float X, Gram and rotations)
   int slot_base = kSlotGram;  // first of the 5 context slots it uses
   bool fused = false;  // W = 32 rounds as one jac_round launch
+  // the inner sweeps of outer rounds 1.. rotate only the cross pairs of
+  // their two column blocks (round 0 of every outer sweep rotates all)
+  bool cross_rounds = false;
   int64_t ldw = 0;     // leading dimension of W (0: cols_p)
   const int64_t* chimax_per = nullptr;  // host, per-problem cap (optional)
   int64_t* chimax_dev = nullptr;
@@ -54,7 +57,8 @@ struct JacobiEngine {
   double* sig = nullptr;  // [batch][cols_p], sorted descending after run
   qtn_ctx* ctx_ = nullptr;
   std::vector<hipEvent_t> prof_events;  // 6 per round while profiling
-  int32_t account_sweep(int n_active, int qpairs, int nchunk);
+  int32_t account_sweep(int n_active, int qpairs, int qpairs_w, int nchunk,
+                        int nchunk_w);
   int* perm = nullptr;    // [batch][cols_p], column of X for each sigma
 
   // n_active: problems that will actually be solved (the rest are masked;
