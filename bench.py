@@ -587,6 +587,19 @@ def main():
         dt = (time.perf_counter() - t0) / args.steps
         return dt, ctx.counters(), infidelity(ref, dev)
 
+    def kept_sv_errors():
+        """Kept (normalised) singular values of every bond of the last
+        timed state against the FP64 state: max relative error per value
+        and max absolute error (identical ranks required)."""
+        rel, ab = 0.0, 0.0
+        for b in range(1, N_QUBITS):
+            s, r = dev.bond_spectrum(b), ref.bond_spectrum(b)
+            if s.size != r.size:
+                return None, None
+            rel = max(rel, float(np.max(np.abs(s - r) / r)))
+            ab = max(ab, float(np.max(np.abs(s - r))))
+        return rel, ab
+
     def profiled(form):
         """One more circuit with per-dispatch kernel timing on (outside
         the timed region)."""
@@ -602,6 +615,7 @@ def main():
     # chosen form (FP32 sweeps, FP64 re-orthogonalisation): the headline
     # when it stays within 1e-6 infidelity of the FP64 state
     chosen_s, cnt32, inf_chosen = timed(FORM_CHOSEN)
+    sv_rel, sv_abs = kept_sv_errors()
     use_chosen = inf_chosen <= 1e-6
     prof = profiled(FORM_CHOSEN if use_chosen else FORM_PLAIN)
 
@@ -656,8 +670,11 @@ def main():
                        "infidelity_vs_fp64": inf_chosen,
                        "jacobi_sweeps_per_svd":
                            cnt32["sweeps"] / max(1, cnt32["svd"]),
+                       "max_rel_err_kept_sv": sv_rel,
+                       "max_abs_err_kept_sv": sv_abs,
                        "vs_plain": (None if plain_rate is None else
-                                    chosen_rate / plain_rate)},
+                                    chosen_rate / plain_rate),
+                       "vs_batched_fp64": chosen_rate / batched_rate},
         },
         "roofline": roofline(prof),
         "pipeline_roofline": {

@@ -226,6 +226,33 @@ int32_t qtn_mps_apply_2q(qtn_mps* mps, int32_t site, const double* g,
                          double cutoff, int64_t chimax, int32_t form,
                          int64_t* kept, double* disc_weight,
                          double* spectrum, int64_t* spectrum_len);
+/* switch!(mps, i, j) (src/switch.jl:63-86) on the device state: wires i
+ * and j trade places.  As in the reference, wires are numbered from the
+ * most significant (wire w is site n - w + 1, src/switch.jl:69-70) and the
+ * exchange is a chain of adjacent swaps (src/switch.jl:18-56, each a
+ * two-site update with the SWAP gate, untruncated but for exact zeros, up
+ * to the state's capacity).  Errors (QTN_ERR_ARG) carry the reference's
+ * messages: "Wire indices `i` and `j` must be positive", "Indices to swap
+ * `i` and `j` must be less than or equal to the number of open wires in
+ * MPS".                                                                   */
+int32_t qtn_mps_switch(qtn_mps* mps, int32_t i, int32_t j);
+/* Base.permute!(mps, order) (src/switch.jl:94-106): order (len = n
+ * values, a permutation of 1..n) is realised by switch! calls as the
+ * reference makes them; its checks and messages are the reference's.     */
+int32_t qtn_mps_permute(qtn_mps* mps, const int64_t* order, int64_t len);
+/* A two-qubit gate on any two sites i != j (1-based, the numbering of
+ * qtn_mps_apply_2q): g is 4x4 column-major with index s_i + 2 s_j.  The
+ * qubit of the farther site travels next to the nearer one by SWAP
+ * updates, the gate is applied there and the qubit travels back; every
+ * step is a two-site update truncated with cutoff / chimax in `form`.
+ * kept / disc (may be NULL) are those of the gate's own update.  Replaces
+ * applying a gate on non-adjacent wires through extend_MPO's identity
+ * pipes and apply_MPO (src/mpo.jl:122-157, 184-252) for the device
+ * state, with the swap network of switch! (src/switch.jl:63-86).          */
+int32_t qtn_mps_apply_2q_long(qtn_mps* mps, int32_t i, int32_t j,
+                              const double* g, double cutoff, int64_t chimax,
+                              int32_t form, int64_t* kept,
+                              double* disc_weight);
 /* A layer of gates on disjoint neighbouring pairs, batched in one pass:
  * sites[i], gates[16*i..] as in apply_2q.  kept/disc per gate.
  * Failure: arguments and gate entries (NaN / Inf) are checked before any
@@ -286,6 +313,30 @@ int32_t qtn_mps_expect_z(const qtn_mps* mps, double* out);
 int32_t qtn_mps_overlap(const qtn_mps* a, const qtn_mps* b, double* out);
 /* <psi|psi> (real).                                                        */
 int32_t qtn_mps_norm2(const qtn_mps* mps, double* out);
+
+/* ---- whole networks (contract, src/contract.jl:242-264) ------------------ */
+/* Contraction of a whole network on the device: the work of
+ * TensorOperations.ncon(tensors, labels; order) that contract(net) calls
+ * (src/contract.jl:257, 263).  ntensors host arrays (column-major complex
+ * doubles, Julia's ComplexF64) with ranks ndims[t]; dims and labels hold
+ * every tensor's leg sizes and ncon labels back to back (positive: a
+ * contracted pair, in two tensors or twice in one for a trace; negative:
+ * an open leg).  order lists every positive label once: the first label
+ * not yet contracted names the next two tensors, contracted over all the
+ * labels they share (pass the labels in increasing order for ncon's
+ * default, or the sequence a planner such as contract_order,
+ * src/contract.jl:184-235, returns -- applied as planned).  Tensors left
+ * unconnected are joined by outer products.  out (out_len = the product of
+ * the open legs' sizes) receives the result with legs ordered -1, -2, ...
+ * Intermediates stay on the device; one copy in, one copy out.  Errors:
+ * "Dimensions of contraction legs do not match" (src/svd.jl:16) and
+ * QTN_ERR_ARG for labels or an order that do not describe a network.     */
+int32_t qtn_contract_network(qtn_ctx* ctx, int32_t ntensors,
+                             const int32_t* ndims, const int64_t* dims,
+                             const int32_t* labels,
+                             const double* const* tensors, int32_t norder,
+                             const int32_t* order, double* out,
+                             int64_t out_len);
 
 /* ---- matrix-product operators (apply_MPO, src/mpo.jl:184-252) ----------- */
 /* MPO of nsites tensors W_j (D_j, 2 out, 2 in, D_{j+1}), column-major host

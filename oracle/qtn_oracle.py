@@ -46,6 +46,13 @@ PINNING = {
     "TebdState": "untruncated runs equal the dense statevector simulator; "
                  "product-state closed forms",
     "StateVector": "closed forms: X/H/CNOT on basis states, unitarity",
+    "TebdState.switch": "the reference's own case (test/test_mps.jl:198-"
+                        "221): switch!(mps, [(2,3),(4,6),(4,5)]) on kron(b..)"
+                        " gives kron(b[[1,3,2,5,6,4]]..); wires i, j "
+                        "exchange bits n-i and n-j of the dense vector",
+    "TebdState.apply_2q_long": "untruncated: equals the dense vector with "
+                               "the gate applied on wires (i, j) by "
+                               "apply_mpo_dense, both orders of i, j",
     "tfim_mpo": "contracted to a dense matrix it equals the kron-sum TFIM "
                 "Hamiltonian (tfim_dense) for n <= 10",
     "expect_mpo": "equals psi^H H psi of the dense vector and matrix for "
@@ -299,6 +306,11 @@ def apply_mpo_dense(psi, m, iwire):
     return out.reshape(-1, order="F")
 
 
+# SWAP on index s_a + 2 s_b: |01> <-> |10>
+SWAP_GATE = np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0],
+                      [0, 0, 0, 1]], dtype=np.complex128)
+
+
 # ---------------------------------------------------------------------------
 # dense statevector (independent check of the MPS path)
 class StateVector:
@@ -312,6 +324,10 @@ class StateVector:
 
     def apply_2q(self, w, g):
         self.psi = apply_mpo_dense(self.psi, g, (w, w + 1))
+
+    def apply_2q_any(self, i, j, g):
+        """g (index s_i + 2 s_j) on wires i != j."""
+        self.psi = apply_mpo_dense(self.psi, g, (i, j))
 
     def amplitudes(self, bits):
         bits = np.asarray(bits, dtype=np.int64)
@@ -388,6 +404,43 @@ class TebdState:
         self.B[i - 1] = ((phi @ Vk) / nrm).reshape((chiL, 2, k), order="F")
         self.lam[i] = S[:k] / nrm
         return k, (disc / total if total > 0 else 0.0), S
+
+    def apply_2q_long(self, i, j, g, cutoff, chimax):
+        """g (4x4, index s_i + 2 s_j) on sites i != j by SWAP routing, the
+        device's rule (qtn_mps_apply_2q_long): the qubit of the farther site
+        travels to next to the nearer one by SWAP updates, the gate acts on
+        (lo, lo + 1) with its bits reordered when i > j, and the qubit
+        travels back; every update is apply_2q with cutoff / chimax.
+        Returns the gate update's (k, disc, spectrum)."""
+        lo, hi = min(i, j), max(i, j)
+        g = np.asarray(g, dtype=np.complex128)
+        if i > j:
+            P = SWAP_GATE
+            g = P @ g @ P
+        for k in range(hi - 1, lo, -1):
+            self.apply_2q(k, SWAP_GATE, cutoff, chimax)
+        out = self.apply_2q(lo, g, cutoff, chimax)
+        for k in range(lo + 1, hi):
+            self.apply_2q(k, SWAP_GATE, cutoff, chimax)
+        return out
+
+    def switch(self, i, j):
+        """switch!(mps, i, j) (src/switch.jl:63-86): wires numbered from the
+        most significant (site n - w + 1), adjacent swaps lo..hi-1 then
+        hi-2..lo (src/switch.jl:79-85), untruncated."""
+        if not (i > 0 and j > 0):
+            raise ValueError("Wire indices `i` and `j` must be positive")
+        if not (i <= self.n and j <= self.n):
+            raise ValueError("Indices to swap `i` and `j` must be less than "
+                             "or equal to the number of open wires in MPS")
+        if i == j:
+            return
+        a, b = self.n - i + 1, self.n - j + 1
+        lo, hi = min(a, b), max(a, b)
+        for k in range(lo, hi):
+            self.apply_2q(k, SWAP_GATE, 0.0, 0)
+        for k in range(hi - 2, lo - 1, -1):
+            self.apply_2q(k, SWAP_GATE, 0.0, 0)
 
     def amplitudes(self, bits):
         bits = np.asarray(bits, dtype=np.int64)

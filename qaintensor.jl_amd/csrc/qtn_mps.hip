@@ -1810,6 +1810,126 @@ int32_t qtn_mps_apply_2q(qtn_mps* m, int32_t site, const double* g,
   return qtn::abi_exception();
 }
 
+// ---- SWAP routing on the device state (switch!, permute!, long-range
+// gates).  A SWAP of sites (k, k+1) is the two-site update with the SWAP
+// gate (index s_k + 2 s_{k+1}: |01> <-> |10>).
+namespace {
+const double kSwapGate[32] = {1, 0, 0, 0, 0, 0, 0, 0,   // column 0: |00>
+                              0, 0, 0, 0, 1, 0, 0, 0,   // column 1 -> row 2
+                              0, 0, 1, 0, 0, 0, 0, 0,   // column 2 -> row 1
+                              0, 0, 0, 0, 0, 0, 1, 0};  // column 3: |11>
+
+int32_t swap_sites(qtn_mps* m, int k, double cutoff, int64_t cm, int form) {
+  return mps_apply_gates(m, 1, &k, kSwapGate, cutoff, cm, form, nullptr,
+                         nullptr, nullptr, nullptr);
+}
+
+// switch!(mps, i, j) (src/switch.jl:63-86) on the device state: wires are
+// numbered from the most significant (wire w is site n - w + 1); site lo
+// travels to hi by adjacent swaps and the qubit met last travels back.
+int32_t mps_switch(qtn_mps* m, int32_t i, int32_t j) {
+  if (!(i > 0 && j > 0)) {
+    set_error("Wire indices `i` and `j` must be positive");
+    return QTN_ERR_ARG;
+  }
+  if (!(i <= m->n && j <= m->n)) {
+    set_error("Indices to swap `i` and `j` must be less than or equal to "
+              "the number of open wires in MPS");
+    return QTN_ERR_ARG;
+  }
+  if (i == j) return QTN_OK;
+  const int a = m->n - i + 1, b = m->n - j + 1;
+  const int lo = std::min(a, b), hi = std::max(a, b);
+  // untruncated (only exact zeros are dropped), up to the state's capacity
+  for (int k = lo; k < hi; ++k)
+    QTN_TRY(swap_sites(m, k, 0.0, m->cap, QTN_FORM_PLAIN));
+  for (int k = hi - 2; k >= lo; --k)
+    QTN_TRY(swap_sites(m, k, 0.0, m->cap, QTN_FORM_PLAIN));
+  return QTN_OK;
+}
+}  // namespace
+
+int32_t qtn_mps_switch(qtn_mps* m, int32_t i, int32_t j) try {
+  clear_error();
+  QTN_ARG(m, "qtn_mps_switch: NULL state");
+  QTN_CHECK_HIP(hipSetDevice(m->ctx->device));
+  return mps_switch(m, i, j);
+} catch (...) {
+  return qtn::abi_exception();
+}
+
+int32_t qtn_mps_permute(qtn_mps* m, const int64_t* order, int64_t len) try {
+  clear_error();
+  QTN_ARG(m && (len == 0 || order), "qtn_mps_permute: NULL argument");
+  const int n = m->n;
+  QTN_ARG(len == n, "Given permutation must be same length as number of "
+                    "Tensors in MPS");
+  std::vector<int64_t> seen(order, order + len);
+  std::sort(seen.begin(), seen.end());
+  QTN_ARG(std::adjacent_find(seen.begin(), seen.end()) == seen.end(),
+          "Permutation order cannot contain repeat values");
+  QTN_ARG(seen.empty() || seen.front() > 0,
+          "Permutation order can only contain positive values");
+  QTN_ARG(seen.empty() || seen.back() == len,
+          "Wire numbers in permutation order cannot exceed number of wires "
+          "in MPS");
+  QTN_CHECK_HIP(hipSetDevice(m->ctx->device));
+  // Base.permute! (src/switch.jl:94-106): wire order[i] is brought to
+  // position i by switch!(mps, i, loc), tracking where each wire sits
+  std::vector<int64_t> seq(n);
+  for (int q = 0; q < n; ++q) seq[q] = q + 1;
+  for (int q = 0; q < n; ++q) {
+    const int loc =
+        (int)(std::find(seq.begin(), seq.end(), order[q]) - seq.begin());
+    QTN_TRY(mps_switch(m, q + 1, loc + 1));
+    std::swap(seq[loc], seq[q]);
+  }
+  return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
+}
+
+int32_t qtn_mps_apply_2q_long(qtn_mps* m, int32_t i, int32_t j,
+                              const double* g, double cutoff, int64_t chimax,
+                              int32_t form, int64_t* kept,
+                              double* disc_weight) try {
+  clear_error();
+  QTN_ARG(m && g, "qtn_mps_apply_2q_long: NULL argument");
+  QTN_ARG(cutoff >= 0.0, "Error must be positive");
+  QTN_ARG(form == QTN_FORM_PLAIN || form == QTN_FORM_CHOSEN,
+          "qtn_mps_apply_2q_long: unknown form");
+  QTN_ARG(chimax <= m->cap, "qtn_mps_apply_2q_long: chimax above capacity");
+  QTN_ARG(i >= 1 && i <= m->n && j >= 1 && j <= m->n && i != j,
+          "qtn_mps_apply_2q_long: sites must be distinct and in 1..n");
+  for (int e = 0; e < 32; ++e)
+    QTN_ARG(std::isfinite(g[e]), "qtn_mps_apply_2q_long: gate contains NaN "
+                                 "or Inf");
+  QTN_CHECK_HIP(hipSetDevice(m->ctx->device));
+  const int64_t cm = (chimax > 0) ? chimax : m->cap;
+  const int lo = std::min(i, j), hi = std::max(i, j);
+  // the gate's index is s_i + 2 s_j; applied to (lo, lo + 1) it must read
+  // s_lo + 2 s_hi, so the two bits trade places when i > j
+  double gl[32];
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) {
+      const int rr = i < j ? r : ((r & 1) << 1) | (r >> 1);
+      const int cc = i < j ? c : ((c & 1) << 1) | (c >> 1);
+      gl[2 * (r + 4 * c)] = g[2 * (rr + 4 * cc)];
+      gl[2 * (r + 4 * c) + 1] = g[2 * (rr + 4 * cc) + 1];
+    }
+  // the qubit of site hi travels down to lo + 1, the gate acts on
+  // (lo, lo + 1), and the qubit travels back; every step is a two-site
+  // update truncated by the same rule
+  for (int k = hi - 1; k > lo; --k) QTN_TRY(swap_sites(m, k, cutoff, cm, form));
+  int lo_site = lo;
+  QTN_TRY(mps_apply_gates(m, 1, &lo_site, gl, cutoff, cm, form, kept,
+                          disc_weight, nullptr, nullptr));
+  for (int k = lo + 1; k < hi; ++k) QTN_TRY(swap_sites(m, k, cutoff, cm, form));
+  return QTN_OK;
+} catch (...) {
+  return qtn::abi_exception();
+}
+
 int32_t qtn_mps_apply_layer(qtn_mps* m, int32_t ngates, const int32_t* sites,
                             const double* gates, double cutoff,
                             int64_t chimax, int32_t form, int64_t* kept,

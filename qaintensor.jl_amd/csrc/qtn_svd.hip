@@ -1191,7 +1191,10 @@ __global__ __launch_bounds__(256) void jac_colnorms(
     s += (double)v.x * v.x + (double)v.y * v.y;
   }
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  // columns the sweeps left alone (below the zero threshold) count as zero
+  // Columns at or below the zero threshold (the rotations' rounding floor)
+  // count as exact zeros: the ones the sweeps left alone, and the ones a
+  // rank-deficient X ends with, whose direction is rounding noise and
+  // would otherwise be kept (at a zero cutoff) as a singular vector.
   if (floor2 && s <= floor2[bz]) s = 0.0;
   // X is still scaled by 2^-e here
   if (lane == 0) sig[(int64_t)bz * cols + c] = ldexp(sqrt(s), scale_e[bz]);
@@ -1557,7 +1560,8 @@ int32_t JacobiEngine::run32(qtn_ctx* ctx, float2* X, int64_t ldx, int64_t sX,
                       relative, chimax, max_sweeps, sweeps_done, mask_host);
 }
 
-// FP64: tol = sqrt(rows) eps / 2 and zero columns below rows eps |X|_F.
+// FP64: tol = sqrt(rows) eps / 2 and zero columns below rows eps |X|_F
+// (before the sweeps: never rotated; after them: exact zeros in sig).
 // FP32 (the chosen form's sweeps): tol = sqrt(rows) eps32 and columns below
 // 8 sqrt(rows) eps32 |X|_F (the rounding noise a column gathers over the
 // sweeps) are left alone.
@@ -1679,8 +1683,7 @@ int32_t JacobiEngine::run_t(qtn_ctx* ctx, typename CType<R>::type* X,
   hipLaunchKernelGGL(jac_colnorms<R>,
                      dim3((unsigned)((cols_p + 3) / 4), batch), dim3(256), 0,
                      s, (const c_t*)X, rows_p, ldx, cols_p, sX, sig,
-                     (const double*)(dbl ? nullptr : zthr),
-                     (const int*)scale_e);
+                     (const double*)zthr, (const int*)scale_e);
   hipLaunchKernelGGL(jac_scale<R>, dim3(sgrid, batch), dim3(256), 0, s, X,
                      rows_p, ldx, cols_p, sX, (const int*)scale_e, 1);
   QTN_CHECK_HIP(hipGetLastError());

@@ -32,11 +32,13 @@ EXPORTS = [
     "qtn_ctx_counters", "qtn_ctx_profile", "qtn_ctx_profile_read",
     "qtn_last_error", "qtn_version",
     "qtn_contract_pair", "qtn_contract_pair_dev", "qtn_permute",
+    "qtn_contract_network",
     "qtn_permute_dev", "qtn_zgemm_dev", "qtn_cgemm_dev",
     "qtn_svd_trunc", "qtn_svd", "qtn_svd_values", "qtn_contract_svd",
     "qtn_mps_create", "qtn_mps_destroy", "qtn_mps_copy", "qtn_mps_set_sites",
     "qtn_mps_bond_dims", "qtn_mps_bond_spectrum", "qtn_mps_get_site",
     "qtn_mps_apply_1q", "qtn_mps_apply_2q", "qtn_mps_apply_layer",
+    "qtn_mps_switch", "qtn_mps_permute", "qtn_mps_apply_2q_long",
     "qtn_mps_apply_layer_multi", "qtn_mps_apply_1q_multi",
     "qtn_mps_apply_layer_learned", "qtn_mps_gate_features",
     "qtn_gate_schmidt", "qtn_tune_jacobi", "qtn_set_jacobi_tiling",
@@ -95,6 +97,9 @@ def _declare(lib):
                                          _pi64, _i32, _pi32, _pi32, _pi32,
                                          _vp]),
         "qtn_permute": (_i32, [_vp, _pd, _i32, _pi64, _pi32, _pd]),
+        "qtn_contract_network": (_i32, [_vp, _i32, _pi32, _pi64, _pi32,
+                                        ctypes.POINTER(_pd), _i32, _pi32,
+                                        _pd, _i64]),
         "qtn_permute_dev": (_i32, [_vp, _vp, _i32, _pi64, _pi32, _vp]),
         "qtn_zgemm_dev": (_i32, [_vp, ctypes.c_char, ctypes.c_char, _i64,
                                  _i64, _i64, _pd, _vp, _i64, _i64, _vp, _i64,
@@ -122,6 +127,10 @@ def _declare(lib):
                                     _pd, _pd, _pi64]),
         "qtn_mps_apply_layer": (_i32, [_vp, _i32, _pi32, _pd, _dbl, _i64,
                                        _i32, _pi64, _pd]),
+        "qtn_mps_switch": (_i32, [_vp, _i32, _i32]),
+        "qtn_mps_permute": (_i32, [_vp, _pi64, _i64]),
+        "qtn_mps_apply_2q_long": (_i32, [_vp, _i32, _i32, _pd, _dbl, _i64,
+                                         _i32, _pi64, _pd]),
         "qtn_mps_apply_layer_multi": (_i32, [_i32, ctypes.POINTER(_vp), _pi32,
                                              _pd, _pi64, _dbl, _i32, _pi64,
                                              _pd]),

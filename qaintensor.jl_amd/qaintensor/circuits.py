@@ -102,3 +102,23 @@ def batch_circuits(count, seed, n_range=(16, 48), depth_range=(8, 24),
         out.append((n, depth, chi, brickwall_layers(n, depth,
                                                     seed * 100003 + i)))
     return out
+
+
+H_GATE = np.array([[1, 1], [1, -1]], dtype=np.complex128) / np.sqrt(2.0)
+
+
+def qft_gates(n):
+    """The quantum Fourier transform on wires 1..n (wire n the most
+    significant) as a gate list, without the final bit reversal:
+    [("1q", w, u)] and [("2q", i, j, g)] with g = diag(1, 1, 1, e^{i pi /
+    2^(k-m)}) the controlled phase of wires k > m (symmetric in its two
+    qubits, index s_i + 2 s_j).  Wire k meets every lower wire: the
+    long-range gates of the circuits SWAP routing serves."""
+    ops = []
+    for k in range(n, 0, -1):
+        ops.append(("1q", k, H_GATE))
+        for m in range(k - 1, 0, -1):
+            ph = np.exp(1j * np.pi / 2 ** (k - m))
+            ops.append(("2q", k, m, np.diag([1, 1, 1, ph]).astype(
+                np.complex128)))
+    return ops

@@ -80,6 +80,49 @@ class DeviceMPS:
             return kept.value, disc.value, spec[:slen.value].copy()
         return kept.value, disc.value
 
+    SWAP = np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]],
+                    dtype=np.complex128)
+
+    def apply_2q_long(self, i, j, g, cutoff, chimax, form=FORM_PLAIN):
+        """Gate g (4x4, index s_i + 2 s_j) on any two sites i != j: SWAP
+        routing of the farther qubit next to the nearer one, the gate, and
+        the way back, each step a truncated two-site update
+        (qtn_mps_apply_2q_long).  Returns (kept rank, discarded weight) of
+        the gate's own update."""
+        g = L.as_c128(g, "F")
+        if g.shape != (4, 4):
+            raise ValueError("two-qubit gate must be 4x4")
+        kept = ctypes.c_int64(0)
+        disc = ctypes.c_double(0.0)
+        L.check(L.load().qtn_mps_apply_2q_long(
+            self.handle, int(i), int(j), L.dptr(g), float(cutoff),
+            int(chimax), int(form), ctypes.byref(kept), ctypes.byref(disc)))
+        return kept.value, disc.value
+
+    def switch(self, i, j=None):
+        """switch! on the device state (src/switch.jl): switch(i, j) swaps
+        wires i and j, numbered from the most significant as the reference
+        does (qtn_mps_switch); switch([(i, j), ...]) swaps pair by pair
+        (src/switch.jl:6-11); switch(i) swaps sites i and i + 1
+        (src/switch.jl:18-56).  Untruncated but for exact zeros, up to the
+        state's capacity."""
+        if j is None and isinstance(i, (list, tuple)):
+            for (a, b) in i:
+                L.check(L.load().qtn_mps_switch(self.handle, int(a), int(b)))
+            return self
+        if j is None:
+            self.apply_2q(int(i), self.SWAP, 0.0, self.cap)
+            return self
+        L.check(L.load().qtn_mps_switch(self.handle, int(i), int(j)))
+        return self
+
+    def permute(self, order):
+        """Base.permute!(mps, order) on the device state
+        (src/switch.jl:94-106; qtn_mps_permute)."""
+        o = np.ascontiguousarray(order, dtype=np.int64)
+        L.check(L.load().qtn_mps_permute(self.handle, L.i64ptr(o), o.size))
+        return self
+
     def apply_layer(self, sites, gates, cutoff, chimax, form=FORM_PLAIN):
         """Gates on disjoint pairs in one batched pass.  Returns (kept[],
         disc[])."""

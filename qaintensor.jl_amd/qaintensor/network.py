@@ -1,9 +1,10 @@
 """Tensor-network data model and contract() (Qaintensor.jl src/tensor.jl,
 src/tensor_network.jl, src/contract.jl:39-60, 242-264).
 
-Arrays keep the reference's (Julia) index order; every pairwise step of a
-contraction runs on the GPU through qtn_contract_pair (permute + complex
-MFMA GEMM).  Leg and tensor numbers are 1-based as in the reference.
+Arrays keep the reference's (Julia) index order; a whole contraction runs
+on the GPU in one qtn_contract_network call (every pairwise step one
+gather-staged complex MFMA GEMM, intermediates kept on the device).  Leg
+and tensor numbers are 1-based as in the reference.
 """
 import numpy as np
 
@@ -115,96 +116,126 @@ def contract_rep(net):
     return indexlist
 
 
-def _ncon(tensors, labels):
-    """ncon of the labelled tensors on the GPU.  Schedule: positive labels in
-    increasing order, each step contracting every label the two holders
-    share; disconnected pieces are joined by outer products; a label held
-    twice by one tensor (a trace) is contracted against an identity.  The
-    result is ordered by negative label, -1 (or the largest) first."""
-    items = [(np.asarray(t, dtype=np.complex128), list(l))
-             for t, l in zip(tensors, labels)]
-    for lab in sorted({x for _, l in items for x in l if x > 0}):
-        holders = [i for i, (_, l) in enumerate(items) if lab in l]
-        if not holders:
-            continue  # already contracted together with an earlier label
-        if len(holders) == 1:
-            (A, la) = items[holders[0]]
-            p, q = [k for k, x in enumerate(la) if x == lab]
-            eye = np.eye(A.shape[p], dtype=np.complex128)
-            C = ops.contract_pair(A, eye, [p + 1, q + 1], [1, 2])
-            items[holders[0]] = (C, [x for x in la if x != lab])
-            continue
-        i, j = holders
-        (A, la), (B, lb) = items[i], items[j]
-        shared = [x for x in la if x > 0 and x in lb]
-        legsA = [la.index(x) + 1 for x in shared]
-        legsB = [lb.index(x) + 1 for x in shared]
-        C = ops.contract_pair(A, B, legsA, legsB)
-        lc = [x for x in la if x not in shared] + \
-             [x for x in lb if x not in shared]
-        items = [it for k, it in enumerate(items) if k not in (i, j)]
-        items.append((C, lc))
-    while len(items) > 1:
-        (A, la), (B, lb) = items[0], items[1]
-        C = ops.contract_pair(A, B, [], [])
-        items = [(C, la + lb)] + items[2:]
-    C, lc = items[0]
-    order = sorted(range(len(lc)), key=lambda k: -lc[k])
-    if order != list(range(len(lc))):
-        C = ops.permute(C, [k + 1 for k in order])
-    return C
+def leg_costs(net):
+    """The leg sizes of contract_rep(net, optimize) (src/contract.jl:8-32):
+    {label: size} for contraction i -> label i and open leg i -> label
+    i + #contractions."""
+    costs = {}
+    for i, c in enumerate(net.contractions, start=1):
+        for (t, leg) in c.idx:
+            costs[i] = net.tensors[t - 1].size(leg)
+    J = len(net.contractions)
+    for i, (t, leg) in enumerate(net.openidx, start=1):
+        costs[i + J] = net.tensors[t - 1].size(leg)
+    return costs
+
+
+class _Obj:
+    """ContractionObject (src/contract.jl:161-169): the legs and tensors of a
+    partial contraction, the leg sequence that builds it and its cost."""
+    __slots__ = ("legs", "tensors", "seq", "cost")
+
+    def __init__(self, legs, tensors, seq, cost):
+        self.legs, self.tensors, self.seq, self.cost = legs, tensors, seq, cost
+
+
+def _build_cost(free, common, costs, mu_old, mu, isnew, c1, c2):
+    """getBuildCost (src/contract.jl:62-86): cost of building a tensor from
+    two (the product of the sizes of all their legs, plus the two costs);
+    refused above the cap mu, or, for objects already built at an earlier
+    cap, at or below mu_old."""
+    new = 1
+    for leg in free | common:
+        new *= costs[leg]
+    new += c1 + c2
+    if new > mu:
+        return new, False
+    if not isnew and new <= mu_old:
+        return float("inf"), False
+    return new, True
+
+
+def _check(S, a, b, costs, newflags, mus):
+    """check_contraction! (src/contract.jl:88-158): every disjoint pair of an
+    a-tensor and a b-tensor object that share a leg builds an (a+b)-tensor
+    object if its cost is within the cap; the cheapest one per tensor set
+    is kept."""
+    mu_old, mu_cap, mu_next = mus
+    Sab = S[a + b]
+    for i, Ta in enumerate(S[a]):
+        for j, Tb in enumerate(S[b]):
+            if Ta.tensors & Tb.tensors:
+                continue
+            common = Ta.legs & Tb.legs
+            if not common:
+                continue
+            free = Ta.legs ^ Tb.legs
+            cost, ok = _build_cost(free, common, costs, mu_old, mu_cap,
+                                   newflags[a][i] or newflags[b][j],
+                                   Ta.cost, Tb.cost)
+            if not ok:
+                mu_next = min(mu_next, cost)
+                continue
+            tens = Ta.tensors | Tb.tensors
+            seq = Ta.seq + Tb.seq + sorted(common)
+            for k, o in enumerate(Sab):
+                if o.tensors == tens:
+                    if o.cost > cost:
+                        Sab[k] = _Obj(free, tens, seq, cost)
+                        newflags[a + b][k] = True
+                    break
+            else:
+                Sab.append(_Obj(free, tens, seq, cost))
+                newflags[a + b].append(True)
+    return mu_old, mu_cap, mu_next
+
+
+def contract_order(net, costs, indexlist):
+    """contract_order (src/contract.jl:177-235): the breadth-first search of
+    Pfeifer, Haegeman and Verstraete (arXiv:1304.6112) over objects built
+    from 1, 2, ..., n tensors under a rising cost cap.  Returns (sequence,
+    cost): the contracted labels in the order the cheapest full build
+    contracts them.  (The reference loops forever on a network that is not
+    connected; this raises instead.)"""
+    n = len(net.tensors)
+    S = [None, [_Obj(frozenset(abs(x) for x in lst), frozenset([i]), [], 0)
+                for i, lst in enumerate(indexlist)]]
+    newflags = [None, [True] * n]
+    for _ in range(2, n + 1):
+        S.append([])
+        newflags.append([])
+    mu_old, mu_cap, mu_next = 0, 1, float("inf")
+    while not S[n]:
+        for c in range(2, n + 1):
+            for d in range(1, c // 2 + 1):
+                mu_old, mu_cap, mu_next = _check(S, d, c - d, costs, newflags,
+                                                 (mu_old, mu_cap, mu_next))
+        if mu_next == float("inf") and not S[n]:
+            raise ErrorException("contract_order: the network is not "
+                                 "connected")
+        mu_old, mu_cap, mu_next = mu_cap, mu_next, float("inf")
+        for fl in newflags[1:]:
+            fl[:] = [False] * len(fl)
+    best = S[n][0]
+    return list(best.seq), best.cost
 
 
 def contract(net, optimize=False):
     """src/contract.jl:242-264.  With one tensor only the open legs are
-    permuted.  optimize=True orders the pairwise steps greedily by the size
-    of each step's result (the reference's opt_einsum-style planner,
-    src/contract.jl:8-235, is host-side planning outside this library's
-    scope; any order gives the same tensor)."""
+    permuted.  Otherwise the whole network contracts on the GPU in one call
+    (qtn_contract_network, intermediates on the device): by increasing label
+    (ncon's default order), or with optimize=True in the order the
+    reference's planner contract_order returns.  (The reference relabels the
+    legs by their position in that sequence and then passes the old labels
+    as ncon's order, src/contract.jl:250-257, so it does not contract in the
+    order it planned; here the planned sequence is the order.)"""
     if len(net.tensors) == 1:
         perm = [leg for (_, leg) in net.openidx]
         return ops.permute(net.tensors[0].data, perm)
     labels = contract_rep(net)
     tensors = [t.data for t in net.tensors]
-    if not optimize:
-        return _ncon(tensors, labels)
-    return _ncon_greedy(tensors, labels)
-
-
-def _ncon_greedy(tensors, labels):
-    items = [(np.asarray(t, dtype=np.complex128), list(l))
-             for t, l in zip(tensors, labels)]
-    dim = {}
-    for t, l in items:
-        for x, d in zip(l, t.shape):
-            dim[x] = d
-    while True:
-        best = None
-        for i in range(len(items)):
-            for j in range(i + 1, len(items)):
-                li, lj = items[i][1], items[j][1]
-                shared = [x for x in li if x > 0 and x in lj]
-                if not shared:
-                    continue
-                out = [x for x in li + lj if x not in shared]
-                cost = float(np.prod([dim[x] for x in out])) if out else 1.0
-                if best is None or cost < best[0]:
-                    best = (cost, i, j, shared)
-        if best is None:
-            break
-        _, i, j, shared = best
-        (A, la), (B, lb) = items[i], items[j]
-        C = ops.contract_pair(A, B, [la.index(x) + 1 for x in shared],
-                              [lb.index(x) + 1 for x in shared])
-        lc = [x for x in la if x not in shared] + \
-             [x for x in lb if x not in shared]
-        items = [it for k, it in enumerate(items) if k not in (i, j)]
-        items.append((C, lc))
-    while len(items) > 1:
-        (A, la), (B, lb) = items[0], items[1]
-        items = [(ops.contract_pair(A, B, [], []), la + lb)] + items[2:]
-    C, lc = items[0]
-    order = sorted(range(len(lc)), key=lambda k: -lc[k])
-    if order != list(range(len(lc))):
-        C = ops.permute(C, [k + 1 for k in order])
-    return C
+    if optimize:
+        order, _ = contract_order(net, leg_costs(net), labels)
+    else:
+        order = sorted({x for l in labels for x in l if x > 0})
+    return ops.contract_network(tensors, labels, order)

@@ -51,6 +51,37 @@ def contract_pair(A, B, legsA, legsB, outperm=None, ctx=None):
     return C
 
 
+def contract_network(tensors, labels, order, ctx=None):
+    """Contract a whole network on the GPU (qtn_contract_network): arrays
+    (Julia index order), ncon labels per tensor, and the contraction order
+    (every positive label once).  Returns the result with its legs ordered
+    by open label -1, -2, ... (a 0-d array for a scalar)."""
+    ctx = _ctx(ctx)
+    arrs = [L.as_c128(t, "F") for t in tensors]
+    if len(arrs) != len(labels):
+        raise ValueError("one label list per tensor")
+    ndims = np.asarray([a.ndim for a in arrs], dtype=np.int32)
+    dims = np.asarray([d for a in arrs for d in a.shape], dtype=np.int64)
+    labs = np.asarray([x for l in labels for x in l], dtype=np.int32)
+    if labs.size != dims.size:
+        raise ValueError("one label per leg")
+    size = {}
+    for a, l in zip(arrs, labels):
+        for x, d in zip(l, a.shape):
+            size[x] = d
+    open_ = sorted((x for x in size if x < 0), reverse=True)
+    shape = tuple(size[x] for x in open_)
+    out = np.zeros(shape, dtype=np.complex128, order="F")
+    ordr = np.ascontiguousarray(order, dtype=np.int32)
+    ptrs = (ctypes.POINTER(ctypes.c_double) * len(arrs))(
+        *[L.dptr(a) for a in arrs])
+    L.check(L.load().qtn_contract_network(
+        ctx.handle, len(arrs), L.i32ptr(ndims), L.i64ptr(dims),
+        L.i32ptr(labs), ptrs, ordr.size, L.i32ptr(ordr),
+        L.dptr(out), int(out.size)))
+    return out
+
+
 def permute(A, perm, ctx=None):
     """permutedims(A, perm) (perm 1-based) on the GPU."""
     ctx = _ctx(ctx)

@@ -450,3 +450,75 @@ def test_mlp_oracle_forward_matches_torch():
         ref = net(torch.from_numpy(inp.T.copy())).numpy().T
     got = O.mlp_forward(p, X, nin, w)
     assert np.max(np.abs(got - ref)) <= 1e-12
+
+
+def _product_tebd(bvecs):
+    """TebdState holding kron(b_1, ..., b_N) (b_1 the most significant
+    factor, so site w holds b_{N-w+1}), unnormalised product tensors."""
+    n = len(bvecs)
+    st = O.TebdState(n)
+    for w in range(1, n + 1):
+        st.B[w - 1] = np.asarray(bvecs[n - w], dtype=np.complex128).reshape(
+            (1, 2, 1))
+    return st
+
+
+def test_switch_reference_case():
+    """test/test_mps.jl:198-221: switch!(mps, [(2,3), (4,6), (4,5)]) on
+    kron(b...) gives kron(b[order]...), order = [1, 3, 2, 5, 6, 4]."""
+    rng = np.random.default_rng(11)
+    N = 6
+    b = [rng.standard_normal(2) + 1j * rng.standard_normal(2)
+         for _ in range(N)]
+    order = [1, 3, 2, 5, 6, 4]
+    psi_perm = b[order[0] - 1]
+    for k in order[1:]:
+        psi_perm = np.kron(psi_perm, b[k - 1])
+    st = _product_tebd(b)
+    for (i, j) in [(2, 3), (4, 6), (4, 5)]:
+        st.switch(i, j)
+    v = st.to_statevector()
+    fid = abs(np.vdot(v, psi_perm)) / (np.linalg.norm(v) *
+                                      np.linalg.norm(psi_perm))
+    assert abs(1 - fid) <= 1e-12
+    with pytest.raises(ValueError, match="must be positive"):
+        st.switch(1, -2)
+    with pytest.raises(ValueError, match="less than or equal"):
+        st.switch(7, 4)
+
+
+def test_switch_exchanges_dense_bits():
+    """switch!(i, j) on an entangled state equals exchanging bits n-i and
+    n-j of the dense vector (wire w <-> site n-w+1 <-> bit n-w)."""
+    n = 7
+    st = O.TebdState(n)
+    O.run_layers(st, C.brickwall_layers(n, 5, 4), 0.0, 0)
+    for (i, j) in [(1, 7), (3, 5), (6, 2)]:
+        v0 = st.to_statevector()
+        st.switch(i, j)
+        v = st.to_statevector()
+        idx = np.arange(2 ** n)
+        a, c = n - i, n - j
+        bi, bj = (idx >> a) & 1, (idx >> c) & 1
+        src = idx ^ ((bi ^ bj) << a) ^ ((bi ^ bj) << c)
+        ref = v0[src]
+        assert np.linalg.norm(v / np.linalg.norm(v) - ref /
+                              np.linalg.norm(ref)) <= 1e-12
+
+
+@pytest.mark.parametrize("i,j", [(2, 6), (6, 2), (1, 8), (8, 1), (4, 5)])
+def test_apply_2q_long_equals_dense(i, j):
+    """SWAP routing without truncation equals the gate applied on wires
+    (i, j) of the dense vector, for both orders of the wires."""
+    n = 8
+    rng = np.random.default_rng(i * 10 + j)
+    layers = C.brickwall_layers(n, 4, i + j)
+    st = O.TebdState(n)
+    O.run_layers(st, layers, 0.0, 0)
+    sv = O.StateVector(n)
+    sv.psi = st.to_statevector()
+    g = C.haar_unitary(rng, 4)
+    st.apply_2q_long(i, j, g, 0.0, 0)
+    sv.apply_2q_any(i, j, g)
+    v = st.to_statevector()
+    assert np.linalg.norm(v - sv.psi) <= 1e-12 * np.linalg.norm(sv.psi)
