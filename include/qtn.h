@@ -120,6 +120,24 @@ int32_t qtn_contract_pair_dev(qtn_ctx* ctx,
                               int32_t ndB, const int64_t* dimsB, int32_t nc,
                               const int32_t* legsA, const int32_t* legsB,
                               const int32_t* outperm, double* C);
+/* qtn_contract_pair_dev that allocates nothing (no hipMalloc / hipFree or
+ * allocating routine in the call): its scratch -- the permuted operands
+ * and the product in natural leg order -- comes from one fixed 32 MiB
+ * device array per GPU that the library holds until the process ends.  No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  A contraction whose scratch (each piece rounded to
+ * 256 bytes) exceeds 32 MiB is refused with QTN_ERR_ARG.  Every call on a
+ * GPU shares that GPU's array, so calls that overlap on one GPU (two
+ * streams, two contexts) race; calls on one context's stream run in order.
+ * Queued on the context's stream like qtn_contract_pair_dev.              */
+int32_t qtn_contract_pair_dev_noalloc(qtn_ctx* ctx, const double* A,
+                                      int32_t ndA, const int64_t* dimsA,
+                                      const double* B, int32_t ndB,
+                                      const int64_t* dimsB, int32_t nc,
+                                      const int32_t* legsA,
+                                      const int32_t* legsB,
+                                      const int32_t* outperm, double* C);
 
 /* Axis permutation of a column-major complex tensor:
  * out = permutedims(in, perm) (perm 1-based).  Replaces permutedims at

@@ -9,6 +9,8 @@
 #include <algorithm>
 #include <numeric>
 
+#include <mutex>
+
 #include "qtn_internal.hpp"
 
 namespace qtn {
@@ -193,20 +195,23 @@ int32_t launch_permute(hipStream_t s, const double2* in, int nd,
 
 namespace {
 
-// Host-side plan of one pairwise contraction.
+// Host-side plan of one pairwise contraction (fixed-size arrays: planning
+// allocates nothing).
 struct PairPlan {
-  std::vector<int> permA, permB;     // 0-based
+  int permA[kMaxNd], permB[kMaxNd];  // 0-based
+  int ndA = 0, ndB = 0;
   int64_t M = 1, N = 1, K = 1;
   bool transA = false, transB = false;  // use op 'T' instead of permuting
   bool permuteA = false, permuteB = false;
-  std::vector<int64_t> natdims;      // natural output dims
-  std::vector<int> outperm0;         // 0-based
+  int64_t natdims[2 * kMaxNd];        // natural output dims
+  int outperm0[2 * kMaxNd];           // 0-based
+  int ndC = 0;
   bool permuteC = false;
 };
 
-bool is_identity(const std::vector<int>& p) {
-  for (size_t i = 0; i < p.size(); ++i)
-    if (p[i] != (int)i) return false;
+bool is_identity(const int* p, int n) {
+  for (int i = 0; i < n; ++i)
+    if (p[i] != i) return false;
   return true;
 }
 
@@ -222,8 +227,8 @@ int32_t make_plan(int ndA, const int64_t* dimsA, int ndB,
     QTN_ARG(dimsA[i] >= 0, "contract_pair: negative dimension in A");
   for (int i = 0; i < ndB; ++i)
     QTN_ARG(dimsB[i] >= 0, "contract_pair: negative dimension in B");
-  std::vector<char> usedA(ndA, 0), usedB(ndB, 0);
-  std::vector<int> la, lb;
+  char usedA[kMaxNd] = {}, usedB[kMaxNd] = {};
+  int la[kMaxNd], lb[kMaxNd], fa[kMaxNd], fb[kMaxNd];
   for (int i = 0; i < nc; ++i) {
     const int a = legsA[i] - 1, b = legsB[i] - 1;
     QTN_ARG(a >= 0 && a < ndA && b >= 0 && b < ndB,
@@ -232,89 +237,95 @@ int32_t make_plan(int ndA, const int64_t* dimsA, int ndB,
     QTN_ARG(dimsA[a] == dimsB[b],
             "Dimensions of contraction legs do not match");
     usedA[a] = usedB[b] = 1;
-    la.push_back(a);
-    lb.push_back(b);
+    la[i] = a;
+    lb[i] = b;
   }
-  std::vector<int> fa, fb;
+  int nfa = 0, nfb = 0;
   for (int i = 0; i < ndA; ++i)
-    if (!usedA[i]) fa.push_back(i);
+    if (!usedA[i]) fa[nfa++] = i;
   for (int i = 0; i < ndB; ++i)
-    if (!usedB[i]) fb.push_back(i);
+    if (!usedB[i]) fb[nfb++] = i;
+  pl.ndA = ndA;
+  pl.ndB = ndB;
   pl.M = pl.N = pl.K = 1;
-  for (int a : fa) pl.M *= dimsA[a];
-  for (int b : fb) pl.N *= dimsB[b];
-  for (int a : la) pl.K *= dimsA[a];
+  for (int i = 0; i < nfa; ++i) pl.M *= dimsA[fa[i]];
+  for (int i = 0; i < nfb; ++i) pl.N *= dimsB[fb[i]];
+  for (int i = 0; i < nc; ++i) pl.K *= dimsA[la[i]];
   // A as (free x contracted): permA = fa ++ la; or (contracted x free).
-  pl.permA = fa;
-  pl.permA.insert(pl.permA.end(), la.begin(), la.end());
-  std::vector<int> altA = la;
-  altA.insert(altA.end(), fa.begin(), fa.end());
-  if (is_identity(pl.permA)) {
+  int altA[kMaxNd];
+  for (int i = 0; i < nfa; ++i) pl.permA[i] = fa[i];
+  for (int i = 0; i < nc; ++i) pl.permA[nfa + i] = la[i];
+  for (int i = 0; i < nc; ++i) altA[i] = la[i];
+  for (int i = 0; i < nfa; ++i) altA[nc + i] = fa[i];
+  if (is_identity(pl.permA, ndA)) {
     pl.transA = false;
     pl.permuteA = false;
-  } else if (is_identity(altA)) {
+  } else if (is_identity(altA, ndA)) {
     pl.transA = true;
     pl.permuteA = false;
   } else {
     pl.transA = false;
     pl.permuteA = true;
   }
-  pl.permB = lb;
-  pl.permB.insert(pl.permB.end(), fb.begin(), fb.end());
-  std::vector<int> altB = fb;
-  altB.insert(altB.end(), lb.begin(), lb.end());
-  if (is_identity(pl.permB)) {
+  int altB[kMaxNd];
+  for (int i = 0; i < nc; ++i) pl.permB[i] = lb[i];
+  for (int i = 0; i < nfb; ++i) pl.permB[nc + i] = fb[i];
+  for (int i = 0; i < nfb; ++i) altB[i] = fb[i];
+  for (int i = 0; i < nc; ++i) altB[nfb + i] = lb[i];
+  if (is_identity(pl.permB, ndB)) {
     pl.transB = false;
     pl.permuteB = false;
-  } else if (is_identity(altB)) {
+  } else if (is_identity(altB, ndB)) {
     pl.transB = true;
     pl.permuteB = false;
   } else {
     pl.transB = false;
     pl.permuteB = true;
   }
-  pl.natdims.clear();
-  for (int a : fa) pl.natdims.push_back(dimsA[a]);
-  for (int b : fb) pl.natdims.push_back(dimsB[b]);
-  const int ndC = (int)pl.natdims.size();
-  pl.outperm0.resize(ndC);
-  std::vector<char> seen(ndC, 0);
-  for (int i = 0; i < ndC; ++i) {
+  pl.ndC = 0;
+  for (int i = 0; i < nfa; ++i) pl.natdims[pl.ndC++] = dimsA[fa[i]];
+  for (int i = 0; i < nfb; ++i) pl.natdims[pl.ndC++] = dimsB[fb[i]];
+  char seen[2 * kMaxNd] = {};
+  for (int i = 0; i < pl.ndC; ++i) {
     const int p = outperm ? outperm[i] - 1 : i;
-    QTN_ARG(p >= 0 && p < ndC && !seen[p],
+    QTN_ARG(p >= 0 && p < pl.ndC && !seen[p],
             "contract_pair: outperm is not a permutation");
     seen[p] = 1;
     pl.outperm0[i] = p;
   }
-  pl.permuteC = !is_identity(pl.outperm0);
+  pl.permuteC = !is_identity(pl.outperm0, pl.ndC);
+  QTN_ARG(!pl.permuteC || pl.ndC <= kMaxNd,
+          "contract_pair: a permuted result has at most 16 legs");
   return QTN_OK;
 }
 
-int32_t run_plan(qtn_ctx* ctx, const PairPlan& pl, const double2* A,
-                 int ndA, const int64_t* dimsA, const double2* B, int ndB,
-                 const int64_t* dimsB, double2* C) {
-  hipStream_t s = ctx->stream;
+// Element counts of the plan's scratch: the permuted A and B and the
+// product in natural leg order (0 where that step is not needed).
+void plan_scratch(const PairPlan& pl, int64_t* nA, int64_t* nB,
+                  int64_t* nC) {
+  *nA = pl.permuteA ? std::max<int64_t>(1, pl.M * pl.K) : 0;
+  *nB = pl.permuteB ? std::max<int64_t>(1, pl.K * pl.N) : 0;
+  *nC = pl.permuteC ? std::max<int64_t>(1, pl.M * pl.N) : 0;
+}
+
+// Permutes and GEMM of one pairwise contraction on stream s, with the
+// scratch given (sA, sB, sC of plan_scratch's sizes; unused ones may be
+// null).  Launches only; allocates nothing.
+int32_t run_plan_with(hipStream_t s, const PairPlan& pl, const double2* A,
+                      const int64_t* dimsA, const double2* B,
+                      const int64_t* dimsB, double2* sA, double2* sB,
+                      double2* sC, double2* C) {
   const double2* Am = A;
   const double2* Bm = B;
-  void* p = nullptr;
   if (pl.permuteA) {
-    QTN_TRY(ctx->buffer(0, std::max<int64_t>(1, pl.M * pl.K) * sizeof(double2),
-                        &p));
-    QTN_TRY(launch_permute(s, A, ndA, dimsA, pl.permA.data(), (double2*)p));
-    Am = (const double2*)p;
+    QTN_TRY(launch_permute(s, A, pl.ndA, dimsA, pl.permA, sA));
+    Am = sA;
   }
   if (pl.permuteB) {
-    QTN_TRY(ctx->buffer(1, std::max<int64_t>(1, pl.K * pl.N) * sizeof(double2),
-                        &p));
-    QTN_TRY(launch_permute(s, B, ndB, dimsB, pl.permB.data(), (double2*)p));
-    Bm = (const double2*)p;
+    QTN_TRY(launch_permute(s, B, pl.ndB, dimsB, pl.permB, sB));
+    Bm = sB;
   }
-  double2* Cn = C;
-  if (pl.permuteC) {
-    QTN_TRY(ctx->buffer(2, std::max<int64_t>(1, pl.M * pl.N) * sizeof(double2),
-                        &p));
-    Cn = (double2*)p;
-  }
+  double2* Cn = pl.permuteC ? sC : C;
   const int opA = pl.transA ? 1 : 0;
   const int opB = pl.transB ? 1 : 0;
   const int64_t lda = pl.transA ? std::max<int64_t>(1, pl.K)
@@ -324,17 +335,106 @@ int32_t run_plan(qtn_ctx* ctx, const PairPlan& pl, const double2* A,
   QTN_TRY(launch_zgemm(s, opA, opB, pl.M, pl.N, pl.K, make_double2(1, 0), Am,
                        lda, 0, Bm, ldb, 0, make_double2(0, 0), Cn,
                        std::max<int64_t>(1, pl.M), 0, 1));
-  if (pl.permuteC) {
-    QTN_TRY(launch_permute(s, Cn, (int)pl.natdims.size(), pl.natdims.data(),
-                           pl.outperm0.data(), C));
-  }
+  if (pl.permuteC)
+    QTN_TRY(launch_permute(s, Cn, pl.ndC, pl.natdims, pl.outperm0, C));
   return QTN_OK;
+}
+
+// The plan with the context's scratch slots 0..2.
+int32_t run_plan(qtn_ctx* ctx, const PairPlan& pl, const double2* A,
+                 const int64_t* dimsA, const double2* B,
+                 const int64_t* dimsB, double2* C) {
+  int64_t nA, nB, nC;
+  plan_scratch(pl, &nA, &nB, &nC);
+  void *pA = nullptr, *pB = nullptr, *pC = nullptr;
+  if (nA) QTN_TRY(ctx->buffer(0, nA * sizeof(double2), &pA));
+  if (nB) QTN_TRY(ctx->buffer(1, nB * sizeof(double2), &pB));
+  if (nC) QTN_TRY(ctx->buffer(2, nC * sizeof(double2), &pC));
+  return run_plan_with(ctx->stream, pl, A, dimsA, B, dimsB, (double2*)pA,
+                       (double2*)pB, (double2*)pC, C);
 }
 
 }  // namespace
 }  // namespace qtn
 
+// ---- the allocation-free pairwise contraction -----------------------------
+// scratch: the device memory qtn_contract_pair_dev_noalloc takes its scratch
+// from (the permuted operands and the product in natural leg order), 32 MiB
+// per GPU for the life of the process.  Host code never passes the array's
+// name as an address (on the host it is only a stand-in): each GPU's copy
+// sits at its own device address, looked up once per device by
+// noalloc_scratch_init (qtn_ctx_create calls it with the device current)
+// and kept in g_scratch_addr.
+constexpr int64_t kNoAllocScratchBytes = (int64_t)32 << 20;
+constexpr int64_t kNoAllocScratchElems = kNoAllocScratchBytes / 16;
+static __device__ __align__(256) double2 scratch[kNoAllocScratchElems];
+
+namespace qtn {
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxScratchDevices] = {};
+}  // namespace
+
+int32_t noalloc_scratch_init(int device) {
+  QTN_ARG(device >= 0 && device < kMaxScratchDevices,
+          "qtn_ctx_create: device number too large for the no-allocation "
+          "scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_addr[device]) return QTN_OK;
+  void* p = nullptr;
+  QTN_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_addr[device] = p;
+  return QTN_OK;
+}
+}  // namespace qtn
+
 extern "C" {
+
+int32_t qtn_contract_pair_dev_noalloc(qtn_ctx* ctx, const double* A,
+                                      int32_t ndA, const int64_t* dimsA,
+                                      const double* B, int32_t ndB,
+                                      const int64_t* dimsB, int32_t nc,
+                                      const int32_t* legsA,
+                                      const int32_t* legsB,
+                                      const int32_t* outperm,
+                                      double* C) try {
+  qtn::clear_error();
+  QTN_ARG(ctx != nullptr, "qtn_contract_pair_dev_noalloc: ctx is NULL");
+  QTN_ARG((ndA == 0 || dimsA) && (ndB == 0 || dimsB) &&
+              (nc == 0 || (legsA && legsB)),
+          "qtn_contract_pair_dev_noalloc: NULL argument");
+  qtn::PairPlan pl;
+  QTN_TRY(qtn::make_plan(ndA, dimsA, ndB, dimsB, nc, legsA, legsB, outperm,
+                         pl));
+  int64_t nA, nB, nC;
+  qtn::plan_scratch(pl, &nA, &nB, &nC);
+  // each piece starts 256-byte aligned (16 complex doubles)
+  const int64_t rA = (nA + 15) / 16 * 16, rB = (nB + 15) / 16 * 16,
+                rC = (nC + 15) / 16 * 16;
+  QTN_ARG(rA + rB + rC <= kNoAllocScratchElems,
+          "qtn_contract_pair_dev_noalloc: the contraction needs more "
+          "scratch than the 32 MiB this entry point has (permuted operands "
+          "and product); use qtn_contract_pair_dev");
+  void* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(qtn::g_scratch_mu);
+    base = qtn::g_scratch_addr[ctx->device];
+  }
+  QTN_ARG(base != nullptr, "qtn_contract_pair_dev_noalloc: no scratch for "
+                           "this device (context not created by "
+                           "qtn_ctx_create)");
+  QTN_CHECK_HIP(hipSetDevice(ctx->device));
+  double2* sA = (double2*)base;
+  double2* sB = sA + rA;
+  double2* sC = sB + rB;
+  return qtn::run_plan_with(ctx->stream, pl, (const double2*)A, dimsA,
+                            (const double2*)B, dimsB, nA ? sA : nullptr,
+                            nB ? sB : nullptr, nC ? sC : nullptr,
+                            (double2*)C);
+} catch (...) {
+  return qtn::abi_exception();
+}
 
 int32_t qtn_contract_pair_dev(qtn_ctx* ctx, const double* A, int32_t ndA,
                               const int64_t* dimsA, const double* B,
@@ -350,8 +450,8 @@ int32_t qtn_contract_pair_dev(qtn_ctx* ctx, const double* A, int32_t ndA,
   QTN_TRY(qtn::make_plan(ndA, dimsA, ndB, dimsB, nc, legsA, legsB, outperm,
                          pl));
   QTN_CHECK_HIP(hipSetDevice(ctx->device));
-  return qtn::run_plan(ctx, pl, (const double2*)A, ndA, dimsA,
-                       (const double2*)B, ndB, dimsB, (double2*)C);
+  return qtn::run_plan(ctx, pl, (const double2*)A, dimsA, (const double2*)B,
+                       dimsB, (double2*)C);
 } catch (...) {
   return qtn::abi_exception();
 }
@@ -383,8 +483,8 @@ int32_t qtn_contract_pair(qtn_ctx* ctx, const double* A, int32_t ndA,
                                hipMemcpyHostToDevice, ctx->stream));
   QTN_CHECK_HIP(hipMemcpyAsync(dB, B, nB * sizeof(double2),
                                hipMemcpyHostToDevice, ctx->stream));
-  QTN_TRY(qtn::run_plan(ctx, pl, (const double2*)dA, ndA, dimsA,
-                        (const double2*)dB, ndB, dimsB, (double2*)dC));
+  QTN_TRY(qtn::run_plan(ctx, pl, (const double2*)dA, dimsA,
+                        (const double2*)dB, dimsB, (double2*)dC));
   QTN_CHECK_HIP(hipMemcpyAsync(C, dC, nC * sizeof(double2),
                                hipMemcpyDeviceToHost, ctx->stream));
   QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));

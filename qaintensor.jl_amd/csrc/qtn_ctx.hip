@@ -102,6 +102,7 @@ int32_t qtn_ctx_create(int32_t device, qtn_ctx** ctx) try {
           "qtn_ctx_create: device " + std::to_string(device) +
               " out of range (" + std::to_string(ndev) + " devices)");
   QTN_CHECK_HIP(hipSetDevice(device));
+  QTN_TRY(qtn::noalloc_scratch_init(device));
   qtn_ctx* c = new (std::nothrow) qtn_ctx();
   if (!c) {
     qtn::set_error("qtn_ctx_create: host allocation failed");

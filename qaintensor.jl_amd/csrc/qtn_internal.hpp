@@ -103,6 +103,10 @@ int32_t launch_zgemm_bf16x3(qtn_ctx* ctx, int opA, int64_t m, int64_t n,
                             const double2* B, int64_t ldb, double2* C,
                             int64_t ldc);
 
+// Looks up this GPU's copy of the no-allocation scratch (qtn_contract.hip);
+// called once per device with it current (qtn_ctx_create).
+int32_t noalloc_scratch_init(int device);
+
 // Rank-predictor MLP (qtn_mlp.hip): prediction on device features, Y
 // points into the MLP's own scratch.
 int32_t mlp_predict_device(qtn_mlp* m, int64_t nb, const float* X,

@@ -32,7 +32,7 @@ EXPORTS = [
     "qtn_ctx_counters", "qtn_ctx_profile", "qtn_ctx_profile_read",
     "qtn_last_error", "qtn_version",
     "qtn_contract_pair", "qtn_contract_pair_dev", "qtn_permute",
-    "qtn_contract_network",
+    "qtn_contract_network", "qtn_contract_pair_dev_noalloc",
     "qtn_permute_dev", "qtn_zgemm_dev", "qtn_cgemm_dev",
     "qtn_svd_trunc", "qtn_svd", "qtn_svd_values", "qtn_contract_svd",
     "qtn_mps_create", "qtn_mps_destroy", "qtn_mps_copy", "qtn_mps_set_sites",
@@ -97,6 +97,9 @@ def _declare(lib):
                                          _pi64, _i32, _pi32, _pi32, _pi32,
                                          _vp]),
         "qtn_permute": (_i32, [_vp, _pd, _i32, _pi64, _pi32, _pd]),
+        "qtn_contract_pair_dev_noalloc": (_i32, [_vp, _vp, _i32, _pi64, _vp,
+                                                 _i32, _pi64, _i32, _pi32,
+                                                 _pi32, _pi32, _vp]),
         "qtn_contract_network": (_i32, [_vp, _i32, _pi32, _pi64, _pi32,
                                         ctypes.POINTER(_pd), _i32, _pi32,
                                         _pd, _i64]),
