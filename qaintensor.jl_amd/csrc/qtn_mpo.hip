@@ -41,13 +41,14 @@ unsigned grid_for(int64_t n) {
                                                           8192));
 }
 
-// U[:, j] *= S[j] (m x k, ld m)
-__global__ void scale_cols(double2* __restrict__ U, int64_t m, int64_t k,
-                           const double* __restrict__ S) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m * k;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const double s = S[i / m];
-    U[i] = make_double2(U[i].x * s, U[i].y * s);
+// out (k x rows, ld k) = V^H for V rows x k (ld rows).
+__global__ void adjoint_into(const double2* __restrict__ V, int64_t rows,
+                             int64_t k, double2* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+       i < rows * k; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = i % k, r = i / k;
+    const double2 v = V[r + j * rows];
+    out[i] = make_double2(v.x, -v.y);
   }
 }
 
@@ -131,23 +132,22 @@ int32_t right_sweep(qtn_mps* m, std::vector<double2*>& work,
   nd[n] = 1;
   for (int j = n - 1; j >= 1; --j) {
     const int64_t a = wd[j], b = nd[j + 1];
-    // A_j as a x (2 b)
+    // A_j as a x (2 b): its right vectors V_k (2b x k) only, then
+    // U_k S_k = A_j V_k by a GEMM
     int64_t k = 0;
     const int64_t cap = std::min<int64_t>(chimax > 0 ? chimax : m->cap,
                                           m->maxb[j]);
-    QTN_TRY(svd_device(ctx, work[j], a, 2 * b, a, true, cutoff, 1, cap, 0,
-                       (double2*)dU, (double*)dS, (double2*)dVh, &k,
-                       nullptr));
+    QTN_TRY(svd_device_left(ctx, work[j], a, 2 * b, a, 1, cutoff, 1, cap,
+                            (double2*)dVh, (double*)dS, &k, nullptr));
     // B_j = V_k^H (k x 2b) straight into the arena (ld k)
-    QTN_CHECK_HIP(hipMemcpyAsync(m->site[j], dVh,
-                                 (size_t)k * 2 * b * sizeof(double2),
-                                 hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(adjoint_into, dim3(grid_for(2 * b * k)), dim3(256), 0,
+                       s, (const double2*)dVh, 2 * b, k, m->site[j]);
     hipLaunchKernelGGL(normalize_lambda, dim3(1), dim3(256), 0, s,
                        (const double*)dS, k, m->lam[j]);
-    // A_{j-1} <- A_{j-1} (U_k S_k): (2 wd[j-1]) x a times a x k
-    hipLaunchKernelGGL(scale_cols, dim3(grid_for(a * k)), dim3(256), 0, s,
-                       (double2*)dU, a, k, (const double*)dS);
     QTN_CHECK_HIP(hipGetLastError());
+    // A_{j-1} <- A_{j-1} (U_k S_k): (2 wd[j-1]) x a times a x k
+    QTN_TRY(gemm(s, 0, 0, a, k, 2 * b, work[j], a, (const double2*)dVh,
+                 2 * b, (double2*)dU, a));
     const int64_t pa = wd[j - 1];
     QTN_TRY(ctx->buffer(kSlotT1, (size_t)2 * pa * k * sizeof(double2), &dT));
     QTN_TRY(gemm(s, 0, 0, 2 * pa, k, a, work[j - 1], 2 * pa,
@@ -458,6 +458,7 @@ int32_t qtn_mps_apply_mpo_form(const qtn_mps* in, const qtn_mpo* w,
   QTN_CHECK_HIP(hipMemcpyAsync(dC, &one, sizeof(double2),
                                hipMemcpyHostToDevice, s));
   int64_t k = 1;
+  bool chosen_off = false;
   double2* C = (double2*)dC;
   double2* T1 = (double2*)dT1;
   double2* T2 = (double2*)dT2;
@@ -487,17 +488,31 @@ int32_t qtn_mps_apply_mpo_form(const qtn_mps* in, const qtn_mpo* w,
     }
     int64_t kn = 0;
     const int64_t kcap = std::min(cm, cap[j + 1]);
-    if (form == QTN_FORM_CHOSEN)
+    // The chosen form tries FP32 sweeps on the splits the cap decides
+    // until one fails its a-posteriori check (a spectrum flat past the
+    // cap); the remaining splits of the zip-up then go straight to FP64, as
+    // the following bonds of such a state have equally flat spectra.
+    bool fp64 = form != QTN_FORM_CHOSEN || chosen_off;
+    if (!fp64) {
       QTN_TRY(svd_device_chosen(ctx, T2, 2 * k, cr * Dr, 2 * k, cutoff, 1,
                                 kcap, 1, (double2*)dU, (double*)dS,
-                                (double2*)dVh, &kn, nullptr));
-    else
-      QTN_TRY(svd_device(ctx, T2, 2 * k, cr * Dr, 2 * k, true, cutoff, 1,
-                         kcap, 1, (double2*)dU, (double*)dS, (double2*)dVh,
-                         &kn, nullptr));
-    QTN_CHECK_HIP(hipMemcpyAsync(work[j], dU,
-                                 (size_t)2 * k * kn * sizeof(double2),
-                                 hipMemcpyDeviceToDevice, s));
+                                (double2*)dVh, &kn, nullptr, &fp64));
+      if (!fp64)
+        QTN_CHECK_HIP(hipMemcpyAsync(work[j], dU,
+                                     (size_t)2 * k * kn * sizeof(double2),
+                                     hipMemcpyDeviceToDevice, s));
+      else if (std::min(2 * k, cr * Dr) > kcap)
+        chosen_off = true;
+    }
+    if (fp64) {
+      // U straight into the work tensor (left vectors only: no rotations
+      // accumulated, and a wide split's sweeps run on its square R^H),
+      // then S V^H = U^H T2
+      QTN_TRY(svd_device_left(ctx, T2, 2 * k, cr * Dr, 2 * k, 0, cutoff, 1,
+                              kcap, work[j], (double*)dS, &kn, nullptr));
+      QTN_TRY(gemm(s, 2, 0, kn, cr * Dr, 2 * k, work[j], 2 * k, T2, 2 * k,
+                   (double2*)dVh, kn));
+    }
     wd[j] = k;
     // C' = (S V^H) as (k', b, e) -> (k', e, b)
     QTN_TRY(permute(s, (const double2*)dVh, {kn, cr, Dr}, {0, 2, 1}, C));

@@ -29,23 +29,6 @@ __device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // conj(a)*b
   return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
 }
 
-// Block-wide sum of `n` complex partials per thread into red[0..n).
-template <int N>
-__device__ void block_sum(double2 (&part)[N], double2* red) {
-  // red: [N][256] staging in LDS (caller provides N*256 double2)
-  for (int i = 0; i < N; ++i) red[i * 256 + threadIdx.x] = part[i];
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-      for (int i = 0; i < N; ++i) {
-        const double2 a = red[i * 256 + threadIdx.x];
-        const double2 b = red[i * 256 + threadIdx.x + w];
-        red[i * 256 + threadIdx.x] = make_double2(a.x + b.x, a.y + b.y);
-      }
-    __syncthreads();
-  }
-}
-
 // Column norms^2 of every column of A (rows x cols used, ld lda).
 // grid (cols_p/4, batch); one wave per column.
 __global__ __launch_bounds__(256) void qr_colnorm2(
@@ -121,176 +104,9 @@ __global__ void qr_gather_cols(const double2* __restrict__ src,
   }
 }
 
-// Factor panel k of every active problem (one workgroup per problem).
-// Writes the reflectors below the diagonal of A, R on and above it, the
-// explicit V (rows x NB, unit diagonal, zeros above) into Vb, V T^H into
-// VTh and V T into VTn (both rows x NB), all with ld = lda.
-__global__ __launch_bounds__(256) void qr_panel(
-    double2* __restrict__ A, int64_t lda, int64_t sA, int k,
-    const QrDims* __restrict__ dims, double2* __restrict__ Vb,
-    double2* __restrict__ VTh, double2* __restrict__ VTn, int64_t sV,
-    int64_t sVh) {
-  const int bz = blockIdx.x;
-  const QrDims d = dims[bz];
-  const int64_t r0 = (int64_t)k * NB;
-  if (!d.active || r0 >= d.cols) return;
-  const int64_t rows = d.rows;
-  const int ncp = (int)min((int64_t)NB, d.cols - r0);
-  A += (int64_t)bz * sA;
-  Vb += (int64_t)bz * sV;   // per-problem stride of the kept panels
-  VTh += (int64_t)bz * sVh; // one panel per problem
-  VTn += (int64_t)bz * sV;
-  const int tid = threadIdx.x;
-  __shared__ double2 red[NB * 256];
-  __shared__ double2 tau_s[NB];
-  __shared__ double2 scale_s;
-  __shared__ double beta_s;
-  __shared__ double2 T[NB][NB + 1];
-  __shared__ double2 Gv[NB][NB + 1];
-
-  for (int j = 0; j < ncp; ++j) {
-    const int64_t col = r0 + j;
-    double2* a = A + col * lda;
-    // 1. |x_rest|^2 with x = A[col:rows, col]
-    double s = 0.0;
-    for (int64_t r = col + 1 + tid; r < rows; r += 256) {
-      const double2 v = a[r];
-      s += v.x * v.x + v.y * v.y;
-    }
-    {
-      double2 part[1] = {make_double2(s, 0.0)};
-      block_sum<1>(part, red);
-    }
-    if (tid == 0) {
-      const double xn2 = red[0].x;
-      const double2 alpha = col < rows ? a[col] : make_double2(0.0, 0.0);
-      if (xn2 == 0.0 && alpha.y == 0.0) {
-        // H = I (zlarfg with a real alpha and a zero tail)
-        tau_s[j] = make_double2(0.0, 0.0);
-        scale_s = make_double2(0.0, 0.0);
-        beta_s = alpha.x;
-      } else {
-        const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
-        const double beta = alpha.x >= 0.0 ? -nrm : nrm;
-        tau_s[j] = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
-        // scale = 1 / (alpha - beta)
-        const double2 den = make_double2(alpha.x - beta, alpha.y);
-        const double dd = den.x * den.x + den.y * den.y;
-        scale_s = make_double2(den.x / dd, -den.y / dd);
-        beta_s = beta;
-      }
-    }
-    __syncthreads();
-    const double2 tau = tau_s[j], scl = scale_s;
-    // 2. v = x_rest * scale, A[col][col] = beta
-    for (int64_t r = col + 1 + tid; r < rows; r += 256) a[r] = cmul(a[r], scl);
-    __syncthreads();
-    if (tid == 0 && col < rows) a[col] = make_double2(beta_s, 0.0);
-    // 3. apply H^H = I - conj(tau) v v^H to the rest of the panel
-    const int nrest = ncp - 1 - j;
-    if (nrest > 0 && (tau.x != 0.0 || tau.y != 0.0)) {
-      double2 part[NB];
-#pragma unroll
-      for (int c = 0; c < NB; ++c) part[c] = make_double2(0.0, 0.0);
-      for (int64_t r = col + tid; r < rows; r += 256) {
-        const double2 v = (r == col) ? make_double2(1.0, 0.0) : a[r];
-#pragma unroll
-        for (int c = 0; c < NB; ++c)
-          if (c < nrest) {
-            const double2 x = A[(col + 1 + c) * lda + r];
-            const double2 p = cmulc(v, x);
-            part[c].x += p.x;
-            part[c].y += p.y;
-          }
-      }
-      block_sum<NB>(part, red);
-      // w_c = v^H x_c; x_c -= conj(tau) * v * w_c
-      for (int64_t r = col + tid; r < rows; r += 256) {
-        const double2 v = (r == col) ? make_double2(1.0, 0.0) : a[r];
-        for (int c = 0; c < nrest; ++c) {
-          const double2 w = red[c * 256];
-          const double2 f = cmulc(tau, w);  // conj(tau) * w
-          const double2 u = cmul(v, f);
-          double2* xp = &A[(col + 1 + c) * lda + r];
-          xp->x -= u.x;
-          xp->y -= u.y;
-        }
-      }
-      __syncthreads();
-    }
-  }
-  // explicit V (rows x NB) for this panel
-  for (int64_t idx = tid; idx < rows * NB; idx += 256) {
-    const int64_t r = idx % rows;
-    const int j = (int)(idx / rows);
-    double2 v = make_double2(0.0, 0.0);
-    if (j < ncp) {
-      if (r == r0 + j)
-        v = make_double2(1.0, 0.0);
-      else if (r > r0 + j)
-        v = A[(r0 + j) * lda + r];
-    }
-    Vb[r + j * lda] = v;
-  }
-  __syncthreads();
-  // Gv = V^H V (NB x NB), thread (i, j) accumulates over all rows
-  {
-    const int i = tid % NB, j = tid / NB;
-    double2 acc = make_double2(0.0, 0.0);
-    for (int64_t r = r0; r < rows; ++r) {
-      const double2 p = cmulc(Vb[r + i * lda], Vb[r + j * lda]);
-      acc.x += p.x;
-      acc.y += p.y;
-    }
-    Gv[i][j] = acc;
-  }
-  __syncthreads();
-  // T (zlarft, forward columnwise): T[j][j] = tau_j,
-  // T[0:j, j] = -tau_j * T[0:j, 0:j] * (V[:, 0:j]^H v_j)
-  if (tid < NB) {
-    for (int c = 0; c < NB; ++c) T[tid][c] = make_double2(0.0, 0.0);
-  }
-  __syncthreads();
-  for (int j = 0; j < ncp; ++j) {
-    const double2 tj = tau_s[j];
-    // thread i < j computes row i of the new column
-    double2 val = make_double2(0.0, 0.0);
-    if (tid < j) {
-      for (int l = tid; l < j; ++l) {  // T upper triangular: T[tid][l], l>=tid
-        const double2 p = cmul(T[tid][l], Gv[l][j]);
-        val.x += p.x;
-        val.y += p.y;
-      }
-      const double2 q = cmul(tj, val);
-      val = make_double2(-q.x, -q.y);
-    }
-    __syncthreads();
-    if (tid < j) T[tid][j] = val;
-    if (tid == j) T[j][j] = tj;
-    __syncthreads();
-  }
-  // VTh = V T^H, VTn = V T
-  for (int64_t idx = tid; idx < rows * NB; idx += 256) {
-    const int64_t r = idx % rows;
-    const int j = (int)(idx / rows);
-    double2 ah = make_double2(0.0, 0.0), an = make_double2(0.0, 0.0);
-    for (int i = 0; i < NB; ++i) {
-      const double2 v = Vb[r + i * lda];
-      const double2 h = cmul(v, make_double2(T[j][i].x, -T[j][i].y));
-      const double2 n = cmul(v, T[i][j]);
-      ah.x += h.x;
-      ah.y += h.y;
-      an.x += n.x;
-      an.y += n.y;
-    }
-    VTh[r + j * lda] = ah;
-    if (VTn) VTn[r + j * lda] = an;
-  }
-}
-
 // Wave-shuffle then LDS sum of N complex partials per thread; every thread
-// gets the totals in out[].  scratch: 4 waves x N double2 in LDS.
-template <int N>
+// gets the totals in out[].  scratch: NW waves x N double2 in LDS.
+template <int N, int NW = 4>
 __device__ __forceinline__ void block_sum_fast(double2 (&part)[N],
                                                double2 (&out)[N],
                                                double2* scratch) {
@@ -331,7 +147,7 @@ __device__ __forceinline__ void block_sum_fast(double2 (&part)[N],
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     double2 t = scratch[i];
-    for (int w = 1; w < 4; ++w) {
+    for (int w = 1; w < NW; ++w) {
       t.x += scratch[w * N + i].x;
       t.y += scratch[w * N + i].y;
     }
@@ -340,35 +156,51 @@ __device__ __forceinline__ void block_sum_fast(double2 (&part)[N],
   __syncthreads();
 }
 
-// qr_panel with the panel (rows - r0 <= kLdsRows rows x NB) held in LDS.
-// Per column one norm reduction and one 16-wide reduction that yields both
-// w_c = v_j^H x_c (c > j, for the reflector application) and
-// Gv[i][j] = v_i^H v_j (i < j, for T).
+// Factor panel k of every active problem (one workgroup of NT threads per
+// problem).  Writes the reflectors below the diagonal of A, R on and above
+// it, the explicit V (rows x NB, unit diagonal, zeros above) into Vb, V T^H
+// into VTh and V T into VTn (both rows x NB), all with ld = lda.
+// kLds: the panel (rows - r0 <= kLdsRows rows x NB) is held in LDS;
+// otherwise it is updated in place in A (the workgroup's own stores are
+// visible to it after each barrier).  Per column one norm reduction and one
+// 16-wide reduction that yields both w_c = v_j^H x_c (c > j, for the
+// reflector application) and Gv[i][j] = v_i^H v_j (i < j, for T).
 constexpr int kLdsRows = 512;
-__global__ __launch_bounds__(256) void qr_panel_lds(
+template <bool kLds, int NT>
+__global__ __launch_bounds__(NT) void qr_panel_t(
     double2* __restrict__ A, int64_t lda, int64_t sA, int k,
     const QrDims* __restrict__ dims, double2* __restrict__ Vb,
     double2* __restrict__ VTh, double2* __restrict__ VTn, int64_t sV,
     int64_t sVh) {
+  constexpr int NW = NT / 64;
   const int bz = blockIdx.x;
   const QrDims d = dims[bz];
   const int64_t r0 = (int64_t)k * NB;
   if (!d.active || r0 >= d.cols) return;
-  const int rows = (int)(d.rows - r0);  // rows of the panel, <= kLdsRows
+  const int rows = (int)(d.rows - r0);  // rows of the panel
   const int ncp = (int)min((int64_t)NB, d.cols - r0);
   A += (int64_t)bz * sA + r0 + r0 * lda;  // panel origin
   Vb += (int64_t)bz * sV;
   VTh += (int64_t)bz * sVh;
   VTn += (int64_t)bz * sV;
   const int tid = threadIdx.x;
-  __shared__ double2 P[NB][kLdsRows + 1];  // P[col][row], panel-local
-  __shared__ double2 scratch[4 * NB];
+  __shared__ double2 P[kLds ? NB : 1][kLds ? kLdsRows + 1 : 1];
+  __shared__ double2 scratch[NW * NB];
   __shared__ double2 tau_s[NB];
   __shared__ double2 T[NB][NB + 1];
   __shared__ double2 Gv[NB][NB + 1];
-  for (int idx = tid; idx < NB * rows; idx += 256) {
-    const int r = idx % rows, c = idx / rows;
-    P[c][r] = c < ncp ? A[r + (int64_t)c * lda] : make_double2(0.0, 0.0);
+  // panel entry (column c, panel row r)
+  auto at = [&](int c, int r) -> double2& {
+    if constexpr (kLds)
+      return P[c][r];
+    else
+      return A[r + (int64_t)c * lda];
+  };
+  if constexpr (kLds) {
+    for (int idx = tid; idx < NB * rows; idx += NT) {
+      const int r = idx % rows, c = idx / rows;
+      P[c][r] = c < ncp ? A[r + (int64_t)c * lda] : make_double2(0.0, 0.0);
+    }
   }
   if (tid < NB)
     for (int c = 0; c < NB; ++c) {
@@ -378,14 +210,14 @@ __global__ __launch_bounds__(256) void qr_panel_lds(
   __syncthreads();
   for (int j = 0; j < ncp; ++j) {
     double2 part[1] = {make_double2(0.0, 0.0)};
-    for (int r = j + 1 + tid; r < rows; r += 256) {
-      const double2 v = P[j][r];
+    for (int r = j + 1 + tid; r < rows; r += NT) {
+      const double2 v = at(j, r);
       part[0].x += v.x * v.x + v.y * v.y;
     }
     double2 tot[1];
-    block_sum_fast<1>(part, tot, scratch);
+    block_sum_fast<1, NW>(part, tot, scratch);
     const double xn2 = tot[0].x;
-    const double2 alpha = P[j][j];
+    const double2 alpha = at(j, j);
     double2 tau, scl;
     double beta;
     if (xn2 == 0.0 && alpha.y == 0.0) {
@@ -401,30 +233,30 @@ __global__ __launch_bounds__(256) void qr_panel_lds(
       scl = make_double2(den.x / dd, -den.y / dd);
     }
     if (tid == 0) tau_s[j] = tau;
-    for (int r = j + 1 + tid; r < rows; r += 256) P[j][r] = cmul(P[j][r], scl);
+    for (int r = j + 1 + tid; r < rows; r += NT) at(j, r) = cmul(at(j, r), scl);
     __syncthreads();
-    if (tid == 0) P[j][j] = make_double2(beta, 0.0);
-    // c > j: w_c = v_j^H x_c ; c < j: g_c = v_c^H v_j (v_c lives in P[c]
-    // below its diagonal, and v_j is zero above row j)
+    if (tid == 0) at(j, j) = make_double2(beta, 0.0);
+    // c > j: w_c = v_j^H x_c ; c < j: g_c = v_c^H v_j (v_c lives in column
+    // c below its diagonal, and v_j is zero above row j)
     double2 wp[NB], w[NB];
 #pragma unroll
     for (int c = 0; c < NB; ++c) wp[c] = make_double2(0.0, 0.0);
-    for (int r = j + tid; r < rows; r += 256) {
-      const double2 v = (r == j) ? make_double2(1.0, 0.0) : P[j][r];
+    for (int r = j + tid; r < rows; r += NT) {
+      const double2 v = (r == j) ? make_double2(1.0, 0.0) : at(j, r);
 #pragma unroll
       for (int c = 0; c < NB; ++c) {
         if (c > j && c < ncp) {
-          const double2 pr = cmulc(v, P[c][r]);
+          const double2 pr = cmulc(v, at(c, r));
           wp[c].x += pr.x;
           wp[c].y += pr.y;
         } else if (c < j) {
-          const double2 pr = cmulc(P[c][r], v);
+          const double2 pr = cmulc(at(c, r), v);
           wp[c].x += pr.x;
           wp[c].y += pr.y;
         }
       }
     }
-    block_sum_fast<NB>(wp, w, scratch);
+    block_sum_fast<NB, NW>(wp, w, scratch);
     if (tid < j) {
       double2 g = make_double2(0.0, 0.0);
 #pragma unroll
@@ -433,23 +265,26 @@ __global__ __launch_bounds__(256) void qr_panel_lds(
       Gv[tid][j] = g;
     }
     if (tau.x != 0.0 || tau.y != 0.0) {
-      for (int r = j + tid; r < rows; r += 256) {
-        const double2 v = (r == j) ? make_double2(1.0, 0.0) : P[j][r];
+      for (int r = j + tid; r < rows; r += NT) {
+        const double2 v = (r == j) ? make_double2(1.0, 0.0) : at(j, r);
 #pragma unroll
         for (int c = 0; c < NB; ++c)
           if (c > j && c < ncp) {
             const double2 u = cmul(v, cmulc(tau, w[c]));
-            P[c][r].x -= u.x;
-            P[c][r].y -= u.y;
+            double2& x = at(c, r);
+            x.x -= u.x;
+            x.y -= u.y;
           }
       }
     }
     __syncthreads();
   }
-  // write R / reflectors back
-  for (int idx = tid; idx < NB * rows; idx += 256) {
-    const int r = idx % rows, c = idx / rows;
-    if (c < ncp) A[r + (int64_t)c * lda] = P[c][r];
+  if constexpr (kLds) {
+    // write R / reflectors back
+    for (int idx = tid; idx < NB * rows; idx += NT) {
+      const int r = idx % rows, c = idx / rows;
+      if (c < ncp) A[r + (int64_t)c * lda] = P[c][r];
+    }
   }
   // T (zlarft): column j = -tau_j T[0:j,0:j] Gv[0:j, j], T[j][j] = tau_j.
   // Row tid of column j only needs T[tid][l] for tid <= l < j.  Barriers
@@ -485,7 +320,7 @@ __global__ __launch_bounds__(256) void qr_panel_lds(
       const double2 t = T[j][i];
       trow[i] = make_double2(t.x, -t.y);  // conj(T[j][i])
     }
-    for (int64_t r = part; r < d.rows; r += 256 / NB) {
+    for (int64_t r = part; r < d.rows; r += NT / NB) {
       const int64_t rl = r - r0;
       double2 ah = make_double2(0.0, 0.0), an = make_double2(0.0, 0.0);
       double2 vj = make_double2(0.0, 0.0);
@@ -494,7 +329,7 @@ __global__ __launch_bounds__(256) void qr_panel_lds(
         for (int i = 0; i < NB; ++i) {
           double2 v = make_double2(0.0, 0.0);
           if (i < ncp && rl >= i)
-            v = (rl == i) ? make_double2(1.0, 0.0) : P[i][rl];
+            v = (rl == i) ? make_double2(1.0, 0.0) : at(i, (int)rl);
           if (i == j) vj = v;
           const double2 h = cmul(v, trow[i]);
           const double2 n = cmul(v, tcol[i]);
@@ -629,12 +464,13 @@ int32_t BatchedQR::factor(qtn_ctx* ctx, double2* A, int64_t lda, int64_t sA,
     double2* Vk = V + (keep_q ? (int64_t)k * vs : 0);
     double2* VTnk = VTn + (keep_q ? (int64_t)k * vs : 0);
     if (maxr - (int64_t)k * NB <= kLdsRows)
-      hipLaunchKernelGGL(qr_panel_lds, dim3(batch), dim3(256), 0, s, A, lda,
-                         sA, k, (const QrDims*)ddims, Vk, VTh, VTnk, vstride,
-                         vs);
+      hipLaunchKernelGGL((qr_panel_t<true, 256>), dim3(batch), dim3(256), 0,
+                         s, A, lda, sA, k, (const QrDims*)ddims, Vk, VTh,
+                         VTnk, vstride, vs);
     else
-      hipLaunchKernelGGL(qr_panel, dim3(batch), dim3(256), 0, s, A, lda, sA,
-                         k, (const QrDims*)ddims, Vk, VTh, VTnk, vstride, vs);
+      hipLaunchKernelGGL((qr_panel_t<false, 512>), dim3(batch), dim3(512), 0,
+                         s, A, lda, sA, k, (const QrDims*)ddims, Vk, VTh,
+                         VTnk, vstride, vs);
     QTN_CHECK_HIP(hipGetLastError());
     if (maxn[k] == 0) continue;
     GemmDesc<double>* dk = dall + (size_t)k * 2 * batch;

@@ -116,16 +116,27 @@ inline int batch_class(int n) {
 }
 
 // SVD of a device matrix (defined in qtn_svd_api.hip; contract documented
-// there).  Uses context slots 12 and 13.
+// there).  Uses context slots 12 and 13, and 118-139 for the
+// QR-preconditioned sweeps of splits with 512 or more columns (opt-in with
+// QTN_SVD_PRECOND=1, and only when allow_pre).
 int32_t svd_device(qtn_ctx* ctx, const double2* dA, int64_t m, int64_t n,
                    int64_t lda, bool truncate, double cutoff, int relative,
                    int64_t chimax, int absorb, double2* dU, double* dS,
-                   double2* dVh, int64_t* k_out, double* disc_out);
+                   double2* dVh, int64_t* k_out, double* disc_out,
+                   bool allow_pre = true);
+// Left singular vectors and values of A (adj == 0) or A^H (adj != 0) only
+// (the caller forms the other factor by a GEMM); contract in
+// qtn_svd_api.hip.  Uses slots 12, 15 and 118-139.
+int32_t svd_device_left(qtn_ctx* ctx, const double2* dA, int64_t m,
+                        int64_t n, int64_t lda, int adj, double cutoff,
+                        int relative, int64_t chimax, double2* dU, double* dS,
+                        int64_t* k_out, double* disc_out);
 // The chosen form (FP32 sweeps where the cap decides the truncation, FP64
 // re-orthonormalisation and recomputation); contract in qtn_svd_api.hip.
 int32_t svd_device_chosen(qtn_ctx* ctx, const double2* dA, int64_t m,
                           int64_t n, int64_t lda, double cutoff, int relative,
                           int64_t chimax, int absorb, double2* dU, double* dS,
-                          double2* dVh, int64_t* k_out, double* disc_out);
+                          double2* dVh, int64_t* k_out, double* disc_out,
+                          bool* fallback);
 
 }  // namespace qtn

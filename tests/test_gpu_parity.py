@@ -116,7 +116,70 @@ def test_svd_rank_deficient_and_cap(gpu_ctx):
     assert abs(disc - np.sum(ref[20:] ** 2)) <= 1e-10 * np.sum(ref ** 2)
 
 
-def test_svd_error_paths(gpu_ctx):
+@pytest.mark.parametrize("m,n", [(1300, 700), (700, 1300), (640, 640),
+                                 (3072, 2048)])
+def test_svd_qr_preconditioned_shapes(gpu_ctx, monkeypatch, m, n):
+    """With QTN_SVD_PRECOND=1, splits with min(m, n) >= 512 run the Jacobi
+    sweeps on R2^H of the two-QR preconditioner (sorted columns,
+    X P = Q1 R1, R1^H = Q2 R2); the factors come back through Q1, Q2 and
+    the column permutation.  Values, reconstruction and orthonormality
+    against LAPACK at 1e-10, untruncated and under a cap."""
+    monkeypatch.setenv("QTN_SVD_PRECOND", "1")
+    rng = np.random.default_rng(m + 7 * n)
+    A = crand(rng, m, n)
+    ref = np.linalg.svd(A, compute_uv=False)
+    U, S, Vh, disc = ops.svd_trunc(A, 0.0, False, 0)
+    k = min(m, n)
+    assert S.size == k
+    assert np.abs(S - ref).max() <= TOL * ref[0]
+    assert np.linalg.norm((U * S) @ Vh - A) <= TOL * np.linalg.norm(A)
+    assert np.abs(U.conj().T @ U - np.eye(k)).max() <= TOL
+    assert np.abs(Vh @ Vh.conj().T - np.eye(k)).max() <= TOL
+    cap = k // 3
+    U, S, Vh, disc = ops.svd_trunc(A, 1e-10, True, cap)
+    assert S.size == cap
+    assert np.abs(S - ref[:cap]).max() <= TOL * ref[0]
+    assert abs(disc - np.sum(ref[cap:] ** 2)) <= TOL * np.sum(ref ** 2)
+    # the best rank-cap approximation: its error is the discarded weight
+    err = np.linalg.norm((U * S) @ Vh - A) ** 2
+    assert abs(err - disc) <= 1e-9 * np.sum(ref ** 2)
+
+
+def test_svd_qr_preconditioned_rank_deficient_graded(gpu_ctx, monkeypatch):
+    """Rank 60 of 600 columns with a graded spectrum (1 .. 1e-9), through
+    the QR preconditioner: the relative cutoff keeps exactly the nonzero
+    values, to LAPACK's."""
+    monkeypatch.setenv("QTN_SVD_PRECOND", "1")
+    rng = np.random.default_rng(21)
+    s = np.logspace(0, -9, 60)
+    Q1 = np.linalg.qr(crand(rng, 900, 60))[0]
+    Q2 = np.linalg.qr(crand(rng, 600, 60))[0]
+    A = (Q1 * s) @ Q2.conj().T
+    for B in (A, A.T.copy()):
+        U, S, Vh, disc = ops.svd_trunc(B, 1e-12, True, 0)
+        assert S.size == 60
+        assert np.abs(S - s).max() <= 1e-12
+        assert np.linalg.norm((U * S) @ Vh - B) <= 1e-12
+        assert np.abs(U.conj().T @ U - np.eye(60)).max() <= TOL
+        assert np.abs(Vh @ Vh.conj().T - np.eye(60)).max() <= TOL
+
+
+@pytest.mark.parametrize("scale", [1e100, 1e-100, 1e160, 1e-160])
+def test_svd_qr_preconditioned_magnitudes(gpu_ctx, monkeypatch, scale):
+    """Entries within 2^+-450 go through the preconditioner unscaled,
+    beyond it the rescaling engine takes the split: both exact relative to
+    the scale."""
+    monkeypatch.setenv("QTN_SVD_PRECOND", "1")
+    rng = np.random.default_rng(13)
+    A = crand(rng, 800, 600)
+    ref = np.linalg.svd(A, compute_uv=False)
+    U, S, Vh, _ = ops.svd_trunc(A * scale, 1e-10, True, 0)
+    assert np.abs(S / scale - ref[:S.size]).max() <= 1e-12 * ref[0]
+    R = (U * (S / scale)) @ Vh
+    assert np.linalg.norm(R - A) <= 1e-11 * np.linalg.norm(A)
+
+
+def test_svd_error_paths(gpu_ctx, monkeypatch):
     A = np.ones((4, 4), dtype=complex)
     with pytest.raises(QtnError, match="Error must be positive") as e:
         ops.svd_trunc(A, -1.0)
@@ -129,6 +192,18 @@ def test_svd_error_paths(gpu_ctx):
     with pytest.raises(QtnError) as e:
         ops.svd_trunc(B, 0.0)
     assert e.value.status == QTN_ERR_NOTFINITE
+    # a split large enough for the QR preconditioner: zero and non-finite
+    # inputs report as on the small ones
+    monkeypatch.setenv("QTN_SVD_PRECOND", "1")
+    with pytest.raises(QtnError) as e:
+        ops.svd_trunc(np.zeros((700, 600), dtype=complex), 0.0)
+    assert e.value.status == QTN_ERR_TRUNC
+    for bad in (np.nan, np.inf):
+        C = np.ones((700, 600), dtype=complex)
+        C[500, 3] = bad
+        with pytest.raises(QtnError) as e:
+            ops.svd_trunc(C, 0.0)
+        assert e.value.status == QTN_ERR_NOTFINITE
 
 
 # -- two-site update / MPS ----------------------------------------------------
