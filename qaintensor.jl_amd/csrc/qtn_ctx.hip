@@ -74,6 +74,16 @@ int32_t qtn_ctx::pinned_buffer(size_t bytes, void** out) {
   return QTN_OK;
 }
 
+int32_t qtn_ctx::side_stream(hipStream_t* out) {
+  if (!side) {
+    QTN_CHECK_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    for (auto& e : side_ev)
+      QTN_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  *out = side;
+  return QTN_OK;
+}
+
 int32_t qtn_ctx::worker(int i, qtn_ctx** out) {
   while ((int)workers.size() <= i) {
     qtn_ctx* w = nullptr;
@@ -129,10 +139,15 @@ int32_t qtn_ctx_destroy(qtn_ctx* ctx) try {
   ctx->workers.clear();
   QTN_CHECK_HIP(hipSetDevice(ctx->device));
   QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->side) QTN_CHECK_HIP(hipStreamSynchronize(ctx->side));
   for (auto& b : ctx->bufs)
     if (b.ptr) QTN_CHECK_HIP(hipFree(b.ptr));
   if (ctx->pinned) QTN_CHECK_HIP(hipHostFree(ctx->pinned));
   for (auto e : ctx->ev_pool) QTN_CHECK_HIP(hipEventDestroy(e));
+  if (ctx->side) {
+    for (auto e : ctx->side_ev) QTN_CHECK_HIP(hipEventDestroy(e));
+    QTN_CHECK_HIP(hipStreamDestroy(ctx->side));
+  }
   QTN_CHECK_HIP(hipStreamDestroy(ctx->stream));
   delete ctx;
   return QTN_OK;

@@ -120,6 +120,12 @@ struct qtn_ctx {
   int device = 0;
   uint64_t sketch_seed = 0x5eed;  // range-finder sketches (learned form)
   hipStream_t stream = nullptr;
+  // Second stream (default priority) and two events for work that overlaps
+  // the main stream inside one call, e.g. the trailing updates of a QR
+  // behind its next panel; created on first use (side_stream()).
+  hipStream_t side = nullptr;
+  hipEvent_t side_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  int32_t side_stream(hipStream_t* out);
   // Jacobi block-pair tiling per (padded column count, FP32 sweeps, batch
   // class of the active problems): set by qtn_tune_jacobi /
   // qtn_set_jacobi_tiling (QTN_TILING_* codes).
