@@ -17,6 +17,7 @@ stays resident in HBM.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
        python bench.py --workload tfim-zipup   (64-site TFIM MPO x MPS)
        python bench.py --workload circuits-batch   (256 mixed circuits)
+       python bench.py --workload qft-contract   (QFT-20 network contract())
 """
 import argparse
 import json
@@ -512,6 +513,105 @@ def bench_learned(args):
     print(json.dumps(line))
 
 
+def bench_qft_contract(args):
+    """contract() of the 20-qubit QFT expectation network (a random
+    closed MPS through the QFT and back onto the MPS, is_decompose false and
+    true): ncon's default order, the treewidth order of
+    optimize_contraction_order! (planned on the host), and copy + plan +
+    contract, each the minimum over repetitions like BenchmarkTools'
+    minimum.  The reference notebook's timings of the same cells
+    (examples/expectation_value_optimization_example.ipynb) are quoted
+    beside them.  Every contraction runs on the GPU in one
+    qtn_contract_network call: host tensors in, the scalar back."""
+    import torch
+    from qaintensor import default_context
+    from qaintensor import network2graph as NG
+    from qaintensor import workloads as Wk
+    from qaintensor.network import contract
+    default_context()
+    n, seed, reps = 20, 7, 5
+    published = {False: {"default_order": 3.004, "treewidth_order": 0.239969,
+                         "copy_plan_contract": 0.619641},
+                 True: {"default_order": 28.279, "treewidth_order": 0.148687,
+                        "copy_plan_contract": 0.698267}}
+
+    def best(fn):
+        out, ts = None, []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return min(ts), out
+
+    forms = {}
+    for dec in (False, True):
+        net0, _ = Wk.qft_expectation_network(n, seed, is_decompose=dec)
+        contract(net0)  # warm-up: scratch buffers and code
+        t_def, v_def = best(lambda: complex(contract(net0)))
+        net = net0.copy()
+        t0 = time.perf_counter()
+        NG.optimize_contraction_order(net)
+        t_plan = time.perf_counter() - t0
+        t_tw, v_tw = best(lambda: complex(contract(net)))
+
+        def copy_plan_contract():
+            m = net0.copy()
+            NG.optimize_contraction_order(m)
+            return complex(contract(m))
+        t_all, _ = best(copy_plan_contract)
+        ref = published[dec]
+        forms["decompose" if dec else "dense_gates"] = {
+            "tensors": len(net0.tensors),
+            "contractions": len(net0.contractions),
+            "seconds": {"default_order": t_def, "treewidth_order": t_tw,
+                        "plan": t_plan, "copy_plan_contract": t_all},
+            "reference_seconds": ref,
+            "speedup_vs_reference": {k: ref[k] / v for k, v in (
+                ("default_order", t_def), ("treewidth_order", t_tw),
+                ("copy_plan_contract", t_all))},
+            "orders_agree_rel": abs(v_tw - v_def) / abs(v_def),
+            "value": [v_tw.real, v_tw.imag]}
+        print("qft-contract decompose=%s: default %.4f s, treewidth %.4f s "
+              "(plan %.3f s), copy+plan+contract %.4f s"
+              % (dec, t_def, t_tw, t_plan, t_all), file=sys.stderr,
+              flush=True)
+    g = forms["dense_gates"]
+    line = {
+        "metric": "contract() of the 20-qubit QFT expectation network, "
+                  "treewidth order",
+        "value": g["seconds"]["treewidth_order"], "unit": "s",
+        "n_gpus": 1, "higher_is_better": False, "dtype": "c128",
+        "data": "synthetic (seeded random closed MPS)",
+        "config": {"workload": "QFT-20 expectation network, contract(net)",
+                   "n_qubits": n},
+        "vs_baseline": None,
+        "forms": forms,
+    }
+    if not args.no_cpu:
+        from oracle import qtn_oracle as O
+        from qaintensor.network import contract_rep
+        net0, _ = Wk.qft_expectation_network(n, seed)
+        net = net0.copy()
+        NG.optimize_contraction_order(net)
+        data, labels = [t.data for t in net.tensors], contract_rep(net)
+        t0 = time.perf_counter()
+        v = complex(O.ncon_ordered(data, labels))
+        t_cpu = time.perf_counter() - t0
+        line["cpu_baseline"] = {
+            "value": t_cpu, "unit": "s per contract()",
+            "cores": int(os.environ.get("OPENBLAS_NUM_THREADS",
+                                        os.environ.get("OMP_NUM_THREADS",
+                                                       "1"))),
+            "kind": "port",
+            "sample": "oracle ncon (numpy pairwise tensordot) of the same "
+                      "network in the same treewidth order, once"}
+        line["cpu_rel_diff"] = abs(v - complex(g["value"][0],
+                                               g["value"][1])) / abs(v)
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -521,7 +621,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="circuit",
                     choices=["circuit", "tfim-zipup", "circuits-batch",
-                             "learned"])
+                             "learned", "qft-contract"])
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, copy a MIB-MiB device buffer 4 "
                          "times (a kernel of known traffic for calibrating "
@@ -546,6 +646,9 @@ def main():
         return
     if args.workload == "learned":
         bench_learned(args)
+        return
+    if args.workload == "qft-contract":
+        bench_qft_contract(args)
         return
 
     import torch

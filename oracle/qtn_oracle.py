@@ -38,6 +38,9 @@ PINNING = {
                     "equals einsum on 4-leg tensors (test_svd.jl:13-25); "
                     "error bound of test_svd.jl:75-81",
     "ncon": "einsum definition; brute-force loops on tiny tensors",
+    "ncon_ordered": "equals ncon (einsum) under several label orders, with "
+                    "traces, parallel legs and an outer product; a chain "
+                    "past einsum's 52 letters equals the matrix product",
     "mps_from_state": "round trip MPS(psi) -> contract == psi "
                       "(test/test_mps.jl:34-49)",
     "mpo_from_matrix": "contract(MPO(U)) == U as a (2,..,2) tensor; "
@@ -137,6 +140,46 @@ def ncon(tensors, indexlists):
     out = "".join(let(l) for l in outs)
     expr = ",".join(subs) + "->" + out
     return np.einsum(expr, *[np.asarray(t) for t in tensors], optimize=True)
+
+
+def ncon_ordered(tensors, indexlists, order=None):
+    """ncon as TensorOperations evaluates it: the positive labels
+    contracted one at a time in `order` (default: increasing), each step a
+    tensordot of the two tensors carrying the label over every label they
+    share (a trace when one tensor carries both ends); tensors left
+    unconnected at the end are joined by outer products in their order;
+    output axes -1, -2, ....  For networks with more labels than einsum's
+    alphabet."""
+    items = [(np.asarray(t), list(l)) for t, l in zip(tensors, indexlists)]
+    if order is None:
+        order = sorted({x for l in indexlists for x in l if x > 0})
+    for lbl in order:
+        holders = [i for i, (_, l) in enumerate(items) if lbl in l]
+        if not holders:
+            continue  # contracted with an earlier label
+        if len(holders) == 1:
+            t, l = items[holders[0]]
+            a, b = [k for k, x in enumerate(l) if x == lbl]
+            t = np.trace(t, axis1=a, axis2=b)
+            items[holders[0]] = (t, [x for k, x in enumerate(l)
+                                     if k not in (a, b)])
+            continue
+        i, j = holders
+        (ta, la), (tb, lb) = items[i], items[j]
+        common = [x for x in la if x > 0 and x in lb]
+        ax_a = [la.index(x) for x in common]
+        ax_b = [lb.index(x) for x in common]
+        t = np.tensordot(ta, tb, axes=(ax_a, ax_b))
+        l = [x for x in la if x not in common] + \
+            [x for x in lb if x not in common]
+        items[i] = (t, l)
+        del items[j]
+    t, l = items[0]
+    for t2, l2 in items[1:]:
+        t = np.multiply.outer(t, t2)
+        l = l + l2
+    outs = sorted([x for x in l if x < 0], reverse=True)
+    return np.transpose(t, [l.index(x) for x in outs]) if outs else t
 
 
 def contract_network(tensors, contractions, openidx):

@@ -9,6 +9,7 @@ import pytest
 
 from oracle import qtn_oracle as O
 from qaintensor import circuits as C, ops
+from qaintensor import network2graph as NG
 from qaintensor import workloads as Wk
 from qaintensor._lib import QTN_ERR_ARG, QtnError
 from qaintensor.mps import ClosedMPS
@@ -83,6 +84,25 @@ def test_qft4_expectation_network_both_orders(gpu_ctx):
         ref = O.ncon([t.data for t in net.tensors], contract_rep(net))
         assert close(contract(net), ref)
         assert close(contract(net, True), ref)
+
+
+@pytest.mark.parametrize("is_decompose", [False, True])
+def test_qft_networks_in_treewidth_order(gpu_ctx, is_decompose):
+    """optimize_contraction_order! (src/network2graph.jl:473-479) then
+    contract(net): the QFT-4 expectation networks of
+    test/test_treewidth.jl:370-377 against ncon in the original order, and
+    QFT-10 against the dense bilinear form."""
+    for n in (4, 10):
+        net0, t0 = Wk.qft_expectation_network(n, 7, is_decompose=is_decompose)
+        net = net0.copy()
+        NG.optimize_contraction_order(net)
+        got = complex(contract(net))
+        if n == 4:
+            ref = complex(O.ncon([t.data for t in net0.tensors],
+                                 contract_rep(net0)))
+        else:
+            ref = dense_expectation(n, t0)
+        assert abs(got - ref) <= 1e-10 * abs(ref)
 
 
 def dense_expectation(n, t0):

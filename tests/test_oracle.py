@@ -522,3 +522,34 @@ def test_apply_2q_long_equals_dense(i, j):
     sv.apply_2q_any(i, j, g)
     v = st.to_statevector()
     assert np.linalg.norm(v - sv.psi) <= 1e-12 * np.linalg.norm(sv.psi)
+
+
+def test_ncon_ordered_matches_einsum_and_long_chains():
+    """ncon_ordered (pairwise tensordot in a label order) against the
+    einsum ncon on a network with a trace, parallel legs and an unconnected
+    vector, under three orders; and a 60-matrix chain (beyond einsum's
+    alphabet) against the plain matrix product."""
+    rng = np.random.default_rng(8)
+
+    def cr(*d):
+        return rng.standard_normal(d) + 1j * rng.standard_normal(d)
+    ts = [cr(3, 4, 5, 2, 2), cr(4, 6), cr(6, 7, 8), cr(7, 3), cr(9),
+          cr(5, 8, 2)]
+    labels = [[4, 1, 6, -2, 5], [1, 2], [2, 3, 7], [3, 4], [-1],
+              [6, 7, 5]]
+    ref = O.ncon(ts, labels)
+    for order in ([1, 2, 3, 4, 5, 6, 7], [7, 6, 5, 4, 3, 2, 1],
+                  [3, 6, 1, 7, 2, 5, 4]):
+        got = O.ncon_ordered(ts, labels, order)
+        assert got.shape == ref.shape
+        assert np.allclose(got, ref, rtol=1e-12, atol=1e-12 * abs(ref).max())
+    tr = cr(3, 5, 3)
+    assert np.allclose(O.ncon_ordered([tr], [[1, -1, 1]]),
+                       np.einsum("aba->b", tr))
+    mats = [cr(2, 2) for _ in range(60)]
+    labels = [[-1, 1]] + [[i, i + 1] for i in range(1, 59)] + [[59, -2]]
+    ref = mats[0]
+    for m in mats[1:]:
+        ref = ref @ m
+    got = O.ncon_ordered(mats, labels)
+    assert np.allclose(got, ref, rtol=1e-10, atol=1e-10 * abs(ref).max())
