@@ -18,6 +18,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
        python bench.py --workload tfim-zipup   (64-site TFIM MPO x MPS)
        python bench.py --workload circuits-batch   (256 mixed circuits)
        python bench.py --workload qft-contract   (QFT-20 network contract())
+       python bench.py --workload kernels   (permute / GEMM microbenchmarks)
 """
 import argparse
 import json
@@ -612,6 +613,107 @@ def bench_qft_contract(args):
     print(json.dumps(line))
 
 
+def copy_probe(mib):
+    """Four device-to-device copies of a MIB-MiB buffer: a kernel of known
+    traffic against which rocprofv3's FETCH_SIZE / WRITE_SIZE are scaled."""
+    import torch
+    nb = mib * (1 << 20)
+    src = torch.ones(nb // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(4):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return nb
+
+
+def bench_kernels(args):
+    """Kernel microbenchmarks on device buffers (the profiler runs read
+    these kernels' counters): permutes of a (1024, 2, 2, 1024) two-site
+    tensor (64 MiB complex FP64) in the orders of the two-site update and a
+    4096 x 2048 transpose, through qtn_permute_dev (permute_gather keeps the
+    unit-stride axis first, permute_tiled transposes it), and complex FP64
+    GEMMs through qtn_zgemm_dev and BF16x3 GEMMs through qtn_zgemm_bf16x3 at
+    n = 2048 and 4096.  Rates are wall-clock over `reps` calls after a
+    warm-up, the context's stream synchronised on both sides (launch gaps
+    included); the profiler's kernel durations give the kernel-only
+    figures."""
+    import ctypes
+    import numpy as np
+    import torch
+    from qaintensor import _lib as L, default_context
+    ctx = default_context()
+    lib = L.load()
+    reps = 20
+
+    def timed(fn, n):
+        fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        ctx.sync()
+        return (time.perf_counter() - t0) / n
+
+    out = {"permute": {}, "zgemm_fp64": {}, "zgemm_bf16x3": {}}
+    for dims, perm in (((1024, 2, 2, 1024), (1, 3, 2, 4)),
+                       ((1024, 2, 2, 1024), (2, 1, 4, 3)),
+                       ((1024, 2, 2, 1024), (4, 2, 3, 1)),
+                       ((4096, 2048), (2, 1))):
+        n_el = int(np.prod(dims))
+        a = torch.randn(2 * n_el, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        d = np.asarray(dims, dtype=np.int64)
+        pm = np.asarray(perm, dtype=np.int32)
+
+        def run():
+            L.check(lib.qtn_permute_dev(ctx.handle,
+                                        ctypes.c_void_p(a.data_ptr()),
+                                        len(dims), L.i64ptr(d), L.i32ptr(pm),
+                                        ctypes.c_void_p(b.data_ptr())))
+        t = timed(run, reps)
+        out["permute"]["%s %s" % (dims, perm)] = {
+            "us": t * 1e6, "GB/s": 2 * 16 * n_el / t / 1e9}
+    one = np.array([1.0, 0.0])
+    zero = np.array([0.0, 0.0])
+    for n in (2048, 4096):
+        A = torch.randn(2 * n * n, dtype=torch.float64, device="cuda:0")
+        B = torch.randn(2 * n * n, dtype=torch.float64, device="cuda:0")
+        Cm = torch.empty_like(A)
+        torch.cuda.synchronize()
+
+        def run64():
+            L.check(lib.qtn_zgemm_dev(
+                ctx.handle, b"N", b"N", n, n, n, L.dptr(one),
+                ctypes.c_void_p(A.data_ptr()), n, 0,
+                ctypes.c_void_p(B.data_ptr()), n, 0, L.dptr(zero),
+                ctypes.c_void_p(Cm.data_ptr()), n, 0, 1))
+        t = timed(run64, 5)
+        out["zgemm_fp64"][str(n)] = {"ms": t * 1e3,
+                                     "TFLOP/s": 8.0 * n ** 3 / t / 1e12}
+        rng = np.random.default_rng(n)
+        Ah = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        Bh = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        from qaintensor import ops
+        ops.zgemm_bf16x3(Ah, Bh)
+        t0 = time.perf_counter()
+        ops.zgemm_bf16x3(Ah, Bh)
+        t = time.perf_counter() - t0
+        out["zgemm_bf16x3"][str(n)] = {
+            "ms_with_host_copies": t * 1e3,
+            "complex_equiv_TFLOP/s_with_host_copies": 8.0 * n ** 3 / t / 1e12}
+    print("kernels: %s" % json.dumps(out), file=sys.stderr, flush=True)
+    line = {"metric": "kernel microbenchmarks (permute GB/s, complex GEMM "
+                      "TFLOP/s)",
+            "value": None, "unit": None, "n_gpus": 1,
+            "data": "synthetic (random device buffers)",
+            "config": {"workload": "permute / zgemm kernels"},
+            "vs_baseline": None, "kernels": out}
+    if args.copy_probe:
+        line["copy_probe_bytes"] = copy_probe(args.copy_probe)
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -621,7 +723,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="circuit",
                     choices=["circuit", "tfim-zipup", "circuits-batch",
-                             "learned", "qft-contract"])
+                             "learned", "qft-contract", "kernels"])
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, copy a MIB-MiB device buffer 4 "
                          "times (a kernel of known traffic for calibrating "
@@ -649,6 +751,9 @@ def main():
         return
     if args.workload == "qft-contract":
         bench_qft_contract(args)
+        return
+    if args.workload == "kernels":
+        bench_kernels(args)
         return
 
     import torch
@@ -793,13 +898,7 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(layers)
     if args.copy_probe:
-        nb = args.copy_probe * (1 << 20)
-        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(4):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        line["copy_probe_bytes"] = nb
+        line["copy_probe_bytes"] = copy_probe(args.copy_probe)
     line["autotune"] = tuned
     print(json.dumps(line))
 
