@@ -84,6 +84,43 @@ int32_t qtn_ctx::side_stream(hipStream_t* out) {
   return QTN_OK;
 }
 
+int32_t qtn_ctx::sweep_graph(const void* key, size_t n, hipStream_t s,
+                             const std::function<int32_t()>& enqueue,
+                             hipGraphExec_t* out) {
+  for (const auto& g : sweep_graphs)
+    if (g.key.size() == n && std::memcmp(g.key.data(), key, n) == 0) {
+      *out = g.exec;
+      return QTN_OK;
+    }
+  QTN_CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int32_t st = enqueue();
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &graph);
+  if (st != QTN_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return st;
+  }
+  QTN_CHECK_HIP(e);
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    QTN_CHECK_HIP(ei);
+  }
+  if (sweep_graphs.size() >= kMaxSweepGraphs) {
+    QTN_CHECK_HIP(hipGraphExecDestroy(sweep_graphs.front().exec));
+    QTN_CHECK_HIP(hipGraphDestroy(sweep_graphs.front().graph));
+    sweep_graphs.erase(sweep_graphs.begin());
+  }
+  CachedGraph c;
+  c.key.assign((const char*)key, (const char*)key + n);
+  c.graph = graph;
+  c.exec = exec;
+  sweep_graphs.push_back(std::move(c));
+  *out = exec;
+  return QTN_OK;
+}
+
 int32_t qtn_ctx::worker(int i, qtn_ctx** out) {
   while ((int)workers.size() <= i) {
     qtn_ctx* w = nullptr;
@@ -144,6 +181,10 @@ int32_t qtn_ctx_destroy(qtn_ctx* ctx) try {
     if (b.ptr) QTN_CHECK_HIP(hipFree(b.ptr));
   if (ctx->pinned) QTN_CHECK_HIP(hipHostFree(ctx->pinned));
   for (auto e : ctx->ev_pool) QTN_CHECK_HIP(hipEventDestroy(e));
+  for (auto& g : ctx->sweep_graphs) {
+    QTN_CHECK_HIP(hipGraphExecDestroy(g.exec));
+    QTN_CHECK_HIP(hipGraphDestroy(g.graph));
+  }
   if (ctx->side) {
     for (auto e : ctx->side_ev) QTN_CHECK_HIP(hipEventDestroy(e));
     QTN_CHECK_HIP(hipStreamDestroy(ctx->side));

@@ -4,6 +4,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <string>
 #include <tuple>
@@ -126,6 +127,19 @@ struct qtn_ctx {
   hipStream_t side = nullptr;
   hipEvent_t side_ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int32_t side_stream(hipStream_t* out);
+  // Instantiated graphs of one Jacobi sweep, keyed by the bytes of every
+  // value its launches take (sweep_graph captures on a miss, evicting the
+  // oldest beyond kMaxSweepGraphs; callers have no graph in flight then).
+  struct CachedGraph {
+    std::vector<char> key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  };
+  static constexpr size_t kMaxSweepGraphs = 32;
+  std::vector<CachedGraph> sweep_graphs;
+  int32_t sweep_graph(const void* key, size_t n, hipStream_t s,
+                      const std::function<int32_t()>& enqueue,
+                      hipGraphExec_t* out);
   // Jacobi block-pair tiling per (padded column count, FP32 sweeps, batch
   // class of the active problems): set by qtn_tune_jacobi /
   // qtn_set_jacobi_tiling (QTN_TILING_* codes).

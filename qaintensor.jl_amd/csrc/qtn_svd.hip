@@ -25,6 +25,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 
 #include <hip/hip_ext.h>
 
@@ -33,6 +34,22 @@
 
 namespace qtn {
 namespace {
+
+// Every value the launches of one sweep take (JacobiEngine::run_t): the
+// key of its cached graph.  Zero-filled before use, so padding compares.
+struct SweepKey {
+  int dbl, w, fused, cross, batch, p, n_inner, nchunk_x, nchunk_w,
+      per_problem_w;
+  int64_t ldx, sx, sw, ldw;
+  double tol, conv2;
+  const void* ptr[11];
+};
+
+// QTN_SWEEP_GRAPHS=0 launches every sweep directly (read at every call).
+bool sweep_graphs_enabled() {
+  const char* e = getenv("QTN_SWEEP_GRAPHS");
+  return !(e && *e && atoi(e) == 0);
+}
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
@@ -1011,10 +1028,14 @@ __global__ __launch_bounds__(256) void jac_round(
 // the sweep was above conv2.  Below it the sweep's own rotations leave the
 // columns orthogonal to O(conv2) (cyclic Jacobi converges quadratically),
 // so the confirming sweep is skipped.
+// misc: [0] problems still active, [1] non-finite input seen, [2] / [3]
+// the sweep's rotated pairs (of X / of X and W); [4..7] receive a snapshot
+// of [0..3] for the host, and [2], [3] restart at zero, so the sweep needs
+// no memset (and can be replayed from a graph).
 __global__ void jac_sweep_end(int* __restrict__ active,
                               int* __restrict__ counters,
                               double* __restrict__ offmax, double conv2,
-                              int batch, int* __restrict__ n_active) {
+                              int batch, int* __restrict__ misc) {
   __shared__ int cnt;
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
@@ -1026,7 +1047,15 @@ __global__ void jac_sweep_end(int* __restrict__ active,
     if (a) atomicAdd(&cnt, 1);
   }
   __syncthreads();
-  if (threadIdx.x == 0) *n_active = cnt;
+  if (threadIdx.x == 0) {
+    misc[0] = cnt;
+    misc[4] = cnt;
+    misc[5] = misc[1];
+    misc[6] = misc[2];
+    misc[7] = misc[3];
+    misc[2] = 0;
+    misc[3] = 0;
+  }
 }
 
 // Frobenius norm, finiteness check, scaling exponent and zero-column
@@ -1636,9 +1665,8 @@ int32_t JacobiEngine::run_t(qtn_ctx* ctx, typename CType<R>::type* X,
   }
   int* host_flags = nullptr;
   QTN_TRY(ctx->pinned_buffer(4 * sizeof(int), (void**)&host_flags));
-  int sweep = 0;
-  bool converged = false;
-  for (; sweep < max_sweeps; ++sweep) {
+  // One sweep: every round, then the convergence bookkeeping.
+  auto enqueue_sweep = [&]() -> int32_t {
     for (int r = 0; r < rounds; ++r) {
       if (this->W == 64)
         QTN_TRY((launch_round<R, 64>(s, X, ldx, sX, Wm, sW, r, tol, n_inner,
@@ -1652,9 +1680,48 @@ int32_t JacobiEngine::run_t(qtn_ctx* ctx, typename CType<R>::type* X,
     QTN_CHECK_HIP(hipGetLastError());
     hipLaunchKernelGGL(jac_sweep_end, dim3(1), dim3(256), 0, s, active,
                        counters, offmax, conv2, batch, misc);
-    QTN_CHECK_HIP(hipMemcpyAsync(host_flags, misc, 4 * sizeof(int),
+    QTN_CHECK_HIP(hipGetLastError());
+    return QTN_OK;
+  };
+  // The sweep's launches are captured once into a graph (kernel nodes
+  // only) and replayed; graphs are cached on the context by every value
+  // the launches take, so a later call with the same buffers and shapes
+  // replays without capturing.  Not while profiling (per-launch events).
+  hipGraphExec_t exec = nullptr;
+  if (!ctx->prof && sweep_graphs_enabled()) {
+    SweepKey key;
+    std::memset(&key, 0, sizeof(key));
+    key.dbl = dbl ? 1 : 0;
+    key.w = this->W;
+    key.fused = fused ? 1 : 0;
+    key.cross = cross_rounds ? 1 : 0;
+    key.batch = batch;
+    key.p = p;
+    key.n_inner = n_inner;
+    key.nchunk_x = nchunk_x;
+    key.nchunk_w = nchunk_w;
+    key.per_problem_w = want_w_host != nullptr ? 1 : 0;
+    key.ldx = ldx;
+    key.sx = sX;
+    key.sw = sW;
+    key.ldw = ldw;
+    key.tol = tol;
+    key.conv2 = conv2;
+    const void* ptrs[] = {X, Wm, want_w, zthr, counters, active, offmax,
+                          misc, Gp, Q, flags};
+    static_assert(sizeof(ptrs) == sizeof(key.ptr), "SweepKey pointers");
+    std::memcpy(key.ptr, ptrs, sizeof(ptrs));
+    QTN_TRY(ctx->sweep_graph(&key, sizeof(key), s, enqueue_sweep, &exec));
+  }
+  int sweep = 0;
+  bool converged = false;
+  for (; sweep < max_sweeps; ++sweep) {
+    if (exec)
+      QTN_CHECK_HIP(hipGraphLaunch(exec, s));
+    else
+      QTN_TRY(enqueue_sweep());
+    QTN_CHECK_HIP(hipMemcpyAsync(host_flags, misc + 4, 4 * sizeof(int),
                                  hipMemcpyDeviceToHost, s));
-    QTN_CHECK_HIP(hipMemsetAsync(misc + 2, 0, 2 * sizeof(int), s));
     QTN_CHECK_HIP(hipStreamSynchronize(s));
     if (ctx->prof)
       QTN_TRY(account_sweep(n_act, host_flags[2], host_flags[3], nchunk_x,
