@@ -734,6 +734,11 @@ def main():
                          "faster on the circuit workloads)")
     ap.add_argument("--only-batched", action="store_true",
                     help="time the headline form only (for profiler runs)")
+    ap.add_argument("--chosen-only", action="store_true",
+                    help="run only chosen-form circuits (warm-up + steps, "
+                         "no FP64 reference): the launches a counter pass "
+                         "averages are then the ones the profiled pass "
+                         "counts")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU "
@@ -768,6 +773,22 @@ def main():
     n2q = C.count_2q(layers)
     pristine = DeviceMPS(N_QUBITS, CHIMAX, ctx)
     dev = DeviceMPS(N_QUBITS, CHIMAX, ctx)
+    if args.chosen_only:
+        for _ in range(args.warmup + args.steps):
+            run_circuit(dev, pristine, layers, batched=True, form=FORM_CHOSEN)
+        torch.cuda.synchronize()
+        ctx.profile(True)
+        run_circuit(dev, pristine, layers, batched=True, form=FORM_CHOSEN)
+        torch.cuda.synchronize()
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        print(json.dumps({"chosen_only": True, "kernels": {
+            k: {"launches": v["count"],
+                "avg_us": 1e3 * v["ms"] / max(1, v["count"]),
+                "algorithmic_bytes_per_launch":
+                    v["bytes"] / max(1, v["count"])}
+            for k, v in prof.items() if v["count"]}}))
+        return
     ref = DeviceMPS(N_QUBITS, CHIMAX, ctx)
 
     # plain form: one two-site update at a time (also the reference state)

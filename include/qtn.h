@@ -232,7 +232,8 @@ int32_t qtn_mps_bond_spectrum(const qtn_mps* mps, int32_t bond, double* S,
 int32_t qtn_mps_get_site(const qtn_mps* mps, int32_t site, double* out,
                          int64_t* dims);
 
-/* Single-qubit gate u (2x2, column-major) on `site` (1-based).             */
+/* Single-qubit gate u (2x2, column-major) on `site` (1-based); enqueued on
+ * the context stream without waiting (later calls on the context follow it).  */
 int32_t qtn_mps_apply_1q(qtn_mps* mps, int32_t site, const double* u);
 /* Two-qubit gate on sites (site, site+1): g is 4x4 column-major, row/col
  * index = s_site + 2*s_{site+1} (wire w <-> bit w-1, as apply_MPO wires

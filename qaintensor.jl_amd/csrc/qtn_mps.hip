@@ -167,10 +167,15 @@ __global__ void write_right(const GateDesc* __restrict__ gd,
   }
 }
 
+// A 2x2 complex matrix passed by value (column-major).
+struct Mat2 {
+  double2 u[4];
+};
+
 // Single-qubit gate on a site tensor in place: B[a, s', b] = u[s', s] B[a,s,b]
 __global__ void apply_1q_kernel(double2* __restrict__ B, int64_t chiL,
-                                int64_t chiR, const double2* __restrict__ u) {
-  const double2 u00 = u[0], u10 = u[1], u01 = u[2], u11 = u[3];
+                                int64_t chiR, Mat2 g) {
+  const double2 u00 = g.u[0], u10 = g.u[1], u01 = g.u[2], u11 = g.u[3];
   const int64_t total = chiL * chiR;
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
        idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
@@ -1777,16 +1782,14 @@ int32_t qtn_mps_apply_1q(qtn_mps* m, int32_t site, const double* u) try {
   QTN_ARG(site >= 1 && site <= m->n, "qtn_mps_apply_1q: site out of range");
   qtn_ctx* ctx = m->ctx;
   QTN_CHECK_HIP(hipSetDevice(ctx->device));
-  void* dg;
-  QTN_TRY(ctx->buffer(kSlotGates, 16 * sizeof(double2), &dg));
-  QTN_CHECK_HIP(hipMemcpyAsync(dg, u, 4 * sizeof(double2),
-                               hipMemcpyHostToDevice, ctx->stream));
+  // the gate travels in the kernel arguments: no upload, and no wait for
+  // the kernel (later work on the context's stream is ordered after it)
+  Mat2 g;
+  for (int i = 0; i < 4; ++i) g.u[i] = make_double2(u[2 * i], u[2 * i + 1]);
   const int64_t cl = m->dims[site - 1], cr = m->dims[site];
   hipLaunchKernelGGL(apply_1q_kernel, dim3(grid_for(cl * cr)), dim3(256), 0,
-                     ctx->stream, m->site[site - 1], cl, cr,
-                     (const double2*)dg);
+                     ctx->stream, m->site[site - 1], cl, cr, g);
   QTN_CHECK_HIP(hipGetLastError());
-  QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   return QTN_OK;
 } catch (...) {
   return qtn::abi_exception();
