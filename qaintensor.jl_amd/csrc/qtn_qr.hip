@@ -637,9 +637,11 @@ __global__ void qr_sum_parts(const double2* __restrict__ parts, int S,
 }
 
 // Slices of the row (K) dimension for W = V^H A (NB x N, K = rows): its
-// few output tiles alone would leave most CUs idle, so the rows are split
-// until about 512 workgroups run, each slice at least 128 rows long.
+// few output tiles alone would leave most CUs idle, so the rows (when
+// there are at least 1024) are split until about 512 workgroups run, each
+// slice at least 128 rows long.
 int split_k(int64_t maxn, int batch, int64_t maxk) {
+  if (maxk < 1024) return 1;  // short reductions: the extra launch costs more
   const int64_t tiles = std::max<int64_t>(1, ((maxn + 31) / 32) * batch);
   int64_t sk = std::max<int64_t>(1, 512 / tiles);
   sk = std::min<int64_t>(sk, std::max<int64_t>(1, maxk / 128));
@@ -688,11 +690,15 @@ int32_t BatchedQR::factor(qtn_ctx* ctx, double2* A, int64_t lda, int64_t sA,
   VTn = (double2*)p;
   const int64_t vstride = (int64_t)nkeep * vs;  // per problem
   auto vslot = [&](int k) { return keep_q ? k : (k & 1); };
-  // Look-ahead: the update of panel k is split into the next panel's
+  // Look-ahead (panels too tall for LDS, whose one-workgroup factorisation
+  // dominates): the update of panel k is split into the next panel's
   // columns (main stream, before panel k+1) and the rest (side stream,
-  // behind panel k+1).  Each part forms W = V^H A_part in `sk` row slices
+  // behind panel k+1).  Otherwise one update of every trailing column on
+  // the main stream.  Each part forms W = V^H A_part in `sk` row slices
   // (partials summed into its own W), then A_part -= (V T^H) W.
-  const int sk_a = split_k(NB, batch, maxr);
+  const bool lookahead = maxr > kLdsRows && maxc > 2 * NB;
+  const int sk_a = lookahead ? split_k(NB, batch, maxr)
+                             : split_k(maxc, batch, maxr);
   const int sk_b = split_k(maxc, batch, maxr);
   const int64_t wsz = (int64_t)NB * cols_p;
   QTN_TRY(ctx->buffer(slot_base + 4, (size_t)2 * batch * wsz * sizeof(double2),
@@ -727,10 +733,12 @@ int32_t BatchedQR::factor(qtn_ctx* ctx, double2* A, int64_t lda, int64_t sA,
       maxm[k] = std::max(maxm[k], mt);
       const int64_t ks_all = mt;
       for (int part = 0; part < 2; ++part) {
-        // part 0: columns c0 .. c0 + NB (the next panel); part 1: the rest
+        // part 0: columns c0 .. c0 + NB (the next panel), or every trailing
+        // column without look-ahead; part 1: the rest
         const int64_t cs = part == 0 ? c0 : c0 + NB;
-        const int64_t n = part == 0 ? std::min<int64_t>(NB, nt)
-                                    : std::max<int64_t>(0, nt - NB);
+        const int64_t n = !lookahead ? (part == 0 ? nt : 0)
+                          : part == 0 ? std::min<int64_t>(NB, nt)
+                                      : std::max<int64_t>(0, nt - NB);
         maxn[2 * k + part] = std::max(maxn[2 * k + part], n);
         const int sk = sks[part];
         GemmDesc<double>* gp = gk + (size_t)(part == 0 ? 0 : sk_a + 1) * batch;
@@ -774,7 +782,7 @@ int32_t BatchedQR::factor(qtn_ctx* ctx, double2* A, int64_t lda, int64_t sA,
                                g.size() * sizeof(GemmDesc<double>),
                                hipMemcpyHostToDevice, s));
   hipStream_t side = nullptr;
-  QTN_TRY(ctx->side_stream(&side));
+  if (lookahead) QTN_TRY(ctx->side_stream(&side));
   // side_ev[0]: panel k factored (main); side_ev[2 + (k & 1)]: the rest of
   // panel k's update done (side)
   hipEvent_t ev_panel = ctx->side_ev[0];
@@ -812,6 +820,10 @@ int32_t BatchedQR::factor(qtn_ctx* ctx, double2* A, int64_t lda, int64_t sA,
                          A, lda, sA, k, (const QrDims*)ddims, Vk, VThk, VTnk,
                          vstride, 2 * vs, keep_q ? 1 : 0);
     QTN_CHECK_HIP(hipGetLastError());
+    if (!lookahead) {
+      QTN_TRY(update(s, k, 0));
+      continue;
+    }
     // the rest of the update, behind the next panel
     QTN_CHECK_HIP(hipEventRecord(ev_panel, s));
     QTN_CHECK_HIP(hipStreamWaitEvent(side, ev_panel, 0));
@@ -822,7 +834,7 @@ int32_t BatchedQR::factor(qtn_ctx* ctx, double2* A, int64_t lda, int64_t sA,
     if (k >= 1) QTN_CHECK_HIP(hipStreamWaitEvent(s, ev_rest(k - 1), 0));
     QTN_TRY(update(s, k, 0));
   }
-  if (npanel >= 1)
+  if (lookahead && npanel >= 1)
     QTN_CHECK_HIP(hipStreamWaitEvent(s, ev_rest(npanel - 1), 0));
   // the host descriptor vector dies here: make sure the copy finished
   QTN_CHECK_HIP(hipStreamSynchronize(s));
