@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include <hip/hip_ext.h>
 
@@ -1023,6 +1024,268 @@ __global__ __launch_bounds__(256) void jac_round(
                          Qr, Qi);
 }
 
+// ---------------------------------------------------------------------------
+// The fused round with the block pair resident in LDS (complex FP32, at
+// most kResRows rows, W = 32).  X_IJ is read from HBM once into LDS
+// planes; its Gram (MFMA, the rows split over the four waves), the inner
+// sweeps and X_IJ Q (MFMA, written back over the tile) all work on that
+// copy, and X_IJ Q leaves in one coalesced pass (W_IJ Q follows through
+// the same tile when wanted).  One launch per round instead of three, X_IJ
+// crosses HBM once each way, and the only serial memory latencies are the
+// tile load and store.  The tile takes most of a CU's LDS (one workgroup
+// per CU).  grid (p, batch), 256 threads.
+constexpr int kResRows = 512;
+constexpr int kResLd = kResRows + 4;  // column stride of the tile planes
+
+// rows x 32 tile of a matrix (columns of the block pair I, J) -> LDS planes
+// Tr / Ti [col][row]; thread t moves rows 2t, 2t + 1 of every column as one
+// 16-byte access (rows, ld and the batch stride are multiples of 64), and
+// all 32 columns' loads are in flight together.
+__device__ __forceinline__ void res_load(const float2* __restrict__ X,
+                                         int64_t ldx, int rows, int I, int J,
+                                         float (*Tr)[kResLd],
+                                         float (*Ti)[kResLd]) {
+  const int r = 2 * threadIdx.x;
+  static_assert(kResRows <= 2 * 256, "two rows per thread and column");
+  if (r >= rows) return;
+  float4 v[32];
+#pragma unroll
+  for (int c = 0; c < 32; ++c)
+    v[c] = *reinterpret_cast<const float4*>(X + pair_col<32>(c, I, J) * ldx +
+                                            r);
+#pragma unroll
+  for (int c = 0; c < 32; ++c) {
+    *reinterpret_cast<float2*>(&Tr[c][r]) = make_float2(v[c].x, v[c].z);
+    *reinterpret_cast<float2*>(&Ti[c][r]) = make_float2(v[c].y, v[c].w);
+  }
+}
+
+// Y = T Q for the tile in LDS, stored straight from the MFMA results into
+// the rows x 32 tile of Y (global; it may be the matrix T was read from).
+// Wave w owns rows [w rows/4, (w+1) rows/4) in 16-row tiles.  The complex
+// product takes three real products (Gauss): with t = a + ib and
+// q = c + id, K1 = c (a + b), K2 = a (d - c), K3 = b (c + d), so that
+// Re = K1 - K3 and Im = K1 + K2; P0 / P1 / P2 hold c, d - c and c + d in
+// [k][n] planes.
+__device__ __forceinline__ void res_update(float2* __restrict__ Y,
+                                           int64_t ldy, int rows, int I,
+                                           int J, const float (*Tr)[kResLd],
+                                           const float (*Ti)[kResLd],
+                                           const float (*P0)[33],
+                                           const float (*P1)[33],
+                                           const float (*P2)[33]) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int per = rows / 4;
+  int64_t col[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+    col[j] = pair_col<32>(j * 16 + (lane & 15), I, J) * ldy;
+  for (int t0 = wave * per; t0 < (wave + 1) * per; t0 += 16) {
+    f4_t k1[2], k2[2], k3[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        k1[j][e] = 0.f;
+        k2[j][e] = 0.f;
+        k3[j][e] = 0.f;
+      }
+#pragma unroll
+    for (int ks = 0; ks < 32; ks += 4) {
+      const int kr = ks + (lane >> 4);
+      const float ar = Tr[kr][t0 + (lane & 15)];
+      const float ai = Ti[kr][t0 + (lane & 15)];
+      const float as = ar + ai;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int n = j * 16 + (lane & 15);
+        k1[j] = JT<float>::mfma(as, P0[kr][n], k1[j]);
+        k2[j] = JT<float>::mfma(ar, P1[kr][n], k2[j]);
+        k3[j] = JT<float>::mfma(ai, P2[kr][n], k3[j]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = t0 + JT<float>::row(lane, e);
+        Y[col[j] + r] = make_float2(k1[j][e] - k3[j][e], k1[j][e] + k2[j][e]);
+      }
+  }
+}
+
+__global__ __launch_bounds__(256) void jac_round_res(
+    float2* __restrict__ X, int64_t ldx, int64_t sX, int nchunk_x,
+    float2* __restrict__ Wm, int64_t ldw, int64_t sW, int nchunk_w,
+    const int* __restrict__ want, int p, int round, double tol,
+    const double* __restrict__ zthr, int n_inner, int* __restrict__ counters,
+    const int* __restrict__ active, double* __restrict__ offmax,
+    int* __restrict__ qpairs, int cross) {
+  constexpr int W = 32;
+  const int bz = blockIdx.y;
+  const int pi = blockIdx.x;
+  if (!active[bz]) return;
+  const int nb = 2 * p;
+  const int b1 = rr_block(pi, round, nb);
+  const int b2 = rr_block(nb - 1 - pi, round, nb);
+  const int I = min(b1, b2), J = max(b1, b2);
+  X += (int64_t)bz * sX;
+  const int rows = nchunk_x * 64;
+
+  __shared__ float Tr[W][kResLd];
+  __shared__ float Ti[W][kResLd];
+  __shared__ float2 G[W][W + 1];
+  __shared__ float2 Q[W][W + 1];
+  __shared__ int any_off;
+  __shared__ int nrot;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+
+  res_load(X, ldx, rows, I, J, Tr, Ti);
+  __syncthreads();
+
+  // 1. Gram, G[c1][c2] = sum_r conj(X[r][c1]) X[r][c2]: every wave takes the
+  // three (W/2)^2 quadrants UL, LR, UR over its quarter of the rows; the
+  // partials are added in the fixed order (w0 + w2) + (w1 + w3).  For the
+  // diagonal quadrants g[q][1] accumulates M = Xr^T Xi only (Im = M - M^T,
+  // formed below), three products per step instead of four.
+  f4_t g[3][2];
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      g[q][0][e] = 0.f;
+      g[q][1][e] = 0.f;
+    }
+  {
+    const int per = rows / 4;
+    for (int k0 = wave * per; k0 < (wave + 1) * per; k0 += 4) {
+      const int kr = k0 + (lane >> 4);
+      float xr[2], xi[2];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        xr[b] = Tr[b * 16 + (lane & 15)][kr];
+        xi[b] = Ti[b * 16 + (lane & 15)][kr];
+      }
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        g[q][0] = JT<float>::mfma(xr[q], xr[q], g[q][0]);
+        g[q][0] = JT<float>::mfma(xi[q], xi[q], g[q][0]);
+        g[q][1] = JT<float>::mfma(xr[q], xi[q], g[q][1]);
+      }
+      g[2][0] = JT<float>::mfma(xr[0], xr[1], g[2][0]);
+      g[2][0] = JT<float>::mfma(xi[0], xi[1], g[2][0]);
+      g[2][1] = JT<float>::mfma(xr[0], xi[1], g[2][1]);
+      g[2][1] = JT<float>::mfma(-xi[0], xr[1], g[2][1]);
+    }
+  }
+  float* sg = reinterpret_cast<float*>(&G[0][0]);  // 1536 floats per wave
+  float* sq = reinterpret_cast<float*>(&Q[0][0]);
+  static_assert(24 * 64 <= W * (W + 1) * 2, "a wave's partial fits");
+  auto put = [&](float* s) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) s[((q * 2 + c) * 4 + e) * 64 + lane] = g[q][c][e];
+  };
+  auto add = [&](const float* s) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) g[q][c][e] += s[((q * 2 + c) * 4 + e) * 64 + lane];
+  };
+  if (wave == 2) put(sg);
+  if (wave == 3) put(sq);
+  __syncthreads();
+  if (wave == 0) add(sg);
+  if (wave == 1) add(sq);
+  __syncthreads();
+  if (wave == 1) put(sq);
+  __syncthreads();
+  if (wave == 0) {
+    add(sq);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int wm = q == 1 ? 16 : 0, wn = q == 0 ? 0 : 16;
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        G[wm + JT<float>::row(lane, e)][wn + (lane & 15)] =
+            make_float2(g[q][0][e], g[q][1][e]);
+    }
+  }
+  __syncthreads();
+  // diagonal quadrants: Im[r][c] = M[r][c] - M[c][r], one thread per
+  // unordered pair {r, c} (it reads both before writing either); the
+  // lower-left quadrant is the adjoint of the upper-right one
+  for (int idx = tid; idx < W * W; idx += 256) {
+    const int r = idx % W, c = idx / W;
+    Q[r][c] = make_float2(r == c ? 1.f : 0.f, 0.f);
+    if ((r < W / 2) == (c < W / 2) && r <= c) {
+      const float m = G[r][c].y - G[c][r].y;
+      G[r][c].y = m;
+      G[c][r].y = -m;
+    } else if (r >= W / 2 && c < W / 2) {
+      const float2 u = G[c][r];
+      G[r][c] = make_float2(u.x, -u.y);
+    }
+  }
+  if (tid == 0) {
+    any_off = 0;
+    nrot = 0;
+  }
+  __syncthreads();
+
+  // 2. inner sweeps
+  const float zt = (float)zthr[bz];
+  const float tol2 = (float)(tol * tol);
+  const int total = inner_solve<float, W>(G, Q, &G[0][0], &Q[0][0], zt, tol2,
+                                          n_inner, &offmax[bz], any_off, nrot,
+                                          cross != 0);
+  if (total == 0) return;
+  const bool do_w = Wm && (!want || want[bz]);
+  if (tid == 0) {
+    atomicAdd(&counters[bz], total);
+    atomicAdd(qpairs, 1);
+    if (do_w) atomicAdd(qpairs + 1, 1);
+  }
+  // 3. X_IJ <- X_IJ Q (and W_IJ <- W_IJ Q); the planes of Q for the three
+  // products: P0 and P1 in G's storage, P2 in Q's once Q has been read
+  float (*P0)[W + 1] = reinterpret_cast<float (*)[W + 1]>(&G[0][0]);
+  float (*P1)[W + 1] = P0 + W;
+  float (*P2)[W + 1] = reinterpret_cast<float (*)[W + 1]>(&Q[0][0]);
+  constexpr int QPT = W * W / 256;
+  float2 qv[QPT];
+#pragma unroll
+  for (int u = 0; u < QPT; ++u) {
+    const int idx = tid + 256 * u;
+    qv[u] = Q[idx % W][idx / W];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < QPT; ++u) {
+    const int idx = tid + 256 * u;
+    const int r = idx % W, c = idx / W;
+    P0[r][c] = qv[u].x;
+    P1[r][c] = qv[u].y - qv[u].x;
+    P2[r][c] = qv[u].x + qv[u].y;
+  }
+  __syncthreads();
+  res_update(X, ldx, rows, I, J, Tr, Ti, P0, P1, P2);
+  if (do_w) {
+    float2* Wb = Wm + (int64_t)bz * sW;
+    const int wrows = nchunk_w * 64;
+    __syncthreads();  // every wave has read its rows of the X tile
+    res_load(Wb, ldw, wrows, I, J, Tr, Ti);
+    __syncthreads();
+    res_update(Wb, ldw, wrows, I, J, Tr, Ti, P0, P1, P2);
+  }
+}
+
 // After a sweep: a problem stays active only if it rotated something and
 // the largest scaled off-diagonal |x_p^H x_q|^2 / (|x_p|^2 |x_q|^2) met in
 // the sweep was above conv2.  Below it the sweep's own rotations leave the
@@ -1342,7 +1605,10 @@ int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
   // The fused round (one workgroup per block pair) leaves each 64-row
   // chunk's load latency exposed with only 1-2 workgroups per CU; the
   // three-launch round overlaps them across all chunks and is faster at
-  // chi = 256, so the fused round is opt-in (QTN_JACOBI_FUSED=1).
+  // chi = 256 in FP64.  FP32 problems of at most 512 rows fuse by default:
+  // their round keeps the whole block pair in LDS (jac_round_res), so only
+  // the tile load and store wait on memory.  QTN_JACOBI_FUSED=1 / 0 forces
+  // fused / three-launch rounds everywhere.
   const size_t csz = single ? sizeof(float2) : sizeof(double2);
   rows_p = rows_p_;
   cols_p = cols_p_;
@@ -1351,7 +1617,7 @@ int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
   // outer sweep grows linearly with W while the MFMA work does not change).
   // The context's tuned tiling for this column bucket (qtn_tune_jacobi)
   // replaces that default; QTN_JACOBI_W / QTN_JACOBI_FUSED override both.
-  fused = false;
+  fused = single && std::max(rows_p_, cols_p_) <= 512;
   int wdef = 32;
   const auto it = ctx->jac_tiling.find(
       {cols_p, single ? 1 : 0,
@@ -1472,6 +1738,16 @@ int32_t JacobiEngine::launch_round(hipStream_t s, void* Xv, int64_t ldx,
       hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, __VA_ARGS__);         \
   } while (0)
   if (fused && WW == 32) {
+    if constexpr (std::is_same<R, float>::value) {
+      if (res_round(nchunk_x, Wm ? nchunk_w : 0)) {
+        QTN_TIMED_LAUNCH(ev[0], ev[1], jac_round_res, dim3(p, batch), X, ldx,
+                         sX, nchunk_x, Wm, ldw ? ldw : cols_p, sW, nchunk_w,
+                         (const int*)(per_problem_w ? want_w : nullptr), p, r,
+                         tol, zthr, n_inner, counters, active, offmax,
+                         misc + 2, (cross_rounds && p > 1 && r > 0) ? 1 : 0);
+        return QTN_OK;
+      }
+    }
     QTN_TIMED_LAUNCH(ev[0], ev[1], (jac_round<R, 32>), dim3(p, batch), X, ldx,
                      sX, nchunk_x, Wm, ldw ? ldw : cols_p, sW, nchunk_w,
                      (const int*)(per_problem_w ? want_w : nullptr), p, r,
@@ -1508,9 +1784,12 @@ int32_t JacobiEngine::account_sweep(int n_active, int qpairs, int qpairs_w,
   if (fused && W == 32) {
     // jac_round (id 3): the Gram of every active pair plus X_IJ <- X_IJ Q
     // for every rotated pair; bytes: X_IJ read once for the Gram, read and
-    // written once more for the update
+    // written once more for the update.  jac_round_res reads X_IJ once and
+    // writes it once, and reads and writes W_IJ once for the pairs that
+    // rotate W.
     const double Wd = W, c16 = single ? 8.0 : 16.0;
     const double rows = (double)nchunk * 64.0;
+    const bool res = res_round(nchunk, nchunk_w);
     for (size_t i = 0; i + 5 < prof_events.size(); i += 6) {
       float t01;
       QTN_CHECK_HIP(hipEventElapsedTime(&t01, prof_events[i],
@@ -1521,7 +1800,12 @@ int32_t JacobiEngine::account_sweep(int n_active, int qpairs, int qpairs_w,
       ctx->prof_bytes[3] += (double)n_active * p * rows * Wd * c16;
     }
     ctx->prof_flops[3] += (double)qpairs * 8.0 * rows * Wd * Wd;
-    ctx->prof_bytes[3] += (double)qpairs * 2.0 * rows * Wd * c16;
+    ctx->prof_bytes[3] += (double)qpairs * (res ? 1.0 : 2.0) * rows * Wd * c16;
+    if (res) {
+      const double wrows = (double)nchunk_w * 64.0;
+      ctx->prof_flops[3] += (double)qpairs_w * 8.0 * wrows * Wd * Wd;
+      ctx->prof_bytes[3] += (double)qpairs_w * 2.0 * wrows * Wd * c16;
+    }
     for (auto e : prof_events) ctx->ev_pool.push_back(e);
     prof_events.clear();
     return QTN_OK;

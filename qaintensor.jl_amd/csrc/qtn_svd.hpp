@@ -25,6 +25,11 @@ struct JacobiEngine {
   bool single = false; // FP32 sweeps (complex float X, Gram and rotations)
   int slot_base = kSlotGram;  // first of the 5 context slots it uses
   bool fused = false;  // W = 32 rounds as one jac_round launch
+  // FP32 fused rounds whose X (and W) tiles fit the LDS-resident kernel
+  // (at most 512 rows) run jac_round_res; the other fused rounds jac_round
+  bool res_round(int nchunk_x, int nchunk_w) const {
+    return single && nchunk_x * 64 <= 512 && nchunk_w * 64 <= 512;
+  }
   // the inner sweeps of outer rounds 1.. rotate only the cross pairs of
   // their two column blocks (round 0 of every outer sweep rotates all)
   bool cross_rounds = false;
