@@ -80,7 +80,8 @@ def roofline(prof):
     counts per launch:
       jac_inner[_fp32]  -- HBM (partial Grams in, rotation block out)
       jac_gram / jac_update[_fp32] -- MFMA GEMMs (FP64 78.6 TF, FP32 157.3)
-      jac_round -- the fused round: the larger fraction names the bound"""
+      jac_round[_fp32] -- the fused round: the larger fraction names the
+      bound"""
     name = max(prof, key=lambda k: prof[k]["ms"])
     p = prof[name]
     n = max(1, p["count"])
@@ -98,7 +99,7 @@ def roofline(prof):
             "avg_launch_us": avg_s * 1e6, "launches": n}
     if name.startswith("jac_inner"):
         return hbm
-    if name == "jac_round":
+    if name.startswith("jac_round"):
         out = hbm if hbm["frac"] >= mfma["frac"] else mfma
         other = mfma if out is hbm else hbm
         out["other"] = {"bound": other["bound"],

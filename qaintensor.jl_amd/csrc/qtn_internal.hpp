@@ -153,8 +153,9 @@ struct qtn_ctx {
   int64_t n_rr_accepted = 0, n_rr_redone = 0;
   // per-kernel event timing (qtn_ctx_profile)
   // ids: 0 jac_inner, 1 jac_gram, 2 jac_update, 3 jac_round (FP64 sweeps),
-  // 4 jac_inner, 5 jac_gram, 6 jac_update (FP32 sweeps of the chosen form)
-  static constexpr int kProfKernels = 7;
+  // 4 jac_inner, 5 jac_gram, 6 jac_update, 7 jac_round (FP32 sweeps of the
+  // chosen form)
+  static constexpr int kProfKernels = 8;
   bool prof = false;
   int64_t prof_count[kProfKernels] = {};
   double prof_ms[kProfKernels] = {};

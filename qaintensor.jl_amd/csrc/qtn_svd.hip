@@ -1782,29 +1782,30 @@ int32_t JacobiEngine::account_sweep(int n_active, int qpairs, int qpairs_w,
                                     int nchunk, int nchunk_w) {
   qtn_ctx* ctx = ctx_;
   if (fused && W == 32) {
-    // jac_round (id 3): the Gram of every active pair plus X_IJ <- X_IJ Q
-    // for every rotated pair; bytes: X_IJ read once for the Gram, read and
-    // written once more for the update.  jac_round_res reads X_IJ once and
+    // jac_round (id 3 FP64, 7 FP32): the Gram of every active pair plus
+    // X_IJ <- X_IJ Q for every rotated pair; bytes: X_IJ read once for the
+    // Gram, read and written once more for the update.  jac_round_res reads X_IJ once and
     // writes it once, and reads and writes W_IJ once for the pairs that
     // rotate W.
     const double Wd = W, c16 = single ? 8.0 : 16.0;
     const double rows = (double)nchunk * 64.0;
     const bool res = res_round(nchunk, nchunk_w);
+    const int kr = single ? 7 : 3;
     for (size_t i = 0; i + 5 < prof_events.size(); i += 6) {
       float t01;
       QTN_CHECK_HIP(hipEventElapsedTime(&t01, prof_events[i],
                                         prof_events[i + 1]));
-      ctx->prof_count[3] += 1;
-      ctx->prof_ms[3] += t01;
-      ctx->prof_flops[3] += (double)n_active * p * 0.75 * 8.0 * rows * Wd * Wd;
-      ctx->prof_bytes[3] += (double)n_active * p * rows * Wd * c16;
+      ctx->prof_count[kr] += 1;
+      ctx->prof_ms[kr] += t01;
+      ctx->prof_flops[kr] += (double)n_active * p * 0.75 * 8.0 * rows * Wd * Wd;
+      ctx->prof_bytes[kr] += (double)n_active * p * rows * Wd * c16;
     }
-    ctx->prof_flops[3] += (double)qpairs * 8.0 * rows * Wd * Wd;
-    ctx->prof_bytes[3] += (double)qpairs * (res ? 1.0 : 2.0) * rows * Wd * c16;
+    ctx->prof_flops[kr] += (double)qpairs * 8.0 * rows * Wd * Wd;
+    ctx->prof_bytes[kr] += (double)qpairs * (res ? 1.0 : 2.0) * rows * Wd * c16;
     if (res) {
       const double wrows = (double)nchunk_w * 64.0;
-      ctx->prof_flops[3] += (double)qpairs_w * 8.0 * wrows * Wd * Wd;
-      ctx->prof_bytes[3] += (double)qpairs_w * 2.0 * wrows * Wd * c16;
+      ctx->prof_flops[kr] += (double)qpairs_w * 8.0 * wrows * Wd * Wd;
+      ctx->prof_bytes[kr] += (double)qpairs_w * 2.0 * wrows * Wd * c16;
     }
     for (auto e : prof_events) ctx->ev_pool.push_back(e);
     prof_events.clear();

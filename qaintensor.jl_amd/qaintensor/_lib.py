@@ -270,7 +270,7 @@ class Context:
 
     PROFILED_KERNELS = ("jac_inner", "jac_gram", "jac_update", "jac_round",
                         "jac_inner_fp32", "jac_gram_fp32",
-                        "jac_update_fp32")
+                        "jac_update_fp32", "jac_round_fp32")
 
     def profile(self, enable=True):
         """Enable (and clear) or disable per-kernel HIP-event timing."""

@@ -20,6 +20,13 @@ def upload(tensors, cap, ctx):
     return d
 
 
+def fp32_sweep_launches(ctx):
+    """Launches of the FP32 sweep kernels (three-launch inner solve or fused
+    round) since profiling was enabled."""
+    prof = ctx.profile_read()
+    return prof["jac_inner_fp32"]["count"] + prof["jac_round_fp32"]["count"]
+
+
 def dense_of(dev):
     return O.mps_dense([dev.site(j) for j in range(1, dev.n + 1)])
 
@@ -120,11 +127,11 @@ def test_apply_mpo_chosen_form_zipup(n, chi, cap):
         mpo = DeviceMPO(W, ctx)
         out64 = DeviceMPS(n, cap, ctx)
         nrm64 = dev.apply_mpo(mpo, 1e-8, cap, out64)
-        assert ctx.profile_read()["jac_inner_fp32"]["count"] == 0
+        assert fp32_sweep_launches(ctx) == 0
         ctx.profile(True)
         out32 = DeviceMPS(n, cap, ctx)
         nrm32 = dev.apply_mpo(mpo, 1e-8, cap, out32, form=FORM_CHOSEN)
-        assert ctx.profile_read()["jac_inner_fp32"]["count"] > 0
+        assert fp32_sweep_launches(ctx) > 0
         ctx.profile(False)
         assert list(out32.bond_dims()) == list(out64.bond_dims())
         assert max(out64.bond_dims()) == cap
@@ -253,7 +260,7 @@ def test_apply_mpo_chosen_form_falls_back_below_the_cap():
         ctx.profile(True)
         out32 = DeviceMPS(n, 3, ctx)
         nrm32 = dev.apply_mpo(mpo, 1e-10, 3, out32, form=FORM_CHOSEN)
-        assert ctx.profile_read()["jac_inner_fp32"]["count"] > 0
+        assert fp32_sweep_launches(ctx) > 0
         ctx.profile(False)
         assert nrm32 == nrm64
         assert list(out32.bond_dims()) == list(out64.bond_dims())
