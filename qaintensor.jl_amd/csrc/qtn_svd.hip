@@ -353,9 +353,12 @@ __device__ __forceinline__ int slot_next(int P, int side, bool cross) {
 // (FP32, a few one-instruction transcendentals) -- transforms in registers
 // and writes.  Gb / Qb (four planes each, 4 NP^2 entries) double-buffer the
 // planes: the round reads one buffer and writes the other, one barrier per
-// round; Gb == Gp (no second buffer): a barrier separates the reads from
-// the writes.  The planes of the first buffer live in G's and Q's own
-// storage (W (W+1) >= 4 NP^2 entries).  The diagonal block of a rotated
+// round.  Gb == Gp (no second G buffer): a barrier separates the reads from
+// the writes.  Qb == Qp with a second G buffer (Q written in place): a
+// thread's Q entries only move between the threads of its row pair, which
+// share its wave, and a wave issues its LDS reads before its writes, so the
+// round keeps its single barrier.  The planes of the first buffer live in
+// G's and Q's own storage (W (W+1) >= 4 NP^2 entries).  The diagonal block of a rotated
 // pair is set to diag(ap, dq) exactly.
 template <typename R, int W>
 __device__ __forceinline__ int inner_solve(
@@ -419,6 +422,8 @@ __device__ __forceinline__ int inner_solve(
   c_t* Gp = &G[0][0];  // planes, first buffer
   c_t* Qp = &Q[0][0];
   const bool dbuf = Gb != Gp;
+  // in-place Q planes rely on a row pair's threads sharing one wave
+  static_assert(NP <= 64 && 64 % NP == 0, "a row pair's threads in one wave");
   // row-major G -> planes (round-0 slots); Q planes = the identity
   {
     c_t v[NBT][4];
@@ -483,6 +488,8 @@ __device__ __forceinline__ int inner_solve(
         }
       }
       if (!dbuf) __syncthreads();  // every read precedes every write
+      // (in-place Q: this wave's reads precede its writes)
+      else if (Qn == Qc) __builtin_amdgcn_wave_barrier();
       // --- rotations
       const Rot<R> jc = make_rot<R>(ga, gd, gg, zt, tol2);
       Rot<R> jr[NBT];
@@ -1137,6 +1144,7 @@ __global__ __launch_bounds__(256) void jac_round_res(
   __shared__ float Ti[W][kResLd];
   __shared__ float2 G[W][W + 1];
   __shared__ float2 Q[W][W + 1];
+  __shared__ float2 G2[W * W];  // second buffer of the G planes
   __shared__ int any_off;
   __shared__ int nrot;
   const int tid = threadIdx.x;
@@ -1243,7 +1251,7 @@ __global__ __launch_bounds__(256) void jac_round_res(
   // 2. inner sweeps
   const float zt = (float)zthr[bz];
   const float tol2 = (float)(tol * tol);
-  const int total = inner_solve<float, W>(G, Q, &G[0][0], &Q[0][0], zt, tol2,
+  const int total = inner_solve<float, W>(G, Q, G2, &Q[0][0], zt, tol2,
                                           n_inner, &offmax[bz], any_off, nrot,
                                           cross != 0);
   if (total == 0) return;
