@@ -87,7 +87,7 @@ int32_t qtn_ctx_counters(qtn_ctx* ctx, int64_t* out, int32_t n,
  * Kernel ids: 0 jac_inner, 1 jac_gram, 2 jac_update (the three-launch
  * block Jacobi round, FP64 sweeps), 3 jac_round (the opt-in fused round),
  * 4 jac_inner, 5 jac_gram, 6 jac_update, 7 jac_round (FP32 sweeps of the
- * chosen form; the fused round is their default up to 512 rows).
+ * chosen form; the fused round is their default from 320 to 512 rows).
  * Times come from start / stop events stamped by each dispatch.  qtn_ctx_profile(ctx, 1) enables and clears the accumulators.
  * qtn_ctx_profile_read returns, for one kernel id, the launch count, the
  * summed launch durations (ms), and the algorithmic HBM bytes and flops of

@@ -1613,10 +1613,13 @@ int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
   // The fused round (one workgroup per block pair) leaves each 64-row
   // chunk's load latency exposed with only 1-2 workgroups per CU; the
   // three-launch round overlaps them across all chunks and is faster at
-  // chi = 256 in FP64.  FP32 problems of at most 512 rows fuse by default:
+  // chi = 256 in FP64.  FP32 problems of 320 to 512 rows fuse by default:
   // their round keeps the whole block pair in LDS (jac_round_res), so only
-  // the tile load and store wait on memory.  QTN_JACOBI_FUSED=1 / 0 forces
-  // fused / three-launch rounds everywhere.
+  // the tile load and store wait on memory.  The tile's LDS allows one
+  // workgroup per CU, which costs throughput on large batches of smaller
+  // problems (a 256-circuit batch of chi <= 128 ran 8% slower fused), so
+  // those keep the three launches.  QTN_JACOBI_FUSED=1 / 0 forces fused /
+  // three-launch rounds everywhere.
   const size_t csz = single ? sizeof(float2) : sizeof(double2);
   rows_p = rows_p_;
   cols_p = cols_p_;
@@ -1625,7 +1628,8 @@ int32_t JacobiEngine::reserve(qtn_ctx* ctx, int batch_, int64_t rows_p_,
   // outer sweep grows linearly with W while the MFMA work does not change).
   // The context's tuned tiling for this column bucket (qtn_tune_jacobi)
   // replaces that default; QTN_JACOBI_W / QTN_JACOBI_FUSED override both.
-  fused = single && std::max(rows_p_, cols_p_) <= 512;
+  fused = single && std::max(rows_p_, cols_p_) > 256 &&
+          std::max(rows_p_, cols_p_) <= 512;
   int wdef = 32;
   const auto it = ctx->jac_tiling.find(
       {cols_p, single ? 1 : 0,
