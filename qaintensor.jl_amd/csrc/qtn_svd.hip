@@ -1069,11 +1069,13 @@ __device__ __forceinline__ void res_load(const float2* __restrict__ X,
 
 // Y = T Q for the tile in LDS, stored straight from the MFMA results into
 // the rows x 32 tile of Y (global; it may be the matrix T was read from).
-// Wave w owns rows [w rows/4, (w+1) rows/4) in 16-row tiles.  The complex
-// product takes three real products (Gauss): with t = a + ib and
-// q = c + id, K1 = c (a + b), K2 = a (d - c), K3 = b (c + d), so that
-// Re = K1 - K3 and Im = K1 + K2; P0 / P1 / P2 hold c, d - c and c + d in
-// [k][n] planes.
+// Wave w owns rows [w rows/4, (w+1) rows/4) in 16-row tiles.  The MFMAs
+// form Y^T = Q^T T^T, so each accumulator element holds one row of a
+// 16-row run and a store instruction writes 16 consecutive rows of four
+// columns (128 contiguous bytes each).  The complex product takes three
+// real products (Gauss): with t = a + ib and q = c + id, K1 = c (a + b),
+// K2 = a (d - c), K3 = b (c + d), so that Re = K1 - K3 and Im = K1 + K2;
+// P0 / P1 / P2 hold c, d - c and c + d in [k][n] planes.
 __device__ __forceinline__ void res_update(float2* __restrict__ Y,
                                            int64_t ldy, int rows, int I,
                                            int J, const float (*Tr)[kResLd],
@@ -1084,10 +1086,13 @@ __device__ __forceinline__ void res_update(float2* __restrict__ Y,
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int per = rows / 4;
-  int64_t col[2];
+  // element e of column tile j: column j 16 + 4 (lane / 16) + e
+  int64_t col[2][4];
 #pragma unroll
   for (int j = 0; j < 2; ++j)
-    col[j] = pair_col<32>(j * 16 + (lane & 15), I, J) * ldy;
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      col[j][e] = pair_col<32>(j * 16 + 4 * (lane >> 4) + e, I, J) * ldy;
   for (int t0 = wave * per; t0 < (wave + 1) * per; t0 += 16) {
     f4_t k1[2], k2[2], k3[2];
 #pragma unroll
@@ -1107,18 +1112,17 @@ __device__ __forceinline__ void res_update(float2* __restrict__ Y,
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         const int n = j * 16 + (lane & 15);
-        k1[j] = JT<float>::mfma(as, P0[kr][n], k1[j]);
-        k2[j] = JT<float>::mfma(ar, P1[kr][n], k2[j]);
-        k3[j] = JT<float>::mfma(ai, P2[kr][n], k3[j]);
+        k1[j] = JT<float>::mfma(P0[kr][n], as, k1[j]);
+        k2[j] = JT<float>::mfma(P1[kr][n], ar, k2[j]);
+        k3[j] = JT<float>::mfma(P2[kr][n], ai, k3[j]);
       }
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int r = t0 + JT<float>::row(lane, e);
-        Y[col[j] + r] = make_float2(k1[j][e] - k3[j][e], k1[j][e] + k2[j][e]);
-      }
+      for (int e = 0; e < 4; ++e)
+        Y[col[j][e] + t0 + (lane & 15)] =
+            make_float2(k1[j][e] - k3[j][e], k1[j][e] + k2[j][e]);
   }
 }
 
