@@ -772,26 +772,27 @@ __global__ __launch_bounds__(256) void jac_update(
       br[j] = Qr[kr][wn + j * 16 + (lane & 15)];
       bi[j] = Qi[kr][wn + j * 16 + (lane & 15)];
     }
+    // Y^T = Q^T X^T: the accumulators hold 16 consecutive rows per column,
+    // so each store instruction writes whole 16-row runs
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int j = 0; j < T; ++j) {
-        yr[i][j] = JT<R>::mfma(ar[i], br[j], yr[i][j]);
-        yr[i][j] = JT<R>::mfma(-ai[i], bi[j], yr[i][j]);
-        yi[i][j] = JT<R>::mfma(ar[i], bi[j], yi[i][j]);
-        yi[i][j] = JT<R>::mfma(ai[i], br[j], yi[i][j]);
+        yr[i][j] = JT<R>::mfma(br[j], ar[i], yr[i][j]);
+        yr[i][j] = JT<R>::mfma(bi[j], -ai[i], yr[i][j]);
+        yi[i][j] = JT<R>::mfma(bi[j], ar[i], yi[i][j]);
+        yi[i][j] = JT<R>::mfma(br[j], ai[i], yi[i][j]);
       }
   }
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < T; ++j) {
-      const int c = wn + j * 16 + (lane & 15);
-      const int64_t gc = pair_col<W>(c, I, J);
+      const int r = wm + i * 16 + (lane & 15);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int r = wm + i * 16 + JT<R>::row(lane, e);
-        X[(int64_t)chunk * 64 + r + gc * ldx] =
+        const int c = wn + j * 16 + JT<R>::row(lane, e);
+        X[(int64_t)chunk * 64 + r + pair_col<W>(c, I, J) * ldx] =
             JT<R>::mk(yr[i][j][e], yi[i][j][e]);
       }
     }
