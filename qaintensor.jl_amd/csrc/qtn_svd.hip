@@ -151,14 +151,17 @@ __device__ __forceinline__ int64_t pair_col(int c, int I, int J) {
 }
 
 // ---------------------------------------------------------------------------
-// 1. Partial Gram of a 64-row chunk of a block pair (W columns).
-// grid (rows_p/64, p, batch); Gp layout [batch][p][chunk][W*W] col-major.
-// Each of the 4 waves computes a (W/2) x (W/2) quadrant.
+// 1. Partial Gram of a block pair (W columns) over cpw consecutive 64-row
+// chunks (the last workgroup may have fewer).  grid (npart, p, batch),
+// npart = ceil(nchunk_x / cpw); Gp layout [batch][p][npart][W*W]
+// col-major.  Each of waves 0, 2, 3 accumulates one (W/2) x (W/2) quadrant
+// over the workgroup's chunks; tall problems take several chunks per
+// workgroup so that the inner solve sums at most 8 partials.
 template <typename R, int W>
 __global__ __launch_bounds__(256) void jac_gram(
     const typename JT<R>::c_t* __restrict__ X, int64_t ldx, int64_t sX,
-    int p, int round, typename JT<R>::c_t* __restrict__ Gp,
-    const int* __restrict__ active) {
+    int p, int round, int nchunk_x, int cpw,
+    typename JT<R>::c_t* __restrict__ Gp, const int* __restrict__ active) {
   using c_t = typename JT<R>::c_t;
   using acc_t = typename JT<R>::acc_t;
   constexpr int T = W / 32;  // 16x16 MFMA tiles per wave per dimension
@@ -166,8 +169,8 @@ __global__ __launch_bounds__(256) void jac_gram(
   if (!active[bz]) return;
   const int nb = 2 * p;
   const int pi = blockIdx.y;
-  const int chunk = blockIdx.x;
-  const int nchunk = gridDim.x;
+  const int part = blockIdx.x;
+  const int npart = gridDim.x;
   const int b1 = rr_block(pi, round, nb);
   const int b2 = rr_block(nb - 1 - pi, round, nb);
   const int I = min(b1, b2), J = max(b1, b2);
@@ -176,19 +179,9 @@ __global__ __launch_bounds__(256) void jac_gram(
   __shared__ R Xr[W][65];  // [local col][row]
   __shared__ R Xi[W][65];
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < 64 * W; idx += 256) {
-    const int r = idx & 63, c = idx >> 6;
-    const c_t v =
-        X[(int64_t)chunk * 64 + r + pair_col<W>(c, I, J) * ldx];
-    Xr[c][r] = v.x;
-    Xi[c][r] = v.y;
-  }
-  __syncthreads();
-
   const int lane = tid & 63, wave = tid >> 6;
   // G is Hermitian: the lower-left quadrant (wave 1) is conj(upper-right)^T
   // and is filled in by jac_inner.
-  if (wave == 1) return;
   const int wm = (wave & 1) * (W / 2), wn = (wave >> 1) * (W / 2);
   acc_t gr[T][T], gi[T][T];
 #pragma unroll
@@ -200,30 +193,44 @@ __global__ __launch_bounds__(256) void jac_gram(
         gr[i][j][e] = (R)0;
         gi[i][j][e] = (R)0;
       }
-  // G[c1][c2] = sum_r conj(X[r][c1]) X[r][c2]
-#pragma unroll 4
-  for (int ks = 0; ks < 64; ks += 4) {
-    const int kr = ks + (lane >> 4);
-    R ar[T], ai[T], br[T], bi[T];
-#pragma unroll
-    for (int i = 0; i < T; ++i) {
-      ar[i] = Xr[wm + i * 16 + (lane & 15)][kr];
-      ai[i] = -Xi[wm + i * 16 + (lane & 15)][kr];
-      br[i] = Xr[wn + i * 16 + (lane & 15)][kr];
-      bi[i] = Xi[wn + i * 16 + (lane & 15)][kr];
+  const int c0 = part * cpw, c1 = min(c0 + cpw, nchunk_x);
+  for (int chunk = c0; chunk < c1; ++chunk) {
+    if (chunk > c0) __syncthreads();  // the previous chunk's reads are done
+    for (int idx = tid; idx < 64 * W; idx += 256) {
+      const int r = idx & 63, c = idx >> 6;
+      const c_t v =
+          X[(int64_t)chunk * 64 + r + pair_col<W>(c, I, J) * ldx];
+      Xr[c][r] = v.x;
+      Xi[c][r] = v.y;
     }
+    __syncthreads();
+    if (wave == 1) continue;
+    // G[c1][c2] = sum_r conj(X[r][c1]) X[r][c2]
+#pragma unroll 4
+    for (int ks = 0; ks < 64; ks += 4) {
+      const int kr = ks + (lane >> 4);
+      R ar[T], ai[T], br[T], bi[T];
 #pragma unroll
-    for (int i = 0; i < T; ++i)
-#pragma unroll
-      for (int j = 0; j < T; ++j) {
-        gr[i][j] = JT<R>::mfma(ar[i], br[j], gr[i][j]);
-        gr[i][j] = JT<R>::mfma(-ai[i], bi[j], gr[i][j]);
-        gi[i][j] = JT<R>::mfma(ar[i], bi[j], gi[i][j]);
-        gi[i][j] = JT<R>::mfma(ai[i], br[j], gi[i][j]);
+      for (int i = 0; i < T; ++i) {
+        ar[i] = Xr[wm + i * 16 + (lane & 15)][kr];
+        ai[i] = -Xi[wm + i * 16 + (lane & 15)][kr];
+        br[i] = Xr[wn + i * 16 + (lane & 15)][kr];
+        bi[i] = Xi[wn + i * 16 + (lane & 15)][kr];
       }
+#pragma unroll
+      for (int i = 0; i < T; ++i)
+#pragma unroll
+        for (int j = 0; j < T; ++j) {
+          gr[i][j] = JT<R>::mfma(ar[i], br[j], gr[i][j]);
+          gr[i][j] = JT<R>::mfma(-ai[i], bi[j], gr[i][j]);
+          gi[i][j] = JT<R>::mfma(ar[i], bi[j], gi[i][j]);
+          gi[i][j] = JT<R>::mfma(ai[i], br[j], gi[i][j]);
+        }
+    }
   }
+  if (wave == 1) return;
   c_t* out =
-      Gp + (((int64_t)bz * p + pi) * nchunk + chunk) * (int64_t)(W * W);
+      Gp + (((int64_t)bz * p + pi) * npart + part) * (int64_t)(W * W);
 #pragma unroll
   for (int i = 0; i < T; ++i)
 #pragma unroll
@@ -235,6 +242,13 @@ __global__ __launch_bounds__(256) void jac_gram(
         out[row + W * col] = JT<R>::mk(gr[i][j][e], gi[i][j][e]);
       }
     }
+}
+
+// Chunks per jac_gram workgroup: at most 8 partial Grams per pair.
+inline int gram_cpw(int nchunk) { return nchunk > 8 ? (nchunk + 7) / 8 : 1; }
+inline int gram_parts(int nchunk) {
+  const int c = gram_cpw(nchunk);
+  return (nchunk + c - 1) / c;
 }
 
 // Rotation of the pair (p, q) from the Gram entries a = G_pp, d = G_qq and
@@ -1771,10 +1785,12 @@ int32_t JacobiEngine::launch_round(hipStream_t s, void* Xv, int64_t ldx,
                      tol, zthr, n_inner, counters, active, offmax, misc + 2);
     return QTN_OK;
   }
-  QTN_TIMED_LAUNCH(ev[0], ev[1], (jac_gram<R, WW>), dim3(nchunk_x, p, batch),
-                   (const c_t*)X, ldx, sX, p, r, G, active);
+  const int npart = gram_parts(nchunk_x);
+  QTN_TIMED_LAUNCH(ev[0], ev[1], (jac_gram<R, WW>), dim3(npart, p, batch),
+                   (const c_t*)X, ldx, sX, p, r, nchunk_x, gram_cpw(nchunk_x),
+                   G, active);
   QTN_TIMED_LAUNCH(ev[2], ev[3], (jac_inner<R, WW>), dim3(p, batch),
-                   (const c_t*)G, nchunk_x, p, tol, zthr, n_inner, Qc, flags,
+                   (const c_t*)G, npart, p, tol, zthr, n_inner, Qc, flags,
                    counters, active, offmax, misc + 2,
                    (cross_rounds && p > 1 && r > 0) ? 1 : 0,
                    (const int*)want_w, Wm ? (per_problem_w ? 2 : 1) : 0);
@@ -1790,7 +1806,8 @@ int32_t JacobiEngine::launch_round(hipStream_t s, void* Xv, int64_t ldx,
 // Adds the event times of the sweep just finished to the context's
 // per-kernel accumulators with the algorithmic work of its launches:
 //   jac_gram   flops 3/4 * 8*64*W^2 per (problem, pair, chunk); bytes: the
-//              64 x W X tile read + the 3/4 W^2 partial Gram written
+//              64 x W X tile read per chunk + the 3/4 W^2 partial Gram
+//              written per workgroup (gram_parts of them per pair)
 //   jac_inner  bytes: the partial Grams read + Q written per (problem, pair)
 //   jac_update flops 8*64*W^2 per (pair with a rotation, chunk of X, and
 //              chunk of W where the problem accumulates W); bytes: the tile
@@ -1831,9 +1848,13 @@ int32_t JacobiEngine::account_sweep(int n_active, int qpairs, int qpairs_w,
   const double Wd = W, c16 = single ? 8.0 : 16.0;
   const double rounds = 2.0 * p - 1.0;
   const int ki = single ? 4 : 0, kg = single ? 5 : 1, ku = single ? 6 : 2;
+  // jac_gram reads each 64-row chunk of X_IJ and writes one partial Gram
+  // per workgroup (npart of them); jac_inner reads those partials
+  const double npart = gram_parts(nchunk);
   const double gram_f = 0.75 * 8.0 * 64.0 * Wd * Wd;
-  const double gram_b = (64.0 * Wd + 0.75 * Wd * Wd) * c16;
-  const double inner_b = (nchunk * 0.75 * Wd * Wd + Wd * Wd) * c16;
+  const double gram_b =
+      (64.0 * Wd + 0.75 * Wd * Wd * npart / (double)nchunk) * c16;
+  const double inner_b = (npart * 0.75 * Wd * Wd + Wd * Wd) * c16;
   const double upd_f = 8.0 * 64.0 * Wd * Wd;
   const double upd_b = (2.0 * 64.0 * Wd + Wd * Wd) * c16;
   const double pairs_per_launch = (double)n_active * p;
