@@ -344,9 +344,28 @@ __device__ __forceinline__ int slot_next(int P, int side, bool cross) {
   return 2 * (P - 1) + 1;
 }
 
-// Check and n_inner cyclic sweeps on the Hermitian Gram block G (LDS,
-// row-major), accumulating the rotations in Q (the identity on entry,
-// row-major on return).  Returns the number of rotations (0 when the block
+// Plane positions (round-0 slots) of the Gram entry (r, c) and of the
+// rotation entry (r, c): the four planes are side x side in block order,
+// index s NP^2 + P NP + P' with s = 2 (row side) + (column side).  Gram
+// rows and columns both sit in slots (index i < NP: pair i, side 0; else
+// pair 2 NP - 1 - i, side 1); rotation rows are natural (row 2P + side)
+// and only its columns sit in slots.
+template <int NP>
+__device__ __forceinline__ int gpos(int r, int c) {
+  const int sr = r >= NP ? 1 : 0, sc = c >= NP ? 1 : 0;
+  const int pr = sr ? 2 * NP - 1 - r : r, pc = sc ? 2 * NP - 1 - c : c;
+  return (2 * sr + sc) * NP * NP + pr * NP + pc;
+}
+template <int NP>
+__device__ __forceinline__ int qpos(int r, int c) {
+  const int sc = c >= NP ? 1 : 0, pc = sc ? 2 * NP - 1 - c : c;
+  return (2 * (r & 1) + sc) * NP * NP + (r >> 1) * NP + pc;
+}
+
+// Check and n_inner cyclic sweeps on the Hermitian Gram block, given as
+// planes in round-0 slots in Gp (gpos), accumulating the rotations in
+// planes (the identity on entry, set here) whose final buffer is returned
+// in qout (qpos).  Returns the number of rotations (0 when the block
 // already meets the tolerance).  Called by all 256 threads; any_off / nrot
 // are zeroed __shared__ ints (visible after a barrier) and offmax_bz the
 // problem's sweep maximum.
@@ -376,10 +395,10 @@ __device__ __forceinline__ int slot_next(int P, int side, bool cross) {
 // pair is set to diag(ap, dq) exactly.
 template <typename R, int W>
 __device__ __forceinline__ int inner_solve(
-    typename JT<R>::c_t (*G)[W + 1], typename JT<R>::c_t (*Q)[W + 1],
+    typename JT<R>::c_t* Gp, typename JT<R>::c_t* Qp,
     typename JT<R>::c_t* Gb, typename JT<R>::c_t* Qb, R zt, R tol2,
-    int n_inner, double* offmax_bz, int& any_off, int& nrot,
-    bool cross = false) {
+    int n_inner, double* offmax_bz, int& any_off, int& nrot, bool cross,
+    typename JT<R>::c_t*& qout) {
   using c_t = typename JT<R>::c_t;
   constexpr int NP = W / 2;             // rotation pairs per round
   constexpr int NBLK = NP * NP;         // 2x2 blocks
@@ -387,8 +406,8 @@ __device__ __forceinline__ int inner_solve(
   constexpr int RS = 256 / NP;          // row-pair stride between them
   constexpr bool kShare = sizeof(R) == sizeof(double);
   static_assert(NBT >= 1 && NBT * 256 == NBLK, "W must be 32 or 64");
-  static_assert(W * (W + 1) >= 4 * NBLK, "planes must fit in G's storage");
   const int tid = threadIdx.x;
+  qout = Qp;
   // Largest scaled off-diagonal |G_pq|^2 / (G_pp G_qq) of the block; nothing
   // to do when it meets the tolerance (the common case in late sweeps).
   // Reduced per wave (shuffles), then across the 4 waves in LDS, so the
@@ -400,9 +419,9 @@ __device__ __forceinline__ int inner_solve(
   for (int idx = tid; idx < W * W; idx += 256) {
     const int r = idx % W, c = idx / W;
     if (r < c) {
-      const R a = G[r][r].x, d = G[c][c].x;
+      const R a = Gp[gpos<NP>(r, r)].x, d = Gp[gpos<NP>(c, c)].x;
       if (a > zt && d > zt) {
-        const c_t g = G[r][c];
+        const c_t g = Gp[gpos<NP>(r, c)];
         const R o2 = (g.x * g.x + g.y * g.y) / (a * d);
         omax = fmax(omax, (double)o2);
         // the sweep can only reduce the pairs it rotates
@@ -433,35 +452,21 @@ __device__ __forceinline__ int inner_solve(
   const int lane = tid & 63;
   const int pc_i = tid % NP;
   const int pr0 = tid / NP;
-  c_t* Gp = &G[0][0];  // planes, first buffer
-  c_t* Qp = &Q[0][0];
   const bool dbuf = Gb != Gp;
   // in-place Q planes rely on a row pair's threads sharing one wave
   static_assert(NP <= 64 && 64 % NP == 0, "a row pair's threads in one wave");
-  // row-major G -> planes (round-0 slots); Q planes = the identity
-  {
-    c_t v[NBT][4];
+  // Q planes = the identity (each thread its own blocks)
 #pragma unroll
-    for (int k = 0; k < NBT; ++k) {
-      const int P = pr0 + RS * k;
+  for (int k = 0; k < NBT; ++k) {
+    const int P = pr0 + RS * k;
+    const int blk = P * NP + pc_i;
 #pragma unroll
-      for (int s = 0; s < 4; ++s)
-        v[k][s] = G[slot_index0<NP>(P, s >> 1)][slot_index0<NP>(pc_i, s & 1)];
+    for (int s = 0; s < 4; ++s) {
+      const bool one = 2 * P + (s >> 1) == slot_index0<NP>(pc_i, s & 1);
+      Qp[s * NBLK + blk] = JT<R>::mk(one ? (R)1 : (R)0, (R)0);
     }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NBT; ++k) {
-      const int P = pr0 + RS * k;
-      const int blk = P * NP + pc_i;
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        Gp[s * NBLK + blk] = v[k][s];
-        const bool one = 2 * P + (s >> 1) == slot_index0<NP>(pc_i, s & 1);
-        Qp[s * NBLK + blk] = JT<R>::mk(one ? (R)1 : (R)0, (R)0);
-      }
-    }
-    __syncthreads();
   }
+  __syncthreads();
   // next-round slots of this thread's column sides and row sides (fixed)
   int dc[2], dr[NBT][2];
 #pragma unroll
@@ -566,25 +571,9 @@ __device__ __forceinline__ int inner_solve(
     if (tid == 0) any_off += total;  // running total (started at 1)
     __syncthreads();
   }
-  // Q planes (every index back in its round-0 slot) -> row-major Q
-  {
-    c_t v[NBT][4];
-#pragma unroll
-    for (int k = 0; k < NBT; ++k) {
-      const int blk = (pr0 + RS * k) * NP + pc_i;
-#pragma unroll
-      for (int s = 0; s < 4; ++s) v[k][s] = Qc[s * NBLK + blk];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NBT; ++k) {
-      const int P = pr0 + RS * k;
-#pragma unroll
-      for (int s = 0; s < 4; ++s)
-        Q[2 * P + (s >> 1)][slot_index0<NP>(pc_i, s & 1)] = v[k][s];
-    }
-  }
-  __syncthreads();
+  // every index is back in its round-0 slot; the last barrier of the loop
+  // made the final planes visible
+  qout = Qc;
   return any_off - 1;
 }
 
@@ -659,12 +648,15 @@ __global__ __launch_bounds__(256) void jac_inner(
         }
       }
   }
+  // G straight into the planes of the inner sweep (round-0 slots)
+  c_t* Gpl = &G[0][0];
+  c_t* Qpl = &Q[0][0];
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {
     const int idx = tid + 256 * i;
     const int r = idx % W, c = idx / W;
-    Q[r][c] = JT<R>::mk(r == c ? (R)1 : (R)0, (R)0);
-    if (!(r >= W / 2 && c < W / 2)) G[r][c] = JT<R>::mk(sr[i], si[i]);
+    if (!(r >= W / 2 && c < W / 2))
+      Gpl[gpos<W / 2>(r, c)] = JT<R>::mk(sr[i], si[i]);
   }
   if (tid == 0) {
     any_off = 0;
@@ -675,13 +667,14 @@ __global__ __launch_bounds__(256) void jac_inner(
   __syncthreads();
   for (int idx = tid; idx < (W / 2) * (W / 2); idx += 256) {
     const int r = W / 2 + idx % (W / 2), c = idx / (W / 2);
-    const c_t u = G[c][r];
-    G[r][c] = JT<R>::mk(u.x, -u.y);
+    const c_t u = Gpl[gpos<W / 2>(c, r)];
+    Gpl[gpos<W / 2>(r, c)] = JT<R>::mk(u.x, -u.y);
   }
   __syncthreads();
+  c_t* qf;
   const int total = inner_solve<R, W>(
-      G, Q, kDB ? G2 : &G[0][0], kDB ? Q2 : &Q[0][0], zt, tol2, n_inner,
-      &offmax[bz], any_off, nrot, cross != 0);
+      Gpl, Qpl, kDB ? G2 : Gpl, kDB ? Q2 : Qpl, zt, tol2, n_inner,
+      &offmax[bz], any_off, nrot, cross != 0, qf);
   if (total == 0) {
     if (tid == 0) flags[bz * p + pi] = 0;
     return;
@@ -699,7 +692,7 @@ __global__ __launch_bounds__(256) void jac_inner(
     c_t* q = Qout + ((int64_t)bz * p + pi) * (int64_t)(W * W);
     for (int idx = tid; idx < W * W; idx += 256) {
       const int r = idx % W, c = idx / W;
-      q[idx] = Q[r][c];
+      q[idx] = qf[qpos<W / 2>(r, c)];
     }
   }
 }
@@ -994,6 +987,8 @@ __global__ __launch_bounds__(256) void jac_round(
       }
     }
   }
+  // G straight into the planes of the inner sweep (round-0 slots)
+  c_t* Gpl = &G[0][0];
   if (wave != 1) {
 #pragma unroll
     for (int i = 0; i < T; ++i)
@@ -1003,13 +998,9 @@ __global__ __launch_bounds__(256) void jac_round(
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const int row = wm + i * 16 + JT<R>::row(lane, e);
-          G[row][col] = JT<R>::mk(gr[i][j][e], gi[i][j][e]);
+          Gpl[gpos<W / 2>(row, col)] = JT<R>::mk(gr[i][j][e], gi[i][j][e]);
         }
       }
-  }
-  for (int idx = tid; idx < W * W; idx += 256) {
-    const int r = idx % W, c = idx / W;
-    Q[r][c] = JT<R>::mk(r == c ? (R)1 : (R)0, (R)0);
   }
   if (tid == 0) {
     any_off = 0;
@@ -1018,16 +1009,18 @@ __global__ __launch_bounds__(256) void jac_round(
   __syncthreads();
   for (int idx = tid; idx < (W / 2) * (W / 2); idx += 256) {
     const int r = W / 2 + idx % (W / 2), c = idx / (W / 2);
-    const c_t u = G[c][r];
-    G[r][c] = JT<R>::mk(u.x, -u.y);
+    const c_t u = Gpl[gpos<W / 2>(c, r)];
+    Gpl[gpos<W / 2>(r, c)] = JT<R>::mk(u.x, -u.y);
   }
   __syncthreads();
 
   // 2. inner sweeps
   const R zt = (R)zthr[bz];
   const R tol2 = (R)(tol * tol);
-  const int total = inner_solve<R, W>(G, Q, &G[0][0], &Q[0][0], zt, tol2,
-                                      n_inner, &offmax[bz], any_off, nrot);
+  c_t* qf;
+  const int total = inner_solve<R, W>(Gpl, &Q[0][0], Gpl, &Q[0][0], zt,
+                                      tol2, n_inner, &offmax[bz], any_off,
+                                      nrot, false, qf);
   if (total == 0) return;
   if (tid == 0) {
     atomicAdd(&counters[bz], total);
@@ -1036,7 +1029,7 @@ __global__ __launch_bounds__(256) void jac_round(
   // 3. X_IJ <- X_IJ Q (and W_IJ <- W_IJ Q)
   for (int idx = tid; idx < W * W; idx += 256) {
     const int r = idx % W, c = idx / W;
-    const c_t q = Q[r][c];
+    const c_t q = qf[qpos<W / 2>(r, c)];
     Qr[r][c] = q.x;
     Qi[r][c] = q.y;
   }
@@ -1234,6 +1227,8 @@ __global__ __launch_bounds__(256) void jac_round_res(
   __syncthreads();
   if (wave == 1) put(sq);
   __syncthreads();
+  // G straight into the planes of the inner sweep (round-0 slots)
+  float2* Gpl = &G[0][0];
   if (wave == 0) {
     add(sq);
 #pragma unroll
@@ -1241,7 +1236,7 @@ __global__ __launch_bounds__(256) void jac_round_res(
       const int wm = q == 1 ? 16 : 0, wn = q == 0 ? 0 : 16;
 #pragma unroll
       for (int e = 0; e < 4; ++e)
-        G[wm + JT<float>::row(lane, e)][wn + (lane & 15)] =
+        Gpl[gpos<W / 2>(wm + JT<float>::row(lane, e), wn + (lane & 15))] =
             make_float2(g[q][0][e], g[q][1][e]);
     }
   }
@@ -1251,14 +1246,15 @@ __global__ __launch_bounds__(256) void jac_round_res(
   // lower-left quadrant is the adjoint of the upper-right one
   for (int idx = tid; idx < W * W; idx += 256) {
     const int r = idx % W, c = idx / W;
-    Q[r][c] = make_float2(r == c ? 1.f : 0.f, 0.f);
     if ((r < W / 2) == (c < W / 2) && r <= c) {
-      const float m = G[r][c].y - G[c][r].y;
-      G[r][c].y = m;
-      G[c][r].y = -m;
+      float2& grc = Gpl[gpos<W / 2>(r, c)];
+      float2& gcr = Gpl[gpos<W / 2>(c, r)];
+      const float m = grc.y - gcr.y;
+      grc.y = m;
+      gcr.y = -m;
     } else if (r >= W / 2 && c < W / 2) {
-      const float2 u = G[c][r];
-      G[r][c] = make_float2(u.x, -u.y);
+      const float2 u = Gpl[gpos<W / 2>(c, r)];
+      Gpl[gpos<W / 2>(r, c)] = make_float2(u.x, -u.y);
     }
   }
   if (tid == 0) {
@@ -1270,9 +1266,10 @@ __global__ __launch_bounds__(256) void jac_round_res(
   // 2. inner sweeps
   const float zt = (float)zthr[bz];
   const float tol2 = (float)(tol * tol);
-  const int total = inner_solve<float, W>(G, Q, G2, &Q[0][0], zt, tol2,
-                                          n_inner, &offmax[bz], any_off, nrot,
-                                          cross != 0);
+  float2* qf;
+  const int total = inner_solve<float, W>(Gpl, &Q[0][0], G2, &Q[0][0], zt,
+                                          tol2, n_inner, &offmax[bz],
+                                          any_off, nrot, cross != 0, qf);
   if (total == 0) return;
   const bool do_w = Wm && (!want || want[bz]);
   if (tid == 0) {
@@ -1290,7 +1287,7 @@ __global__ __launch_bounds__(256) void jac_round_res(
 #pragma unroll
   for (int u = 0; u < QPT; ++u) {
     const int idx = tid + 256 * u;
-    qv[u] = Q[idx % W][idx / W];
+    qv[u] = qf[qpos<W / 2>(idx % W, idx / W)];
   }
   __syncthreads();
 #pragma unroll
