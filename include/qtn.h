@@ -319,7 +319,9 @@ int32_t qtn_mps_apply_layer_multi(int32_t ngates, qtn_mps* const* states,
                                   int32_t form, int64_t* kept,
                                   double* disc_weight);
 /* Single-qubit gates us[8*i..] (2x2 column-major) on (states[i],
- * sites[i]), one launch.                                                   */
+ * sites[i]), one launch, enqueued on the states' context stream without a
+ * wait (the next call that reads or rewrites the states orders after it).
+ * The (state, site) pairs must be distinct.                                */
 int32_t qtn_mps_apply_1q_multi(int32_t ngates, qtn_mps* const* states,
                                const int32_t* sites, const double* us);
 

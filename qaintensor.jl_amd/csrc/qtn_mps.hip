@@ -13,6 +13,7 @@
 // Site-index conventions follow switch! (src/switch.jl:26-55): theta rows are
 // (a, s) with a fastest, columns (t, b) with t fastest, column-major.
 #include <algorithm>
+#include <set>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -2104,11 +2105,15 @@ int32_t qtn_mps_apply_1q_multi(int32_t ngates, qtn_mps* const* states,
   QTN_ARG(ctx != nullptr, "qtn_mps_apply_1q_multi: NULL state");
   std::vector<OneQDesc> dd(ngates);
   int64_t maxt = 1;
+  // one launch: a site may carry only one of its gates
+  std::set<std::pair<const qtn_mps*, int>> seen;
   for (int g = 0; g < ngates; ++g) {
     const qtn_mps* m = states[g];
     QTN_ARG(m && m->ctx == ctx, "qtn_mps_apply_1q_multi: bad state");
     const int site = sites[g];
     QTN_ARG(site >= 1 && site <= m->n, "qtn_mps_apply_1q_multi: bad site");
+    QTN_ARG(seen.insert({m, site}).second,
+            "qtn_mps_apply_1q_multi: two gates on one site");
     dd[g].B = m->site[site - 1];
     dd[g].chiL = m->dims[site - 1];
     dd[g].chiR = m->dims[site];
@@ -2125,7 +2130,8 @@ int32_t qtn_mps_apply_1q_multi(int32_t ngates, qtn_mps* const* states,
                      dim3(grid_for(maxt), (unsigned)ngates), dim3(256), 0,
                      ctx->stream, (const OneQDesc*)dptr);
   QTN_CHECK_HIP(hipGetLastError());
-  QTN_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  // enqueued without a wait: every later call that reads or rewrites these
+  // states (or this descriptor buffer) is ordered after it on the stream
   return QTN_OK;
 } catch (...) {
   return qtn::abi_exception();

@@ -58,6 +58,19 @@ class DeviceMPS:
             raise ValueError("single-qubit gate must be 2x2")
         L.check(L.load().qtn_mps_apply_1q(self.handle, int(site), L.dptr(u)))
 
+    def apply_1q_layer(self, singles):
+        """Single-qubit gates [(site, u)]: one launch when the sites are
+        distinct (qtn_mps_apply_1q_multi), else one call per gate in order
+        (gates on one site do not commute)."""
+        if not singles:
+            return
+        sites = [int(w) for w, _ in singles]
+        if len(set(sites)) == len(sites):
+            apply_1q_multi([self] * len(sites), sites, [u for _, u in singles])
+            return
+        for (w, u) in singles:
+            self.apply_1q(w, u)
+
     def apply_2q(self, site, g, cutoff, chimax, form=FORM_PLAIN,
                  want_spectrum=False):
         """Gate g (4x4, index s_site + 2 s_{site+1}) on (site, site+1).
@@ -149,8 +162,7 @@ class DeviceMPS:
         applies the two-qubit gates one at a time (the plain per-gate
         form)."""
         for singles, pairs in layers:
-            for (w, u) in singles:
-                self.apply_1q(w, u)
+            self.apply_1q_layer(singles)
             if not pairs:
                 continue
             if batched:

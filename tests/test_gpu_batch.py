@@ -58,6 +58,24 @@ def test_multi_rejects_overlap(gpu_ctx):
         apply_layer_multi([d, d], [2, 3], [g, g], [8, 8], 1e-10)
 
 
+def test_1q_multi_rejects_two_gates_on_one_site(gpu_ctx):
+    """One launch applies each site's gate once: a second gate on the same
+    (state, site) is refused before anything runs; DeviceMPS.run_layers
+    then applies such gates one by one, in order (X then H on wire 1 gives
+    H X |0> = (|0> - |1>) / sqrt(2))."""
+    from qaintensor.device import apply_1q_multi
+    d = DeviceMPS(3, 4, gpu_ctx)
+    x = np.array([[0, 1], [1, 0]], dtype=complex)
+    h = np.array([[1, 1], [1, -1]], dtype=complex) / np.sqrt(2.0)
+    with pytest.raises(QtnError, match="two gates on one site"):
+        apply_1q_multi([d, d], [1, 1], [x, h])
+    d.run_layers([([(1, x), (1, h)], [])], 1e-10, 4)
+    bits = np.array([[0, 0, 0], [1, 0, 0]], dtype=np.uint8)
+    amp = d.amplitudes(bits)
+    assert np.allclose(amp, [1 / np.sqrt(2.0), -1 / np.sqrt(2.0)],
+                       atol=1e-14)
+
+
 def test_jacobi_autotuner_table_and_results(gpu_ctx):
     """qtn_tune_jacobi times the tilings of a bucket and keeps the fastest;
     states run under each forced tiling agree with the default run to the
