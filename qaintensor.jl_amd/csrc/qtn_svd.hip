@@ -380,11 +380,9 @@ __device__ __forceinline__ int qpos(int r, int c) {
 // the diagonal block of pair P' at stride NP + 1 (no LDS bank conflicts);
 // after the update each entry is written straight to the slot its index
 // moves to, i.e. where its owner of the next round reads it.  Per round a
-// thread reads its blocks, the diagonal block of its column pair (and of
-// its row pair in FP32), derives J_P' -- J_P comes from the lane of its wave
-// that derived it as its own J_P' (FP64, a shuffle) or is derived too
-// (FP32, a few one-instruction transcendentals) -- transforms in registers
-// and writes.  Gb / Qb (four planes each, 4 NP^2 entries) double-buffer the
+// thread reads its blocks and the diagonal block of its column pair,
+// derives J_P' -- J_P comes from the lane of its wave that derived it as
+// its own J_P' (a shuffle) -- transforms in registers and writes.  Gb / Qb (four planes each, 4 NP^2 entries) double-buffer the
 // planes: the round reads one buffer and writes the other, one barrier per
 // round.  Gb == Gp (no second G buffer): a barrier separates the reads from
 // the writes.  Qb == Qp with a second G buffer (Q written in place): a
@@ -404,7 +402,6 @@ __device__ __forceinline__ int inner_solve(
   constexpr int NBLK = NP * NP;         // 2x2 blocks
   constexpr int NBT = NBLK / 256;       // blocks per thread
   constexpr int RS = 256 / NP;          // row-pair stride between them
-  constexpr bool kShare = sizeof(R) == sizeof(double);
   static_assert(NBT >= 1 && NBT * 256 == NBLK, "W must be 32 or 64");
   const int tid = threadIdx.x;
   qout = Qp;
@@ -489,8 +486,6 @@ __device__ __forceinline__ int inner_solve(
       const R ga = Gc[0 * NBLK + dcol].x, gd = Gc[3 * NBLK + dcol].x;
       const c_t gg = Gc[1 * NBLK + dcol];
       c_t b[NBT][4], q[NBT][4];
-      R ra[NBT], rd[NBT];
-      c_t rg[NBT];
 #pragma unroll
       for (int k = 0; k < NBT; ++k) {
         const int blk = (pr0 + RS * k) * NP + pc_i;
@@ -498,12 +493,6 @@ __device__ __forceinline__ int inner_solve(
         for (int s = 0; s < 4; ++s) {
           b[k][s] = Gc[s * NBLK + blk];
           q[k][s] = Qc[s * NBLK + blk];
-        }
-        if (!kShare) {
-          const int drow = (pr0 + RS * k) * (NP + 1);
-          ra[k] = Gc[0 * NBLK + drow].x;
-          rd[k] = Gc[3 * NBLK + drow].x;
-          rg[k] = Gc[1 * NBLK + drow];
         }
       }
       if (!dbuf) __syncthreads();  // every read precedes every write
@@ -514,16 +503,12 @@ __device__ __forceinline__ int inner_solve(
       Rot<R> jr[NBT];
 #pragma unroll
       for (int k = 0; k < NBT; ++k) {
-        if (kShare) {
-          // the lane of this wave whose column pair is this row pair
-          const int src = (lane & ~(NP - 1)) | (pr0 + RS * k);
-          jr[k].cs = __shfl(jc.cs, src, 64);
-          jr[k].sn = __shfl(jc.sn, src, 64);
-          jr[k].er = __shfl(jc.er, src, 64);
-          jr[k].ei = __shfl(jc.ei, src, 64);
-        } else {
-          jr[k] = make_rot<R>(ra[k], rd[k], rg[k], zt, tol2);
-        }
+        // the lane of this wave whose column pair is this row pair
+        const int src = (lane & ~(NP - 1)) | (pr0 + RS * k);
+        jr[k].cs = __shfl(jc.cs, src, 64);
+        jr[k].sn = __shfl(jc.sn, src, 64);
+        jr[k].er = __shfl(jc.er, src, 64);
+        jr[k].ei = __shfl(jc.ei, src, 64);
       }
       // --- transform, then write each entry to its next-round slot
 #pragma unroll
